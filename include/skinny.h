@@ -1,0 +1,144 @@
+/*
+ * skinny.h -- C ABI of the MI355X tall-and-skinny matrix product library.
+ *
+ * Every product entry point keeps the argument order and meaning of the
+ * reference operator boundary
+ *
+ *   using MatmulFunctionType = std::function<bool(
+ *       size_t blockCount, int M, int N, int K, const dtype* A, int lda,
+ *       dtype alpha, const dtype* B, int ldb, dtype beta, dtype* C, int ldc)>;
+ *                                              (versions.hpp:43-45)
+ *
+ * with 64-bit sizes, an explicit library handle and an explicit HIP stream
+ * appended.  All matrices are caller-owned device pointers; the calls are
+ * asynchronous on `stream` and never synchronise or allocate once the
+ * handle's workspace is large enough (see skinny_reserve).
+ *
+ *   TSMTTSM  C = alpha * A^T * B + beta * C
+ *            A: K x M row-major (lda >= M), B: K x N row-major (ldb >= N),
+ *            C: M x N column-major (ldc >= M).  Complex types use the plain
+ *            transpose (no conjugation).  A == B with lda == ldb, M == N is
+ *            the Gram (self) case and reads A once.
+ *   TSMM     C = alpha * A * B + beta * C
+ *            A: K x M row-major (lda >= M), B: M x N column-major (ldb >= M),
+ *            C: K x N row-major (ldc >= N).  C may alias A (in place) when
+ *            lda == ldc >= max(M, N); every row of A is read before the same
+ *            row of C is written.
+ *   beta == 0  C is not read (BLAS semantics).
+ *
+ * blockCount: number of workgroups along K (TSMTTSM: the split-K partial
+ * count; TSMM: the row-parallel grid).  0 selects the tuned value.  Results of
+ * TSMTTSM are bitwise reproducible for a fixed blockCount.
+ *
+ * Return values: SKINNY_OK, or SKINNY_UNSUPPORTED where the reference variant
+ * would return false (unsupported shape / aliasing), SKINNY_INVALID_ARG for
+ * arguments no variant accepts (negative sizes, ld too small), SKINNY_HIP_ERROR
+ * and SKINNY_OOM for runtime failures.  skinny_last_error() describes the most
+ * recent failure on the calling thread.
+ */
+#ifndef SKINNY_H
+#define SKINNY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct skinny_handle_s* skinny_handle_t;
+/* a hipStream_t; 0 is the null stream */
+typedef void* skinny_stream_t;
+
+typedef enum {
+  SKINNY_OK = 0,
+  SKINNY_UNSUPPORTED = 1,
+  SKINNY_INVALID_ARG = 2,
+  SKINNY_HIP_ERROR = 3,
+  SKINNY_OOM = 4
+} skinny_status_t;
+
+typedef struct {
+  float x, y;
+} skinny_float_complex;
+typedef struct {
+  double x, y;
+} skinny_double_complex;
+/* a double-double value s + t (the reference's PseudoQuad, PseudoQuad.cuh:5-8) */
+typedef struct {
+  double s, t;
+} skinny_dd;
+
+/* ---- library / handle ------------------------------------------------- */
+/* Replaces the reference's namespace-static scratch (d_temp_storage,
+ * genv3.cuh:82-98) and static cuBLAS handles (gen_cublas.cuh:121-131) with
+ * a per-handle workspace on the device that is current at creation. */
+skinny_status_t skinny_create(skinny_handle_t* handle);
+skinny_status_t skinny_destroy(skinny_handle_t handle);
+/* Grow the workspace to at least `bytes` now (allocation happens here, not in
+ * later calls); required before capturing calls into a hipGraph. */
+skinny_status_t skinny_reserve(skinny_handle_t handle, size_t bytes);
+/* Workspace bytes a TSMTTSM call of this shape needs (elem = 4, 8 or 16). */
+size_t skinny_tsmttsm_workspace(skinny_handle_t handle, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K,
+                                int elem_bytes);
+/* Process-wide handle used by the C++ drop-in shim (created on first use). */
+skinny_handle_t skinny_default_handle(void);
+const char* skinny_status_string(skinny_status_t s);
+const char* skinny_last_error(void);
+/* Compute units of the handle's device and the blockCount 0 resolves to. */
+int skinny_cu_count(skinny_handle_t handle);
+int skinny_tuned_blockcount_tsmttsm(skinny_handle_t handle, int64_t M,
+                                    int64_t N, int64_t K, int elem_bytes);
+int skinny_tuned_blockcount_tsmm(skinny_handle_t handle, int64_t M, int64_t N,
+                                 int64_t K, int elem_bytes);
+
+/* ---- TSMTTSM: C = alpha A^T B + beta C --------------------------------- */
+/* replaces GENV1..8/SPECSMALL::tsmttsm<T,...> and tsmttsm_cublas<T>
+ * (tsmttsm/genv3.cuh:85-112, tsmttsm/gen_cublas.cuh:124-160); types per
+ * types.hpp:9-57: s = FR, d = DR, c = FC, z = DC, h = HR */
+skinny_status_t skinny_dtsmttsm(skinny_handle_t h, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K,
+                                const double* A, int64_t lda, double alpha,
+                                const double* B, int64_t ldb, double beta,
+                                double* C, int64_t ldc, skinny_stream_t stream);
+skinny_status_t skinny_stsmttsm(skinny_handle_t h, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K,
+                                const float* A, int64_t lda, float alpha,
+                                const float* B, int64_t ldb, float beta,
+                                float* C, int64_t ldc, skinny_stream_t stream);
+
+/* ---- TSMM: C = alpha A B + beta C -------------------------------------- */
+/* replaces tsmm_fix1..4/fix_fb/fix_blend/var1/varip1..3/varip_blend/
+ * fix_ip_ghost/var_ip_ghost and tsmm_cublas<T> (tsmm/fix3.cuh:41-65,
+ * tsmm/varip3.cuh, tsmm/cublas.cuh:178-222) */
+skinny_status_t skinny_dtsmm(skinny_handle_t h, size_t blockCount, int64_t M,
+                             int64_t N, int64_t K, const double* A,
+                             int64_t lda, double alpha, const double* B,
+                             int64_t ldb, double beta, double* C, int64_t ldc,
+                             skinny_stream_t stream);
+skinny_status_t skinny_stsmm(skinny_handle_t h, size_t blockCount, int64_t M,
+                             int64_t N, int64_t K, const float* A, int64_t lda,
+                             float alpha, const float* B, int64_t ldb,
+                             float beta, float* C, int64_t ldc,
+                             skinny_stream_t stream);
+
+/* ---- device data generation (counter RNG, identical to the host's) ----- */
+/* out[i] = U[-1,1) on a 2^-52 grid from (seed, stream_id, offset + i); the
+ * same sequence perf.cu's initKernel role needs (perf.cu:36-42), but random */
+skinny_status_t skinny_fill_uniform(double* out, int64_t n, uint64_t seed,
+                                    uint64_t stream_id, int64_t offset,
+                                    skinny_stream_t stream);
+skinny_status_t skinny_fill_uniform_f(float* out, int64_t n, uint64_t seed,
+                                      uint64_t stream_id, int64_t offset,
+                                      skinny_stream_t stream);
+/* integers in [lo, hi] as doubles (the exact-arithmetic test data of
+ * test.cu:82-95) */
+skinny_status_t skinny_fill_int(double* out, int64_t n, uint64_t seed,
+                                uint64_t stream_id, int64_t offset, int lo,
+                                int hi, skinny_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SKINNY_H */

@@ -1,0 +1,271 @@
+"""skinny_matmul_amd: tall-and-skinny matrix products on AMD MI355X (gfx950).
+
+Python binding of the C ABI in ``include/skinny.h`` (libskinny.so, hand-written
+HIP kernels).  The callables mirror the reference operator boundary
+
+    MatmulFunctionType = std::function<bool(blockCount, M, N, K, A, lda,
+                                            alpha, B, ldb, beta, C, ldc)>
+                                                    (versions.hpp:43-45)
+
+and return ``True`` on success and ``False`` where the reference variant would
+return ``false`` (unsupported arguments, the SKIP of test.cu:151-155).  Other
+failures raise :class:`SkinnyError` (the reference aborts through GPU_ERROR,
+gpu_error.cuh:4-13).  Matrices are torch tensors on the GPU (any shape; only
+their storage and the explicit leading dimensions matter) or raw device
+addresses given as ``int``.
+
+There is no CPU fallback: if libskinny.so is missing every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Callable, List, Tuple
+
+__all__ = [
+    "SkinnyError", "lib", "library_path", "available", "Handle",
+    "tsmttsm", "tsmm", "dtsmttsm", "stsmttsm", "dtsmm", "stsmm",
+    "fill_uniform", "fill_uniform_f", "fill_int",
+    "get_enabled_tsmttsm_versions", "get_enabled_tsmm_versions",
+    "types", "flops_per_cell",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "lib", "libskinny.so")
+_lib = None
+
+OK, UNSUPPORTED, INVALID_ARG, HIP_ERROR, OOM = 0, 1, 2, 3, 4
+
+i64 = C.c_int64
+u64 = C.c_uint64
+sz = C.c_size_t
+vp = C.c_void_p
+dbl = C.c_double
+flt = C.c_float
+
+
+class SkinnyError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"skinny status {status}: {msg}")
+        self.status = status
+
+
+class _DComplex(C.Structure):
+    _fields_ = [("x", C.c_double), ("y", C.c_double)]
+
+
+class _FComplex(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float)]
+
+
+def library_path() -> str:
+    return _LIB_PATH
+
+
+def available() -> bool:
+    return os.path.exists(_LIB_PATH)
+
+
+_SIGS = {
+    "skinny_create": (C.c_int, [C.POINTER(vp)]),
+    "skinny_destroy": (C.c_int, [vp]),
+    "skinny_reserve": (C.c_int, [vp, sz]),
+    "skinny_tsmttsm_workspace": (sz, [vp, sz, i64, i64, i64, C.c_int]),
+    "skinny_default_handle": (vp, []),
+    "skinny_status_string": (C.c_char_p, [C.c_int]),
+    "skinny_last_error": (C.c_char_p, []),
+    "skinny_cu_count": (C.c_int, [vp]),
+    "skinny_tuned_blockcount_tsmttsm": (C.c_int, [vp, i64, i64, i64, C.c_int]),
+    "skinny_tuned_blockcount_tsmm": (C.c_int, [vp, i64, i64, i64, C.c_int]),
+    "skinny_dtsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, vp]),
+    "skinny_stsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, flt, vp, i64, flt, vp, i64, vp]),
+    "skinny_dtsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, vp]),
+    "skinny_stsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, flt, vp, i64, flt, vp, i64, vp]),
+    "skinny_fill_uniform": (C.c_int, [vp, i64, u64, u64, i64, vp]),
+    "skinny_fill_uniform_f": (C.c_int, [vp, i64, u64, u64, i64, vp]),
+    "skinny_fill_int": (C.c_int, [vp, i64, u64, u64, i64, C.c_int, C.c_int, vp]),
+}
+
+# every symbol include/skinny.h declares (checked by the CPU test suite)
+EXPORTED = tuple(_SIGS)
+
+
+def lib():
+    """Load libskinny.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise SkinnyError(HIP_ERROR, f"{_LIB_PATH} is not built: run "
+                              "`python -m skinny_matmul_amd.build`")
+        L = C.CDLL(_LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _last_error() -> str:
+    return lib().skinny_last_error().decode(errors="replace")
+
+
+def _addr(x) -> int:
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        if getattr(x, "device", None) is not None and x.device.type != "cuda":
+            raise SkinnyError(INVALID_ARG, "matrices must live on the GPU")
+        return int(x.data_ptr())
+    raise TypeError(f"cannot take a device address of {type(x)!r}")
+
+
+def _stream(stream) -> int:
+    if stream is None:
+        import torch
+        return int(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)
+
+
+def _check(status: int, allow_unsupported: bool = True) -> bool:
+    if status == OK:
+        return True
+    if status == UNSUPPORTED and allow_unsupported:
+        return False
+    raise SkinnyError(status, _last_error())
+
+
+class Handle:
+    """A library handle owning its device workspace (skinny_create)."""
+
+    def __init__(self):
+        h = vp()
+        _check(lib().skinny_create(C.byref(h)), False)
+        self._h = h.value
+
+    @property
+    def value(self) -> int:
+        return self._h
+
+    def reserve(self, nbytes: int) -> None:
+        _check(lib().skinny_reserve(self._h, nbytes), False)
+
+    def close(self) -> None:
+        if self._h:
+            lib().skinny_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _h(handle) -> int:
+    if handle is None:
+        return 0
+    return handle.value if isinstance(handle, Handle) else int(handle)
+
+
+# ---------------------------------------------------------------------------
+# typed entry points (blockCount, M, N, K, A, lda, alpha, B, ldb, beta, C, ldc)
+# ---------------------------------------------------------------------------
+def dtsmttsm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
+             handle=None, stream=None) -> bool:
+    return _check(lib().skinny_dtsmttsm(_h(handle), blockCount, M, N, K, _addr(A), lda,
+                                        float(alpha), _addr(B), ldb, float(beta),
+                                        _addr(Cm), ldc, _stream(stream)))
+
+
+def stsmttsm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
+             handle=None, stream=None) -> bool:
+    return _check(lib().skinny_stsmttsm(_h(handle), blockCount, M, N, K, _addr(A), lda,
+                                        float(alpha), _addr(B), ldb, float(beta),
+                                        _addr(Cm), ldc, _stream(stream)))
+
+
+def dtsmm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
+          handle=None, stream=None) -> bool:
+    return _check(lib().skinny_dtsmm(_h(handle), blockCount, M, N, K, _addr(A), lda,
+                                     float(alpha), _addr(B), ldb, float(beta),
+                                     _addr(Cm), ldc, _stream(stream)))
+
+
+def stsmm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
+          handle=None, stream=None) -> bool:
+    return _check(lib().skinny_stsmm(_h(handle), blockCount, M, N, K, _addr(A), lda,
+                                     float(alpha), _addr(B), ldb, float(beta),
+                                     _addr(Cm), ldc, _stream(stream)))
+
+
+def _dtype_key(t) -> str:
+    import torch
+    return {torch.float64: "DR", torch.float32: "FR"}[t.dtype]
+
+
+def tsmttsm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
+            handle=None, stream=None) -> bool:
+    """C = alpha A^T B + beta C, the routine picked from C's dtype."""
+    fn = {"DR": dtsmttsm, "FR": stsmttsm}[_dtype_key(Cm)]
+    return fn(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc, handle, stream)
+
+
+def tsmm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
+         handle=None, stream=None) -> bool:
+    """C = alpha A B + beta C, the routine picked from C's dtype."""
+    fn = {"DR": dtsmm, "FR": stsmm}[_dtype_key(Cm)]
+    return fn(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc, handle, stream)
+
+
+# ---------------------------------------------------------------------------
+# registries (versions.hpp:47-177): the variants a test or perf sweep runs
+# ---------------------------------------------------------------------------
+types = ("FR", "DR")
+flops_per_cell = {"FC": 8, "DC": 8, "FR": 2, "DR": 2, "HR": 9}  # types.hpp:17-57
+
+MatmulFunction = Callable[..., bool]
+
+
+def get_enabled_tsmttsm_versions(type_key: str = "DR") -> List[Tuple[MatmulFunction, str]]:
+    table = {"DR": dtsmttsm, "FR": stsmttsm}
+    return [(table[type_key], "SKINNY")]
+
+
+def get_enabled_tsmm_versions(type_key: str = "DR") -> List[Tuple[MatmulFunction, str]]:
+    table = {"DR": dtsmm, "FR": stsmm}
+    return [(table[type_key], "SKINNY")]
+
+
+# ---------------------------------------------------------------------------
+# device data generation
+# ---------------------------------------------------------------------------
+def fill_uniform(out, n, seed, stream_id, offset=0, stream=None) -> None:
+    _check(lib().skinny_fill_uniform(_addr(out), n, seed, stream_id, offset,
+                                     _stream(stream)), False)
+
+
+def fill_uniform_f(out, n, seed, stream_id, offset=0, stream=None) -> None:
+    _check(lib().skinny_fill_uniform_f(_addr(out), n, seed, stream_id, offset,
+                                       _stream(stream)), False)
+
+
+def fill_int(out, n, seed, stream_id, lo=-2, hi=2, offset=0, stream=None) -> None:
+    _check(lib().skinny_fill_int(_addr(out), n, seed, stream_id, offset, lo, hi,
+                                 _stream(stream)), False)
+
+
+def cu_count(handle=None) -> int:
+    return int(lib().skinny_cu_count(_h(handle)))
+
+
+def tuned_blockcount_tsmttsm(M, N, K, elem_bytes=8, handle=None) -> int:
+    return int(lib().skinny_tuned_blockcount_tsmttsm(_h(handle), M, N, K, elem_bytes))
+
+
+def tuned_blockcount_tsmm(M, N, K, elem_bytes=8, handle=None) -> int:
+    return int(lib().skinny_tuned_blockcount_tsmm(_h(handle), M, N, K, elem_bytes))

@@ -1,0 +1,139 @@
+// extern "C" product entry points: argument validation, planning, dispatch.
+#include <algorithm>
+
+#include "internal.hpp"
+
+using namespace skinny;
+
+namespace {
+
+skinny_status_t check_common(const char* fn, int64_t M, int64_t N, int64_t K,
+                             const void* A, const void* B, const void* C) {
+  if (M < 0 || N < 0 || K < 0)
+    return fail(SKINNY_INVALID_ARG, std::string(fn) + ": negative size");
+  if (M > 0 && N > 0 && !C)
+    return fail(SKINNY_INVALID_ARG, std::string(fn) + ": null C");
+  if (M > 0 && N > 0 && K > 0 && (!A || !B))
+    return fail(SKINNY_INVALID_ARG, std::string(fn) + ": null A or B");
+  return SKINNY_OK;
+}
+
+template <typename T>
+skinny_status_t tsmttsm_real(const char* fn, skinny_handle_t h,
+                             size_t blockCount, int64_t M, int64_t N, int64_t K,
+                             const T* A, int64_t lda, T alpha, const T* B,
+                             int64_t ldb, T beta, T* C, int64_t ldc,
+                             skinny_stream_t stream) {
+  skinny_status_t s = check_common(fn, M, N, K, A, B, C);
+  if (s != SKINNY_OK) return s;
+  if (lda < std::max<int64_t>(1, M) || ldb < std::max<int64_t>(1, N) ||
+      ldc < std::max<int64_t>(1, M))
+    return fail(SKINNY_INVALID_ARG, std::string(fn) + ": leading dimension too small");
+  if (M == 0 || N == 0) return SKINNY_OK;
+  if (M > 1024 || N > 1024)
+    return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": M, N <= 1024 only");
+  h = resolve(h);
+  if (!h) return fail(SKINNY_HIP_ERROR, std::string(fn) + ": no device handle");
+  TsmttsmPlan p = plan_tsmttsm(h, blockCount, M, N, K, (int)sizeof(T), (int)sizeof(T));
+  return launch_tsmttsm_real<T>(h, p, M, N, K, A, lda, alpha, B, ldb, beta, C,
+                                ldc, (hipStream_t)stream);
+}
+
+template <typename T>
+skinny_status_t tsmm_real(const char* fn, skinny_handle_t h, size_t blockCount,
+                          int64_t M, int64_t N, int64_t K, const T* A,
+                          int64_t lda, T alpha, const T* B, int64_t ldb, T beta,
+                          T* C, int64_t ldc, skinny_stream_t stream) {
+  skinny_status_t s = check_common(fn, M, N, K, A, B, C);
+  if (s != SKINNY_OK) return s;
+  if (lda < std::max<int64_t>(1, M) || ldb < std::max<int64_t>(1, M) ||
+      ldc < std::max<int64_t>(1, N))
+    return fail(SKINNY_INVALID_ARG, std::string(fn) + ": leading dimension too small");
+  if (K == 0 || N == 0) return SKINNY_OK;
+  // in place: C aliases A row for row (test.cu:130-132, perf.cu:182)
+  const bool inplace = (const void*)A == (const void*)C;
+  if (inplace && lda != ldc)
+    return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": in place needs lda == ldc");
+  if (!inplace && K > 0 && M > 0) {
+    // any other overlap of A and C is not supported
+    const char* a0 = (const char*)A;
+    const char* a1 = a0 + sizeof(T) * ((K - 1) * lda + M);
+    const char* c0 = (const char*)C;
+    const char* c1 = c0 + sizeof(T) * ((K - 1) * ldc + N);
+    if (a0 < c1 && c0 < a1)
+      return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": partially overlapping A and C");
+  }
+  h = resolve(h);
+  if (!h) return fail(SKINNY_HIP_ERROR, std::string(fn) + ": no device handle");
+  if (M == 0) {
+    // C = beta * C: the M-loop is empty, the kernel handles it
+  }
+  return launch_tsmm_real<T>(h, blockCount, M, N, K, A, lda, alpha, B, ldb,
+                             beta, C, ldc, (hipStream_t)stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t skinny_tsmttsm_workspace(skinny_handle_t h, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K,
+                                int elem_bytes) {
+  h = resolve(h);
+  if (!h || M <= 0 || N <= 0) return 0;
+  TsmttsmPlan p = plan_tsmttsm(h, blockCount, M, N, K, elem_bytes, elem_bytes);
+  return p.scratch_bytes;
+}
+
+int skinny_tuned_blockcount_tsmttsm(skinny_handle_t h, int64_t M, int64_t N,
+                                    int64_t K, int elem_bytes) {
+  h = resolve(h);
+  if (!h || M <= 0 || N <= 0) return 0;
+  return plan_tsmttsm(h, 0, M, N, K, elem_bytes, elem_bytes).G;
+}
+
+int skinny_tuned_blockcount_tsmm(skinny_handle_t h, int64_t M, int64_t N,
+                                 int64_t K, int elem_bytes) {
+  h = resolve(h);
+  if (!h) return 0;
+  return tuned_blocks_tsmm(h, M, N, K, elem_bytes);
+}
+
+skinny_status_t skinny_dtsmttsm(skinny_handle_t h, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K,
+                                const double* A, int64_t lda, double alpha,
+                                const double* B, int64_t ldb, double beta,
+                                double* C, int64_t ldc,
+                                skinny_stream_t stream) {
+  return tsmttsm_real<double>("skinny_dtsmttsm", h, blockCount, M, N, K, A,
+                              lda, alpha, B, ldb, beta, C, ldc, stream);
+}
+
+skinny_status_t skinny_stsmttsm(skinny_handle_t h, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K,
+                                const float* A, int64_t lda, float alpha,
+                                const float* B, int64_t ldb, float beta,
+                                float* C, int64_t ldc, skinny_stream_t stream) {
+  return tsmttsm_real<float>("skinny_stsmttsm", h, blockCount, M, N, K, A, lda,
+                             alpha, B, ldb, beta, C, ldc, stream);
+}
+
+skinny_status_t skinny_dtsmm(skinny_handle_t h, size_t blockCount, int64_t M,
+                             int64_t N, int64_t K, const double* A,
+                             int64_t lda, double alpha, const double* B,
+                             int64_t ldb, double beta, double* C, int64_t ldc,
+                             skinny_stream_t stream) {
+  return tsmm_real<double>("skinny_dtsmm", h, blockCount, M, N, K, A, lda,
+                           alpha, B, ldb, beta, C, ldc, stream);
+}
+
+skinny_status_t skinny_stsmm(skinny_handle_t h, size_t blockCount, int64_t M,
+                             int64_t N, int64_t K, const float* A, int64_t lda,
+                             float alpha, const float* B, int64_t ldb,
+                             float beta, float* C, int64_t ldc,
+                             skinny_stream_t stream) {
+  return tsmm_real<float>("skinny_stsmm", h, blockCount, M, N, K, A, lda,
+                          alpha, B, ldb, beta, C, ldc, stream);
+}
+
+}  // extern "C"

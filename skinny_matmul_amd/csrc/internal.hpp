@@ -1,0 +1,81 @@
+// Internal declarations shared by the HIP translation units of libskinny.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+
+#include "skinny.h"
+
+struct skinny_handle_s {
+  int device = 0;
+  int cu_count = 256;
+  // Workspace: [counter region (zeroed at allocation, reset to zero by the
+  // kernels that use it)] [scratch region].
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+  std::mutex mu;
+};
+
+namespace skinny {
+
+// bytes at the start of the workspace reserved for arrival counters
+constexpr size_t kCounterBytes = 64 * 1024;
+constexpr int kMaxCounters = kCounterBytes / sizeof(unsigned int);
+
+void set_error(const std::string& msg);
+skinny_status_t fail(skinny_status_t s, const std::string& msg);
+skinny_status_t hip_fail(hipError_t e, const char* what);
+
+// Make sure the handle's scratch region holds `scratch_bytes` (grows the
+// allocation, synchronising the device, only when it is too small).
+skinny_status_t ensure_workspace(skinny_handle_t h, size_t scratch_bytes);
+
+inline unsigned int* ws_counters(skinny_handle_t h) {
+  return static_cast<unsigned int*>(h->ws);
+}
+inline void* ws_scratch(skinny_handle_t h) {
+  return static_cast<char*>(h->ws) + kCounterBytes;
+}
+
+inline skinny_handle_t resolve(skinny_handle_t h) {
+  return h ? h : skinny_default_handle();
+}
+
+// ---- TSMTTSM ------------------------------------------------------------
+struct TsmttsmPlan {
+  int P = 1;          // packed blocks per 16-lane MFMA tile (small widths)
+  int MT = 1, NT = 1; // 16-wide subtiles per wave tile
+  int tilesM = 1, tilesN = 1;
+  int tileM = 16, tileN = 16;  // C rows / cols per tile
+  int G = 1;          // workgroups along K per tile
+  int GS = 1;         // first-level reduction group size
+  int ngroups = 1;
+  int outM = 1, outN = 1;  // entries of one partial (outM x outN)
+  size_t scratch_bytes = 0;
+};
+
+TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
+                         int64_t N, int64_t K, int elem_bytes, int partial_bytes);
+
+template <typename T>
+skinny_status_t launch_tsmttsm_real(skinny_handle_t h, const TsmttsmPlan& p,
+                                    int64_t M, int64_t N, int64_t K,
+                                    const T* A, int64_t lda, T alpha,
+                                    const T* B, int64_t ldb, T beta, T* C,
+                                    int64_t ldc, hipStream_t stream);
+
+// ---- TSMM ---------------------------------------------------------------
+int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
+                      int elem_bytes);
+
+template <typename T>
+skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
+                                 int64_t M, int64_t N, int64_t K, const T* A,
+                                 int64_t lda, T alpha, const T* B, int64_t ldb,
+                                 T beta, T* C, int64_t ldc, hipStream_t stream);
+
+}  // namespace skinny

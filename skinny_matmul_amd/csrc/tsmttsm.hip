@@ -1,0 +1,480 @@
+// TSMTTSM  C = alpha * A^T * B + beta * C  on gfx950 matrix cores.
+//
+// Reference counterparts: the stage-1 blockProductKernel variants and the
+// stage-2 deviceReduce of tsmttsm/genv*.cuh and specsmall.cuh (e.g.
+// genv3.cuh:9-27 and 31-80, genv7.cuh:28-111), replaced by one kernel:
+//
+//  * The contraction index of A^T B is the row index k of the row-major
+//    K x M / K x N inputs.  One v_mfma_f64_16x16x4f64 consumes four rows:
+//    lane (i, g) = (lane % 16, lane / 16) supplies A(row_g, col_i) and
+//    B(row_g, col_i), so each 16-lane group reads one contiguous row segment
+//    and the matrix core forms the 16 x 16 outer products.  No LDS and no
+//    cross-lane shuffles are needed in the streaming loop.
+//  * Narrow blocks (max(M, N) <= 8) pack P = 16 / max(M, N) rows into the 16
+//    MFMA lanes (lane i = r * M + m); only the block-diagonal of the MFMA
+//    result is kept.  M = N = 1 therefore streams 64 rows per instruction.
+//  * Wider blocks use 16-wide subtiles, up to 4 x 4 of them (64 x 64) per wave;
+//    larger C is split into 64 x 64 tiles along the grid.
+//  * The four waves of a workgroup stream disjoint rows and are summed
+//    through LDS in a fixed pairwise order; each workgroup writes one partial.
+//    The last workgroup to arrive in a group of GS partials sums the group,
+//    and the last group to arrive sums the group partials and applies the
+//    alpha/beta epilogue (genv3.cuh:25-26).  Summation order is fixed, so the
+//    result is bitwise reproducible for a fixed blockCount.  Arrival counters
+//    are reset by their last user, so no memset is needed between calls.
+#include "internal.hpp"
+
+namespace skinny {
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T>
+struct TsArgs {
+  const T* A;
+  const T* B;
+  T* C;
+  int64_t K, lda, ldb, ldc;
+  int M, N;
+  T alpha, beta;
+  T* part1;      // [tile][G][PS]
+  T* part2;      // [tile][ngroups][PS]
+  unsigned* cnt; // [tile][ngroups] then [tile]
+  int G, GS, ngroups, tilesM, tilesN, outM, outN;
+};
+
+constexpr int kWaves = 4;
+
+template <int MT, int NT, int P>
+struct Unroll {
+  // independent loads kept in flight per lane: about 16
+  static constexpr int L = MT + NT;
+  static constexpr int value = P > 1 ? 8 : (L >= 8 ? 2 : (L >= 4 ? 4 : 8));
+};
+
+template <typename T>
+__device__ __forceinline__ T epilogue(T s, T alpha, T beta, const T* c) {
+#pragma clang fp contract(off)
+  T r = alpha * s;
+  if (beta != T(0)) r = beta * *c + r;
+  return r;
+}
+
+// Sum `count` partials of PS entries starting at src, in order, into dst or
+// (final) into C through the epilogue.
+template <typename T, int P>
+__device__ void reduce_partials(const TsArgs<T>& a, const T* src, int count,
+                                int PS, T* dst, bool final, int m0, int n0,
+                                int Mloc, int Nloc) {
+  for (int e = threadIdx.x; e < PS; e += blockDim.x) {
+    T s = T(0);
+    const T* p = src + e;
+    int q = 0;
+    for (; q + 4 <= count; q += 4) {
+      T v0 = p[(int64_t)(q + 0) * PS];
+      T v1 = p[(int64_t)(q + 1) * PS];
+      T v2 = p[(int64_t)(q + 2) * PS];
+      T v3 = p[(int64_t)(q + 3) * PS];
+      s += v0;
+      s += v1;
+      s += v2;
+      s += v3;
+    }
+    for (; q < count; q++) s += p[(int64_t)q * PS];
+    if (!final) {
+      dst[e] = s;
+    } else {
+      int ml = e % a.outM, nl = e / a.outM;
+      if (ml < Mloc && nl < Nloc) {
+        T* c = a.C + (int64_t)(n0 + nl) * a.ldc + (m0 + ml);
+        *c = epilogue(s, a.alpha, a.beta, c);
+      }
+    }
+  }
+}
+
+template <typename T, int P, int MT, int NT, bool SELF>
+__global__ void __launch_bounds__(256)
+    tsmttsm_kernel(TsArgs<T> a) {
+  using V4 = typename Mfma<T>::v4;
+  constexpr int U = Unroll<MT, NT, P>::value;
+  constexpr int RS = 4 * P;   // rows per MFMA step
+  constexpr int CH = RS * U;  // rows per chunk
+  constexpr int DM = 16 * MT, DN = 16 * NT;
+  constexpr int NACC = MT * NT * 4;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+
+  const int ntiles = a.tilesM * a.tilesN;
+  const int tile = blockIdx.x % ntiles;
+  const int kp = blockIdx.x / ntiles;
+  const int m0 = (tile % a.tilesM) * DM;
+  const int n0 = (tile / a.tilesM) * DN;
+  const int Mloc = min(a.M - m0, DM);
+  const int Nloc = min(a.N - n0, DN);
+
+  // lane -> (row within step, column) of its A and B operands
+  int rA = 0, cA = li, rB = 0, cB = li;
+  bool vA = true, vB = true;
+  if (P > 1) {
+    rA = li / a.M;
+    cA = li % a.M;
+    vA = rA < P;
+    rB = li / a.N;
+    cB = li % a.N;
+    vB = rB < P;
+  }
+  const int rowA = rA + P * lg;
+  const int rowB = rB + P * lg;
+  bool va[MT], vb[NT];
+#pragma unroll
+  for (int t = 0; t < MT; t++) va[t] = vA && (P > 1 || 16 * t + li < Mloc);
+#pragma unroll
+  for (int u = 0; u < NT; u++) vb[u] = vB && (P > 1 || 16 * u + li < Nloc);
+
+  const T* Ab = a.A + (int64_t)rowA * a.lda + m0 + cA;
+  const T* Bb = a.B + (int64_t)rowB * a.ldb + n0 + cB;
+  const int64_t stepA = (int64_t)RS * a.lda;
+  const int64_t stepB = (int64_t)RS * a.ldb;
+
+  V4 acc[MT][NT];
+#pragma unroll
+  for (int t = 0; t < MT; t++)
+#pragma unroll
+    for (int u = 0; u < NT; u++) acc[t][u] = V4{0, 0, 0, 0};
+
+  const int64_t K = a.K;
+  const int64_t nfull = K / CH;
+  const int64_t wstride = (int64_t)a.G * kWaves;
+  const int64_t w0 = (int64_t)kp * kWaves + wave;
+
+  for (int64_t c = w0; c < nfull; c += wstride) {
+    const T* pa = Ab + c * CH * a.lda;
+    const T* pb = Bb + c * CH * a.ldb;
+    T av[U][MT], bv[U][NT];
+#pragma unroll
+    for (int s = 0; s < U; s++) {
+#pragma unroll
+      for (int t = 0; t < MT; t++)
+        av[s][t] = va[t] ? pa[s * stepA + 16 * t] : T(0);
+      if (!SELF) {
+#pragma unroll
+        for (int u = 0; u < NT; u++)
+          bv[s][u] = vb[u] ? pb[s * stepB + 16 * u] : T(0);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < U; s++)
+#pragma unroll
+      for (int t = 0; t < MT; t++)
+#pragma unroll
+        for (int u = 0; u < NT; u++)
+          acc[t][u] = Mfma<T>::mac(av[s][t], SELF ? av[s][u] : bv[s][u],
+                                   acc[t][u]);
+  }
+
+  // ragged tail: the one partial chunk, taken by the wave whose turn it is
+  if (nfull * CH < K && (nfull % wstride) == w0) {
+    const int64_t r0 = nfull * CH;
+#pragma unroll
+    for (int s = 0; s < U; s++) {
+      const bool okA = r0 + s * RS + rowA < K;
+      const bool okB = r0 + s * RS + rowB < K;
+      T av[MT], bv[NT];
+#pragma unroll
+      for (int t = 0; t < MT; t++)
+        av[t] = (va[t] && okA) ? Ab[(r0 + s * RS) * a.lda + 16 * t] : T(0);
+#pragma unroll
+      for (int u = 0; u < NT; u++) {
+        if (SELF)
+          bv[u] = av[u];
+        else
+          bv[u] = (vb[u] && okB) ? Bb[(r0 + s * RS) * a.ldb + 16 * u] : T(0);
+      }
+#pragma unroll
+      for (int t = 0; t < MT; t++)
+#pragma unroll
+        for (int u = 0; u < NT; u++)
+          acc[t][u] = Mfma<T>::mac(av[t], bv[u], acc[t][u]);
+    }
+  }
+
+  // ---- workgroup reduction through LDS: ((w0 + w2) + (w1 + w3)) ----------
+  __shared__ T red[2][NACC][64];
+  if (wave >= 2) {
+#pragma unroll
+    for (int t = 0; t < MT; t++)
+#pragma unroll
+      for (int u = 0; u < NT; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          red[wave - 2][(t * NT + u) * 4 + v][lane] = acc[t][u][v];
+  }
+  __syncthreads();
+  if (wave < 2) {
+#pragma unroll
+    for (int t = 0; t < MT; t++)
+#pragma unroll
+      for (int u = 0; u < NT; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          acc[t][u][v] += red[wave][(t * NT + u) * 4 + v][lane];
+  }
+  __syncthreads();
+  if (wave == 1) {
+#pragma unroll
+    for (int t = 0; t < MT; t++)
+#pragma unroll
+      for (int u = 0; u < NT; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          red[0][(t * NT + u) * 4 + v][lane] = acc[t][u][v];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int t = 0; t < MT; t++)
+#pragma unroll
+      for (int u = 0; u < NT; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          acc[t][u][v] += red[0][(t * NT + u) * 4 + v][lane];
+  }
+  __syncthreads();
+  // D tile as a DM x DN row-major matrix (MFMA D layout: lane l, element v
+  // holds D[4 * (l / 16) + v][l % 16])
+  T* Dm = &red[1][0][0];
+  if (wave == 0) {
+#pragma unroll
+    for (int t = 0; t < MT; t++)
+#pragma unroll
+      for (int u = 0; u < NT; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          Dm[(16 * t + 4 * lg + v) * DN + 16 * u + li] = acc[t][u][v];
+  }
+  __syncthreads();
+
+  const int PS = a.outM * a.outN;
+  T* mine = a.part1 + ((int64_t)tile * a.G + kp) * PS;
+  for (int e = threadIdx.x; e < PS; e += blockDim.x) {
+    int ml = e % a.outM, nl = e / a.outM;
+    T s;
+    if (P > 1) {
+      s = T(0);
+#pragma unroll
+      for (int r = 0; r < P; r++) s += Dm[(r * a.M + ml) * DN + r * a.N + nl];
+    } else {
+      s = Dm[ml * DN + nl];
+    }
+    mine[e] = s;
+  }
+
+  // ---- grid reduction: last arrival per group, then last group ----------
+  __shared__ int s_last;
+  __threadfence();
+  __syncthreads();
+  const int grp = kp / a.GS;
+  const int gsz = min(a.GS, a.G - grp * a.GS);
+  unsigned* cnt1 = a.cnt + tile * a.ngroups + grp;
+  unsigned* cnt2 = a.cnt + ntiles * a.ngroups + tile;
+  if (threadIdx.x == 0) {
+    unsigned ticket = atomicAdd(cnt1, 1u);
+    s_last = (ticket == (unsigned)(gsz - 1));
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+
+  const T* gsrc = a.part1 + ((int64_t)tile * a.G + (int64_t)grp * a.GS) * PS;
+  if (a.ngroups == 1) {
+    reduce_partials<T, P>(a, gsrc, gsz, PS, nullptr, true, m0, n0, Mloc, Nloc);
+    if (threadIdx.x == 0) *cnt1 = 0u;
+    return;
+  }
+  T* gdst = a.part2 + ((int64_t)tile * a.ngroups + grp) * PS;
+  reduce_partials<T, P>(a, gsrc, gsz, PS, gdst, false, m0, n0, Mloc, Nloc);
+  if (threadIdx.x == 0) *cnt1 = 0u;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned ticket = atomicAdd(cnt2, 1u);
+    s_last = (ticket == (unsigned)(a.ngroups - 1));
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  reduce_partials<T, P>(a, a.part2 + (int64_t)tile * a.ngroups * PS,
+                        a.ngroups, PS, nullptr, true, m0, n0, Mloc, Nloc);
+  if (threadIdx.x == 0) *cnt2 = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// planning
+// ---------------------------------------------------------------------------
+static int isqrt_ceil(int x) {
+  int r = 1;
+  while (r * r < x) r++;
+  return r;
+}
+
+TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
+                         int64_t N, int64_t K, int elem_bytes,
+                         int partial_bytes) {
+  TsmttsmPlan p;
+  int64_t W = M > N ? M : N;
+  if (W <= 8) {
+    p.P = (int)(16 / (W > 0 ? W : 1));
+    p.MT = p.NT = 1;
+    p.tilesM = p.tilesN = 1;
+    p.tileM = 16;
+    p.tileN = 16;
+    p.outM = (int)M;
+    p.outN = (int)N;
+  } else {
+    p.P = 1;
+    int64_t mt = (M + 15) / 16, nt = (N + 15) / 16;
+    p.MT = (int)(mt < 4 ? mt : 4);
+    p.NT = (int)(nt < 4 ? nt : 4);
+    p.tileM = 16 * p.MT;
+    p.tileN = 16 * p.NT;
+    p.tilesM = (int)((M + p.tileM - 1) / p.tileM);
+    p.tilesN = (int)((N + p.tileN - 1) / p.tileN);
+    p.outM = p.tileM;
+    p.outN = p.tileN;
+  }
+  const int ntiles = p.tilesM * p.tilesN;
+  int U = p.P > 1 ? 8 : ((p.MT + p.NT) >= 8 ? 2 : ((p.MT + p.NT) >= 4 ? 4 : 8));
+  int64_t CH = 4LL * p.P * U;
+  int64_t G;
+  if (blockCount > 0) {
+    G = (int64_t)blockCount;
+  } else {
+    int64_t target = (2LL * h->cu_count) / ntiles;
+    if (target < 1) target = 1;
+    int64_t byrows = K / (CH * kWaves * 2);
+    if (byrows < 1) byrows = 1;
+    G = byrows < target ? byrows : target;
+  }
+  if (G > 65536) G = 65536;
+  p.G = (int)G;
+  if (p.G <= 16) {
+    p.GS = p.G;
+    p.ngroups = 1;
+  } else {
+    p.GS = isqrt_ceil(p.G);
+    p.ngroups = (p.G + p.GS - 1) / p.GS;
+  }
+  size_t PS = (size_t)p.outM * p.outN;
+  p.scratch_bytes = (size_t)ntiles * p.G * PS * partial_bytes;
+  if (p.ngroups > 1) p.scratch_bytes += (size_t)ntiles * p.ngroups * PS * partial_bytes;
+  p.scratch_bytes = (p.scratch_bytes + 255) & ~(size_t)255;
+  (void)elem_bytes;
+  return p;
+}
+
+template <typename T, int P, int MT, int NT, bool SELF>
+static void launch_one(const TsArgs<T>& a, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL((tsmttsm_kernel<T, P, MT, NT, SELF>), dim3(blocks),
+                     dim3(256), 0, s, a);
+}
+
+template <typename T, bool SELF>
+static bool dispatch(const TsmttsmPlan& p, const TsArgs<T>& a, int blocks,
+                     hipStream_t s) {
+  if (p.P > 1) {
+    switch (p.P) {
+      case 16: launch_one<T, 16, 1, 1, SELF>(a, blocks, s); return true;
+      case 8: launch_one<T, 8, 1, 1, SELF>(a, blocks, s); return true;
+      case 5: launch_one<T, 5, 1, 1, SELF>(a, blocks, s); return true;
+      case 4: launch_one<T, 4, 1, 1, SELF>(a, blocks, s); return true;
+      case 3: launch_one<T, 3, 1, 1, SELF>(a, blocks, s); return true;
+      case 2: launch_one<T, 2, 1, 1, SELF>(a, blocks, s); return true;
+      default: return false;
+    }
+  }
+#define SK_CASE(mt, nt)                                   \
+  case (mt)*8 + (nt):                                     \
+    launch_one<T, 1, mt, nt, SELF>(a, blocks, s);         \
+    return true;
+  switch (p.MT * 8 + p.NT) {
+    SK_CASE(1, 1) SK_CASE(1, 2) SK_CASE(1, 3) SK_CASE(1, 4)
+    SK_CASE(2, 1) SK_CASE(2, 2) SK_CASE(2, 3) SK_CASE(2, 4)
+    SK_CASE(3, 1) SK_CASE(3, 2) SK_CASE(3, 3) SK_CASE(3, 4)
+    SK_CASE(4, 1) SK_CASE(4, 2) SK_CASE(4, 3) SK_CASE(4, 4)
+    default: return false;
+  }
+#undef SK_CASE
+}
+
+template <typename T>
+skinny_status_t launch_tsmttsm_real(skinny_handle_t h, const TsmttsmPlan& p,
+                                    int64_t M, int64_t N, int64_t K,
+                                    const T* A, int64_t lda, T alpha,
+                                    const T* B, int64_t ldb, T beta, T* C,
+                                    int64_t ldc, hipStream_t stream) {
+  const int ntiles = p.tilesM * p.tilesN;
+  if ((size_t)ntiles * (p.ngroups + 1) > (size_t)kMaxCounters)
+    return fail(SKINNY_UNSUPPORTED, "tsmttsm: too many partial groups");
+  skinny_status_t st = ensure_workspace(h, p.scratch_bytes);
+  if (st != SKINNY_OK) return st;
+  TsArgs<T> a;
+  a.A = A;
+  a.B = B;
+  a.C = C;
+  a.K = K;
+  a.lda = lda;
+  a.ldb = ldb;
+  a.ldc = ldc;
+  a.M = (int)M;
+  a.N = (int)N;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.part1 = static_cast<T*>(ws_scratch(h));
+  a.part2 = a.part1 + (size_t)ntiles * p.G * p.outM * p.outN;
+  a.cnt = ws_counters(h);
+  a.G = p.G;
+  a.GS = p.GS;
+  a.ngroups = p.ngroups;
+  a.tilesM = p.tilesM;
+  a.tilesN = p.tilesN;
+  a.outM = p.outM;
+  a.outN = p.outN;
+  // the Gram shortcut (B operand = A operand) holds only when the one tile is
+  // on the diagonal; off-diagonal tiles of a large Gram read two column blocks
+  const bool self = (A == B) && (lda == ldb) && (M == N) && ntiles == 1;
+  const int blocks = p.G * ntiles;
+  bool ok = self ? dispatch<T, true>(p, a, blocks, stream)
+                 : dispatch<T, false>(p, a, blocks, stream);
+  if (!ok) return fail(SKINNY_UNSUPPORTED, "tsmttsm: no kernel for this shape");
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "tsmttsm launch");
+  return SKINNY_OK;
+}
+
+template skinny_status_t launch_tsmttsm_real<double>(
+    skinny_handle_t, const TsmttsmPlan&, int64_t, int64_t, int64_t,
+    const double*, int64_t, double, const double*, int64_t, double, double*,
+    int64_t, hipStream_t);
+template skinny_status_t launch_tsmttsm_real<float>(
+    skinny_handle_t, const TsmttsmPlan&, int64_t, int64_t, int64_t,
+    const float*, int64_t, float, const float*, int64_t, float, float*,
+    int64_t, hipStream_t);
+
+}  // namespace skinny

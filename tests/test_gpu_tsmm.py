@@ -1,0 +1,121 @@
+"""TSMM parity on the GPU: libskinny (C ABI) against the CPU oracle.
+
+Follows test.cu:245-293 for TSMM: integer data (bitwise comparison), padded
+leading dimensions, random blockCount, beta in {0, 1}, and the in-place case
+C == A with ldc = max(N + U[0,4], M) (test.cu:267) where the oracle reads the
+old A as tsmm/cublas.cuh:27-33 does.
+"""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+from gpu_util import EPS32, EPS64, dev, host, note, rel_fro
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _run(sk, bc, M, N, K, A, lda, alpha, B, ldb, beta, C, ldc, inplace=False):
+    dC = dev(C)
+    dA = dC if inplace else dev(A)
+    dB = dev(B)
+    ok = sk.tsmm(bc, M, N, K, dA, lda, alpha, dB, ldb, beta, dC, ldc)
+    return ok, host(dC)
+
+
+def test_golden_integer(gpu, orc):
+    with open(os.path.join(GOLD, "tsmm_int.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        for bc in (0, 1, 3, 200):
+            note(f"tsmm golden M={c['M']} N={c['N']} K={c['K']} inplace={c['inplace']} bc={bc}")
+            B = np.array(c["B"], dtype=np.float64)
+            C = np.array(c["C0"], dtype=np.float64)
+            A = C if c["inplace"] else np.array(c["A"], dtype=np.float64)
+            ok, got = _run(gpu, bc, c["M"], c["N"], c["K"], A, c["lda"], c["alpha"], B,
+                           c["ldb"], c["beta"], C, c["ldc"], c["inplace"])
+            assert ok
+            assert np.array_equal(got, np.array(c["C"], dtype=np.float64))
+
+
+def _fuzz_one(sk, orc, rng, M, N, dtype, Kmax, inplace=None):
+    if inplace is None:
+        inplace = rng.random() < 0.5
+    beta = float(rng.randint(0, 1))
+    ldb = M + rng.randint(0, 4)
+    if inplace:
+        ldc = max(N + rng.randint(0, 4), M)
+        lda = ldc
+    else:
+        lda = M + rng.randint(0, 4)
+        ldc = N + rng.randint(0, 4)
+    K = rng.randint(1, Kmax)
+    bc = rng.randint(1, 200)
+    seed = rng.randint(0, 1 << 30)
+    note(f"tsmm fuzz {np.dtype(dtype).name} M={M} N={N} K={K} lda={lda} ldb={ldb} "
+         f"ldc={ldc} bc={bc} inplace={inplace} beta={beta} seed={seed}")
+    B = orc.integers(N * ldb, seed, 1).astype(dtype)
+    C = orc.integers(K * ldc, seed, 2).astype(dtype)
+    A = C.copy() if inplace else orc.integers(K * lda, seed, 0).astype(dtype)
+    want = C.copy()
+    orc.tsmm(M, N, K, want if inplace else A, lda, 1.0, B, ldb, beta, want, ldc)
+    ok, got = _run(sk, bc, M, N, K, A, lda, 1.0, B, ldb, beta, C, ldc, inplace)
+    assert ok
+    assert np.array_equal(got, want), (M, N, K, lda, ldb, ldc, bc, inplace, beta)
+
+
+@pytest.mark.parametrize("M", list(range(1, 17)) + [20, 24, 31, 32, 33, 40, 48, 63, 64])
+def test_fuzz_square_fp64(gpu, orc, M):
+    rng = random.Random(2000 + M)
+    for _ in range(6):
+        _fuzz_one(gpu, orc, rng, M, M, np.float64, 30000)
+
+
+def test_fuzz_rectangular_fp64(gpu, orc):
+    rng = random.Random(11)
+    for _ in range(40):
+        _fuzz_one(gpu, orc, rng, rng.randint(1, 64), rng.randint(1, 64), np.float64, 20000)
+
+
+@pytest.mark.parametrize("M,N", [(1, 1), (4, 4), (8, 8), (5, 11), (16, 16), (32, 32), (64, 64)])
+def test_fuzz_fp32(gpu, orc, M, N):
+    rng = random.Random(M * 17 + N)
+    for _ in range(4):
+        _fuzz_one(gpu, orc, rng, M, N, np.float32, 30000)
+
+
+@pytest.mark.parametrize("M", [1, 2, 4, 8, 16, 32, 64])
+def test_uniform_fp64(gpu, orc, M):
+    K = 1 << 15
+    note(f"tsmm uniform M=N={M} K={K}")
+    A = orc.uniform(K * M, 5, 0)
+    B = orc.uniform(M * M, 5, 1)
+    C = orc.uniform(K * M, 5, 2)
+    want = C.copy()
+    orc.tsmm(M, M, K, A, M, 1.5, B, M, -0.5, want, M)
+    ok, got = _run(gpu, 0, M, M, K, A, M, 1.5, B, M, -0.5, C, M)
+    assert ok
+    assert rel_fro(got, want) <= 8 * M * EPS64
+
+
+def test_beta0_does_not_read_C(gpu, orc):
+    M, N, K = 5, 3, 700
+    A = orc.integers(K * M, 8, 0)
+    B = orc.integers(N * M, 8, 1)
+    want = np.zeros(K * N)
+    orc.tsmm(M, N, K, A, M, 1.0, B, M, 0.0, want, N)
+    ok, got = _run(gpu, 0, M, N, K, A, M, 1.0, B, M, 0.0, np.full(K * N, np.nan), N)
+    assert ok and np.array_equal(got, want)
+
+
+def test_unsupported_overlap_returns_false(gpu):
+    import torch
+    buf = torch.zeros(1000, dtype=torch.float64, device="cuda")
+    B = torch.zeros(16, dtype=torch.float64, device="cuda")
+    # A == C with lda != ldc: the reference variants return false (SKIP)
+    assert gpu.dtsmm(0, 4, 4, 10, buf, 8, 1.0, B, 4, 0.0, buf, 4) is False
+    # partially overlapping A and C
+    assert gpu.dtsmm(0, 4, 4, 10, buf, 4, 1.0, B, 4, 0.0, buf[2:], 4) is False

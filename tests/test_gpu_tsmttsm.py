@@ -1,0 +1,173 @@
+"""TSMTTSM parity on the GPU: libskinny (C ABI) against the CPU oracle.
+
+Follows test.cu:245-293: integer data in [-2, 2] (every sum exact, so the
+comparison is bitwise whatever the summation order), padded leading
+dimensions ld = dim + U[0, 4], blockCount ~ U[1, 200], self (A == B) and
+beta in {0, 1}.  Floating-point data is checked against the exact
+(superaccumulator) oracle within 8 * K * eps relative Frobenius error.
+"""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+from gpu_util import EPS32, EPS64, dev, host, note, rel_fro
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _run(sk, blockCount, M, N, K, A, lda, alpha, B, ldb, beta, C, ldc, self_=False):
+    dA = dev(A)
+    dB = dA if self_ else dev(B)
+    dC = dev(C)
+    ok = sk.tsmttsm(blockCount, M, N, K, dA, lda, alpha, dB, ldb, beta, dC, ldc)
+    return ok, host(dC)
+
+
+def test_golden_integer(gpu, orc):
+    with open(os.path.join(GOLD, "tsmttsm_int.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        for bc in (0, 1, 7, 200):
+            note(f"tsmttsm golden M={c['M']} N={c['N']} K={c['K']} bc={bc}")
+            A = np.array(c["A"], dtype=np.float64)
+            B = A if c["self"] else np.array(c["B"], dtype=np.float64)
+            C = np.array(c["C0"], dtype=np.float64)
+            ok, got = _run(gpu, bc, c["M"], c["N"], c["K"], A, c["lda"], c["alpha"], B,
+                           c["ldb"], c["beta"], C, c["ldc"], c["self"])
+            assert ok
+            assert np.array_equal(got, np.array(c["C"], dtype=np.float64)), (c["M"], c["N"], bc)
+
+
+def _fuzz_one(sk, orc, rng, M, N, dtype, Kmax):
+    self_ = rng.random() < 0.5 and M == N
+    beta = float(rng.randint(0, 1))
+    lda = M + rng.randint(0, 4)
+    ldb = lda if self_ else N + rng.randint(0, 4)
+    ldc = M + rng.randint(0, 4)
+    K = rng.randint(1, Kmax)
+    bc = rng.randint(1, 200)
+    seed = rng.randint(0, 1 << 30)
+    note(f"tsmttsm fuzz {np.dtype(dtype).name} M={M} N={N} K={K} lda={lda} ldb={ldb} "
+         f"ldc={ldc} bc={bc} self={self_} beta={beta} seed={seed}")
+    A = orc.integers(K * lda, seed, 0).astype(dtype)
+    B = A if self_ else orc.integers(K * ldb, seed, 1).astype(dtype)
+    C = orc.integers(N * ldc, seed, 2).astype(dtype)
+    want = C.copy()
+    orc.tsmttsm(M, N, K, A, lda, 1.0, B, ldb, beta, want, ldc)
+    ok, got = _run(sk, bc, M, N, K, A, lda, 1.0, B, ldb, beta, C, ldc, self_)
+    assert ok
+    assert np.array_equal(got, want), (M, N, K, lda, ldb, ldc, bc, self_, beta)
+
+
+@pytest.mark.parametrize("M", list(range(1, 17)) + [20, 24, 31, 32, 33, 40, 48, 63, 64])
+def test_fuzz_square_fp64(gpu, orc, M):
+    rng = random.Random(1000 + M)
+    for _ in range(6):
+        _fuzz_one(gpu, orc, rng, M, M, np.float64, 40000)
+
+
+def test_fuzz_rectangular_fp64(gpu, orc):
+    rng = random.Random(7)
+    for _ in range(40):
+        M = rng.randint(1, 64)
+        N = rng.randint(1, 64)
+        _fuzz_one(gpu, orc, rng, M, N, np.float64, 20000)
+
+
+@pytest.mark.parametrize("M,N", [(65, 65), (96, 96), (128, 128), (100, 7), (3, 130), (256, 256)])
+def test_fuzz_wide_fp64(gpu, orc, M, N):
+    rng = random.Random(M * 7 + N)
+    for _ in range(2):
+        _fuzz_one(gpu, orc, rng, M, N, np.float64, 3000)
+
+
+@pytest.mark.parametrize("M,N", [(1, 1), (4, 4), (8, 8), (5, 11), (16, 16), (32, 32), (64, 64)])
+def test_fuzz_fp32(gpu, orc, M, N):
+    rng = random.Random(M * 31 + N)
+    for _ in range(4):
+        _fuzz_one(gpu, orc, rng, M, N, np.float32, 30000)
+
+
+@pytest.mark.parametrize("M", [1, 2, 4, 8, 16, 32, 64])
+def test_uniform_fp64_vs_exact(gpu, orc, M):
+    """U[-1,1) data, K = 2^16: <= 8 K eps relative Frobenius vs the exact result."""
+    K = 1 << 16
+    note(f"tsmttsm uniform M=N={M} K={K}")
+    A = orc.uniform(K * M, 1, 0)
+    B = orc.uniform(K * M, 1, 1)
+    hi, lo = orc.xtsmttsm(M, M, K, A, M, B, M)
+    ok, got = _run(gpu, 0, M, M, K, A, M, 1.0, B, M, 0.0, np.zeros(M * M), M)
+    assert ok
+    err = rel_fro(got - lo, hi)
+    assert err <= 8 * K * EPS64, err
+    # fixed blockCount: bitwise reproducible
+    ok, again = _run(gpu, 0, M, M, K, A, M, 1.0, B, M, 0.0, np.zeros(M * M), M)
+    assert np.array_equal(got, again)
+
+
+def test_c1_config(gpu, orc):
+    """K = 2^16, M = N = 8, alpha = 1, beta = 0, seed 1 (streams 0 and 1)."""
+    M = N = 8
+    K = 1 << 16
+    A = orc.uniform(K * M, 1, 0)
+    B = orc.uniform(K * N, 1, 1)
+    want = np.zeros(M * N)
+    orc.tsmttsm(M, N, K, A, M, 1.0, B, N, 0.0, want, M)
+    hi, lo = orc.xtsmttsm(M, N, K, A, M, B, N)
+    ok, got = _run(gpu, 0, M, N, K, A, M, 1.0, B, N, 0.0, np.zeros(M * N), M)
+    assert ok
+    assert rel_fro(got, want) <= 8 * K * EPS64
+    assert rel_fro(got - lo, hi) <= 8 * K * EPS64
+
+
+def test_alpha_beta_and_nan_beta0(gpu, orc):
+    M, N, K = 6, 5, 1000
+    A = orc.integers(K * M, 4, 0)
+    B = orc.integers(K * N, 4, 1)
+    C = orc.integers(M * N, 4, 2)
+    want = C.copy()
+    orc.tsmttsm(M, N, K, A, M, -2.0, B, N, 3.0, want, M)
+    ok, got = _run(gpu, 0, M, N, K, A, M, -2.0, B, N, 3.0, C.copy(), M)
+    assert ok and np.array_equal(got, want)
+    # beta == 0: C is not read (NaN in C must not leak)
+    ok, got = _run(gpu, 0, M, N, K, A, M, 1.0, B, N, 0.0, np.full(M * N, np.nan), M)
+    want = np.zeros(M * N)
+    orc.tsmttsm(M, N, K, A, M, 1.0, B, N, 0.0, want, M)
+    assert ok and np.array_equal(got, want)
+
+
+def test_edge_sizes(gpu, orc):
+    # K = 0: C = beta * C;  K = 1;  M = N = 0 is a no-op
+    C = np.arange(9, dtype=np.float64)
+    ok, got = _run(gpu, 0, 3, 3, 0, np.zeros(3), 3, 1.0, np.zeros(3), 3, 2.0, C.copy(), 3)
+    assert ok and np.array_equal(got, 2.0 * C)
+    ok, got = _run(gpu, 5, 3, 3, 1, np.array([1.0, 2, 3]), 3, 1.0, np.array([1.0, 0, -1]), 3, 0.0, np.zeros(9), 3)
+    assert ok and np.array_equal(got, np.array([1.0, 2, 3, 0, 0, 0, -1, -2, -3]))
+    ok, _ = _run(gpu, 0, 0, 3, 10, np.zeros(30), 1, 1.0, np.zeros(30), 3, 0.0, np.zeros(3), 1)
+    assert ok
+
+
+def test_invalid_arguments_raise(gpu):
+    import torch
+    A = torch.zeros(100, dtype=torch.float64, device="cuda")
+    with pytest.raises(gpu.SkinnyError):
+        gpu.dtsmttsm(0, 4, 4, 10, A, 3, 1.0, A, 4, 0.0, A, 4)  # lda < M
+    with pytest.raises(gpu.SkinnyError):
+        gpu.dtsmttsm(0, -1, 4, 10, A, 4, 1.0, A, 4, 0.0, A, 4)
+
+
+def test_many_blockcounts_reproducible(gpu, orc):
+    M = N = 16
+    K = 100000
+    A = orc.uniform(K * M, 3, 0)
+    B = orc.uniform(K * N, 3, 1)
+    hi, lo = orc.xtsmttsm(M, N, K, A, M, B, N)
+    for bc in (1, 2, 17, 64, 255, 256, 1000, 4096):
+        note(f"tsmttsm blockCount={bc}")
+        ok, got = _run(gpu, bc, M, N, K, A, M, 1.0, B, N, 0.0, np.zeros(M * N), M)
+        assert ok
+        assert rel_fro(got - lo, hi) <= 8 * K * EPS64
