@@ -138,6 +138,18 @@ skinny_status_t skinny_fill_int(double* out, int64_t n, uint64_t seed,
                                 uint64_t stream_id, int64_t offset, int lo,
                                 int hi, skinny_stream_t stream);
 
+/* ---- microbenchmarks: this chip's own ceilings ------------------------ */
+/* HBM stream over arrays of `bytes_per_array` (allocated internally, far
+ * above the Infinity Cache when >= 1 GiB); kind 0 read, 1 copy, 2 triad,
+ * 3 write, 4 update (a += s b); best of `reps`, in GB/s of array traffic.
+ * Replaces microbenches/membw.cu:19-164 (copy/scale/update/sum/triad). */
+skinny_status_t skinny_mb_stream(int kind, size_t bytes_per_array, int reps,
+                                 double* gbs, skinny_stream_t stream);
+/* kind 0: FP64 vector FMA, 1: FP64 matrix core, 2: INT8 matrix core;
+ * TFLOP/s (TOP/s).  Replaces microbenches/roofline.cu:19-66. */
+skinny_status_t skinny_mb_peak(int kind, int reps, double* tflops,
+                               skinny_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

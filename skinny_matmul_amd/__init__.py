@@ -84,6 +84,8 @@ _SIGS = {
     "skinny_fill_uniform": (C.c_int, [vp, i64, u64, u64, i64, vp]),
     "skinny_fill_uniform_f": (C.c_int, [vp, i64, u64, u64, i64, vp]),
     "skinny_fill_int": (C.c_int, [vp, i64, u64, u64, i64, C.c_int, C.c_int, vp]),
+    "skinny_mb_stream": (C.c_int, [C.c_int, sz, C.c_int, C.POINTER(C.c_double), vp]),
+    "skinny_mb_peak": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double), vp]),
 }
 
 # every symbol include/skinny.h declares (checked by the CPU test suite)
@@ -257,6 +259,25 @@ def fill_uniform_f(out, n, seed, stream_id, offset=0, stream=None) -> None:
 def fill_int(out, n, seed, stream_id, lo=-2, hi=2, offset=0, stream=None) -> None:
     _check(lib().skinny_fill_int(_addr(out), n, seed, stream_id, offset, lo, hi,
                                  _stream(stream)), False)
+
+
+STREAM_KINDS = {"read": 0, "copy": 1, "triad": 2, "write": 3, "update": 4}
+PEAK_KINDS = {"fp64_fma": 0, "fp64_mfma": 1, "int8_mfma": 2}
+
+
+def mb_stream(kind: str, bytes_per_array: int = 4 << 30, reps: int = 5, stream=None) -> float:
+    """Measured HBM GB/s for one access mix (microbenches/membw.cu)."""
+    out = C.c_double()
+    _check(lib().skinny_mb_stream(STREAM_KINDS[kind], bytes_per_array, reps, C.byref(out),
+                                  _stream(stream)), False)
+    return out.value
+
+
+def mb_peak(kind: str, reps: int = 3, stream=None) -> float:
+    """Measured compute peak in TFLOP/s (TOP/s for int8)."""
+    out = C.c_double()
+    _check(lib().skinny_mb_peak(PEAK_KINDS[kind], reps, C.byref(out), _stream(stream)), False)
+    return out.value
 
 
 def cu_count(handle=None) -> int:

@@ -29,6 +29,7 @@ SOURCES = [
     "tsmttsm.hip",
     "tsmm.hip",
     "capi.hip",
+    "microbench.hip",
 ]
 
 CFLAGS = [

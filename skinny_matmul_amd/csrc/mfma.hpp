@@ -1,0 +1,34 @@
+// Matrix-core (MFMA) building blocks shared by the TSMTTSM and TSMM kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace skinny {
+
+// 16 x 16 x 4 MFMA, fp64 and fp32.  Operands: lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16].  Result element v of lane l is
+// D[drow(l / 16, v)][l % 16]; the fp64 form interleaves the rows (4 v + l/16),
+// the fp32 form blocks them (4 (l/16) + v).  Both mappings were measured on
+// gfx950 by the rectangular parity tests.
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int drow(int g, int v) { return 4 * v + g; }
+};
+
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int drow(int g, int v) { return 4 * g + v; }
+};
+
+}  // namespace skinny

@@ -18,8 +18,23 @@
 //    dimension and base pointer allow it.
 //  * The row length is rounded up to a compile-time class MC in
 //    {1, 2, 4, 8, 16, 32, 64} and masked with uniform branches; N is a runtime
-//    loop unrolled by four, so one instantiation serves a range of widths.
+//    loop unrolled by two, so one instantiation serves a range of widths.
+//
+// For 3 <= M <= 64 and N <= 64 the matrix cores take over (tsmm_mfma_kernel):
+//  * One v_mfma_f64_16x16x4f64 produces a 16-row x 16-column tile of C from
+//    four columns of A: lane (i, g) = (lane % 16, lane / 16) supplies
+//    A(row0 + i, m_g) and B(m_g, n0 + i).
+//  * Lane (i, g) owns the contiguous slice m = g*S .. g*S + S - 1 of row
+//    row0 + i (S = ceil(M / 4)), read with 16-byte loads; MFMA step s uses
+//    m_g = g*S + s.  The 16 rows x M columns of a block are one contiguous
+//    region when lda == M.
+//  * B is the same for every row: each lane keeps its S x NT operand values
+//    in registers for the whole kernel.
+//  * A wave owns whole 16-row blocks: every A load of a block completes before
+//    the MFMAs that consume it, and C of the block is stored after them, so C
+//    may alias A.
 #include "internal.hpp"
+#include "mfma.hpp"
 
 namespace skinny {
 
@@ -163,13 +178,130 @@ __global__ void __launch_bounds__(256) tsmm_kernel(TmArgs<T> a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// matrix-core TSMM
+// ---------------------------------------------------------------------------
+template <int S>
+struct TmBlocks {
+  // 16-row blocks per wave iteration: about 16 loads in flight per lane
+  static constexpr int value = S >= 16 ? 1 : (S >= 8 ? 2 : (S >= 4 ? 4 : 8));
+};
+
+template <typename T, int S, int NT, bool BETA0, bool VEC>
+__global__ void __launch_bounds__(256) tsmm_mfma_kernel(TmArgs<T> a) {
+  using V4 = typename Mfma<T>::v4;
+  constexpr int RB = TmBlocks<S>::value;
+  const int lane = threadIdx.x & 63;
+  const int li = lane & 15, lg = lane >> 4;
+  const int M = a.M, N = a.N;
+  const int64_t K = a.K;
+  const int64_t gw = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t nw = (int64_t)gridDim.x * 4;
+
+  // B operand of step s, subtile u: B(g*S + s, 16u + i), B column-major
+  T bop[S][NT];
+#pragma unroll
+  for (int s = 0; s < S; s++)
+#pragma unroll
+    for (int u = 0; u < NT; u++) {
+      const int m = lg * S + s, n = 16 * u + li;
+      bop[s][u] = (m < M && n < N) ? a.B[(int64_t)n * a.ldb + m] : T(0);
+    }
+  const int mbeg = lg * S;
+
+  const int64_t nblk = (K + 15) >> 4;
+  for (int64_t b0 = gw * RB; b0 < nblk; b0 += nw * RB) {
+    T av[RB][S];
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      const int64_t row = (b0 + r) * 16 + li;
+      const bool ok = row < K;
+      const T* ar = a.A + row * a.lda + mbeg;
+      if (VEC) {
+        // S even, lda even, A 16-byte aligned, M a multiple of 4: no masks
+#pragma unroll
+        for (int s = 0; s < S; s += 2) {
+          if (ok) {
+            double2 v = *reinterpret_cast<const double2*>(ar + s);
+            av[r][s] = v.x;
+            av[r][s + 1] = v.y;
+          } else {
+            av[r][s] = T(0);
+            av[r][s + 1] = T(0);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < S; s++)
+          av[r][s] = (ok && mbeg + s < M) ? ar[s] : T(0);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      V4 acc[NT];
+#pragma unroll
+      for (int u = 0; u < NT; u++) acc[u] = V4{0, 0, 0, 0};
+#pragma unroll
+      for (int s = 0; s < S; s++)
+#pragma unroll
+        for (int u = 0; u < NT; u++)
+          acc[u] = Mfma<T>::mac(av[r][s], bop[s][u], acc[u]);
+#pragma unroll
+      for (int u = 0; u < NT; u++) {
+        const int col = 16 * u + li;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int64_t row = (b0 + r) * 16 + Mfma<T>::drow(lg, v);
+          if (row < K && col < N) {
+            T* c = a.C + row * a.ldc + col;
+            T old = BETA0 ? T(0) : *c;
+            *c = axpby_t(acc[u][v], old, a.alpha, a.beta, BETA0);
+          }
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int S, int NT>
+static void launch_mfma_s_nt(const TmArgs<T>& a, int blocks, bool beta0,
+                             bool vec, hipStream_t st) {
+  if (vec && S % 2 == 0 && sizeof(T) == 8) {
+    if (beta0)
+      hipLaunchKernelGGL((tsmm_mfma_kernel<T, S, NT, true, (S % 2 == 0)>),
+                         dim3(blocks), dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL((tsmm_mfma_kernel<T, S, NT, false, (S % 2 == 0)>),
+                         dim3(blocks), dim3(256), 0, st, a);
+  } else {
+    if (beta0)
+      hipLaunchKernelGGL((tsmm_mfma_kernel<T, S, NT, true, false>),
+                         dim3(blocks), dim3(256), 0, st, a);
+    else
+      hipLaunchKernelGGL((tsmm_mfma_kernel<T, S, NT, false, false>),
+                         dim3(blocks), dim3(256), 0, st, a);
+  }
+}
+
+template <typename T, int S>
+static void launch_mfma_s(const TmArgs<T>& a, int blocks, int nt, bool beta0,
+                          bool vec, hipStream_t st) {
+  switch (nt) {
+    case 1: launch_mfma_s_nt<T, S, 1>(a, blocks, beta0, vec, st); break;
+    case 2: launch_mfma_s_nt<T, S, 2>(a, blocks, beta0, vec, st); break;
+    case 3: launch_mfma_s_nt<T, S, 3>(a, blocks, beta0, vec, st); break;
+    default: launch_mfma_s_nt<T, S, 4>(a, blocks, beta0, vec, st); break;
+  }
+}
+
 int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
                       int elem_bytes) {
-  (void)M;
   (void)N;
   (void)elem_bytes;
+  // measured on MI355X at K = 2^26 (fp64): the one-row-per-thread kernel for
+  // M = 1 wants four workgroups per CU, everything else two
   int64_t want = (K + 255) / 256;
-  int64_t cap = 8LL * h->cu_count;
+  int64_t cap = (M <= 1 ? 4LL : 2LL) * h->cu_count;
   if (want > cap) want = cap;
   if (want < 1) want = 1;
   return (int)want;
@@ -219,6 +351,24 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
   int blocks = blockCount > 0 ? (int)blockCount
                               : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T));
   const bool beta0 = (beta == T(0));
+  if (M >= 3 && N <= 64) {
+    const int S = (int)((M + 3) / 4);
+    const int Sc = S <= 1 ? 1 : S <= 2 ? 2 : S <= 4 ? 4 : S <= 8 ? 8 : 16;
+    const int nt = (int)((N + 15) / 16);
+    // unmasked 16-byte row slices need every slice start 16-byte aligned
+    const bool vecm = sizeof(T) == 8 && (M % 4 == 0) && (S % 2 == 0) &&
+                      (Sc == S) && (lda % 2 == 0) && ((uintptr_t)A % 16 == 0);
+    switch (Sc) {
+      case 1: launch_mfma_s<T, 1>(a, blocks, nt, beta0, false, stream); break;
+      case 2: launch_mfma_s<T, 2>(a, blocks, nt, beta0, vecm, stream); break;
+      case 4: launch_mfma_s<T, 4>(a, blocks, nt, beta0, vecm, stream); break;
+      case 8: launch_mfma_s<T, 8>(a, blocks, nt, beta0, vecm, stream); break;
+      default: launch_mfma_s<T, 16>(a, blocks, nt, beta0, vecm, stream); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "tsmm launch");
+    return SKINNY_OK;
+  }
   // 16-byte row accesses need 16-byte aligned rows for both A and C
   const bool vec = sizeof(T) == 8 && (lda % 2 == 0) && (ldc % 2 == 0) &&
                    ((uintptr_t)A % 16 == 0) && ((uintptr_t)C % 16 == 0);

@@ -23,25 +23,9 @@
 //    result is bitwise reproducible for a fixed blockCount.  Arrival counters
 //    are reset by their last user, so no memset is needed between calls.
 #include "internal.hpp"
+#include "mfma.hpp"
 
 namespace skinny {
-
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<double> {
-  typedef double v4 __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ v4 mac(double a, double b, v4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-};
-template <>
-struct Mfma<float> {
-  typedef float v4 __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ v4 mac(float a, float b, v4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-};
 
 template <typename T>
 struct TsArgs {
@@ -109,6 +93,7 @@ __device__ void reduce_partials(const TsArgs<T>& a, const T* src, int count,
 
 template <typename T, int P, int MT, int NT, bool SELF>
 __global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu((MT * NT >= 9) ? 2 : 1)))
     tsmttsm_kernel(TsArgs<T> a) {
   using V4 = typename Mfma<T>::v4;
   constexpr int U = Unroll<MT, NT, P>::value;
@@ -258,7 +243,7 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   // D tile as a DM x DN row-major matrix (MFMA D layout: lane l, element v
-  // holds D[4 * (l / 16) + v][l % 16])
+  // holds D[drow(l / 16, v)][l % 16])
   T* Dm = &red[1][0][0];
   if (wave == 0) {
 #pragma unroll
@@ -267,7 +252,7 @@ __global__ void __launch_bounds__(256)
       for (int u = 0; u < NT; u++)
 #pragma unroll
         for (int v = 0; v < 4; v++)
-          Dm[(16 * t + 4 * lg + v) * DN + 16 * u + li] = acc[t][u][v];
+          Dm[(16 * t + Mfma<T>::drow(lg, v)) * DN + 16 * u + li] = acc[t][u][v];
   }
   __syncthreads();
 
@@ -287,20 +272,25 @@ __global__ void __launch_bounds__(256)
   }
 
   // ---- grid reduction: last arrival per group, then last group ----------
+  // Arrival publishes this workgroup's partial with a release fence at agent
+  // scope (L2 write-back, no invalidate); only the last arrival takes the
+  // acquire side (L2 invalidate) before it reads the other partials.  A full
+  // __threadfence() would invalidate the XCD's L2 in every workgroup.
   __shared__ int s_last;
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   const int grp = kp / a.GS;
   const int gsz = min(a.GS, a.G - grp * a.GS);
   unsigned* cnt1 = a.cnt + tile * a.ngroups + grp;
   unsigned* cnt2 = a.cnt + ntiles * a.ngroups + tile;
   if (threadIdx.x == 0) {
-    unsigned ticket = atomicAdd(cnt1, 1u);
+    unsigned ticket = __hip_atomic_fetch_add(cnt1, 1u, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
     s_last = (ticket == (unsigned)(gsz - 1));
   }
   __syncthreads();
   if (!s_last) return;
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
   const T* gsrc = a.part1 + ((int64_t)tile * a.G + (int64_t)grp * a.GS) * PS;
   if (a.ngroups == 1) {
@@ -311,15 +301,16 @@ __global__ void __launch_bounds__(256)
   T* gdst = a.part2 + ((int64_t)tile * a.ngroups + grp) * PS;
   reduce_partials<T, P>(a, gsrc, gsz, PS, gdst, false, m0, n0, Mloc, Nloc);
   if (threadIdx.x == 0) *cnt1 = 0u;
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned ticket = atomicAdd(cnt2, 1u);
+    unsigned ticket = __hip_atomic_fetch_add(cnt2, 1u, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
     s_last = (ticket == (unsigned)(a.ngroups - 1));
   }
   __syncthreads();
   if (!s_last) return;
-  __threadfence();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   reduce_partials<T, P>(a, a.part2 + (int64_t)tile * a.ngroups * PS,
                         a.ngroups, PS, nullptr, true, m0, n0, Mloc, Nloc);
   if (threadIdx.x == 0) *cnt2 = 0u;
@@ -366,7 +357,10 @@ TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
   if (blockCount > 0) {
     G = (int64_t)blockCount;
   } else {
-    int64_t target = (2LL * h->cu_count) / ntiles;
+    // measured on MI355X at K = 2^26 (fp64): one workgroup per CU for tiles
+    // up to 16 x 16, two for the 32 x 32 and 64 x 64 tiles
+    int64_t per_cu = (p.MT * p.NT >= 4) ? 2 : 1;
+    int64_t target = (per_cu * h->cu_count) / ntiles;
     if (target < 1) target = 1;
     int64_t byrows = K / (CH * kWaves * 2);
     if (byrows < 1) byrows = 1;
