@@ -108,6 +108,24 @@ skinny_status_t skinny_stsmttsm(skinny_handle_t h, size_t blockCount,
                                 const float* B, int64_t ldb, float beta,
                                 float* C, int64_t ldc, skinny_stream_t stream);
 
+/* complex (DC, FC): plain transpose; M, N <= 512 */
+skinny_status_t skinny_ztsmttsm(skinny_handle_t h, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K,
+                                const skinny_double_complex* A, int64_t lda,
+                                skinny_double_complex alpha,
+                                const skinny_double_complex* B, int64_t ldb,
+                                skinny_double_complex beta,
+                                skinny_double_complex* C, int64_t ldc,
+                                skinny_stream_t stream);
+skinny_status_t skinny_ctsmttsm(skinny_handle_t h, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K,
+                                const skinny_float_complex* A, int64_t lda,
+                                skinny_float_complex alpha,
+                                const skinny_float_complex* B, int64_t ldb,
+                                skinny_float_complex beta,
+                                skinny_float_complex* C, int64_t ldc,
+                                skinny_stream_t stream);
+
 /* ---- TSMM: C = alpha A B + beta C -------------------------------------- */
 /* replaces tsmm_fix1..4/fix_fb/fix_blend/var1/varip1..3/varip_blend/
  * fix_ip_ghost/var_ip_ghost and tsmm_cublas<T> (tsmm/fix3.cuh:41-65,
@@ -121,6 +139,24 @@ skinny_status_t skinny_stsmm(skinny_handle_t h, size_t blockCount, int64_t M,
                              int64_t N, int64_t K, const float* A, int64_t lda,
                              float alpha, const float* B, int64_t ldb,
                              float beta, float* C, int64_t ldc,
+                             skinny_stream_t stream);
+
+/* complex (DC, FC): M, N <= 64; in place as for the real types */
+skinny_status_t skinny_ztsmm(skinny_handle_t h, size_t blockCount, int64_t M,
+                             int64_t N, int64_t K,
+                             const skinny_double_complex* A, int64_t lda,
+                             skinny_double_complex alpha,
+                             const skinny_double_complex* B, int64_t ldb,
+                             skinny_double_complex beta,
+                             skinny_double_complex* C, int64_t ldc,
+                             skinny_stream_t stream);
+skinny_status_t skinny_ctsmm(skinny_handle_t h, size_t blockCount, int64_t M,
+                             int64_t N, int64_t K,
+                             const skinny_float_complex* A, int64_t lda,
+                             skinny_float_complex alpha,
+                             const skinny_float_complex* B, int64_t ldb,
+                             skinny_float_complex beta,
+                             skinny_float_complex* C, int64_t ldc,
                              skinny_stream_t stream);
 
 /* ---- device data generation (counter RNG, identical to the host's) ----- */

@@ -25,6 +25,7 @@ from typing import Callable, List, Tuple
 __all__ = [
     "SkinnyError", "lib", "library_path", "available", "Handle",
     "tsmttsm", "tsmm", "dtsmttsm", "stsmttsm", "dtsmm", "stsmm",
+    "ztsmttsm", "ctsmttsm", "ztsmm", "ctsmm",
     "fill_uniform", "fill_uniform_f", "fill_int",
     "get_enabled_tsmttsm_versions", "get_enabled_tsmm_versions",
     "types", "flops_per_cell",
@@ -81,6 +82,10 @@ _SIGS = {
     "skinny_stsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, flt, vp, i64, flt, vp, i64, vp]),
     "skinny_dtsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, vp]),
     "skinny_stsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, flt, vp, i64, flt, vp, i64, vp]),
+    "skinny_ztsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, _DComplex, vp, i64, _DComplex, vp, i64, vp]),
+    "skinny_ctsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, _FComplex, vp, i64, _FComplex, vp, i64, vp]),
+    "skinny_ztsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, _DComplex, vp, i64, _DComplex, vp, i64, vp]),
+    "skinny_ctsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, _FComplex, vp, i64, _FComplex, vp, i64, vp]),
     "skinny_fill_uniform": (C.c_int, [vp, i64, u64, u64, i64, vp]),
     "skinny_fill_uniform_f": (C.c_int, [vp, i64, u64, u64, i64, vp]),
     "skinny_fill_int": (C.c_int, [vp, i64, u64, u64, i64, C.c_int, C.c_int, vp]),
@@ -205,41 +210,80 @@ def stsmm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
                                      _addr(Cm), ldc, _stream(stream)))
 
 
+def _z(v) -> _DComplex:
+    v = complex(v)
+    return _DComplex(v.real, v.imag)
+
+
+def _c(v) -> _FComplex:
+    v = complex(v)
+    return _FComplex(v.real, v.imag)
+
+
+def ztsmttsm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
+             handle=None, stream=None) -> bool:
+    return _check(lib().skinny_ztsmttsm(_h(handle), blockCount, M, N, K, _addr(A), lda,
+                                        _z(alpha), _addr(B), ldb, _z(beta),
+                                        _addr(Cm), ldc, _stream(stream)))
+
+
+def ctsmttsm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
+             handle=None, stream=None) -> bool:
+    return _check(lib().skinny_ctsmttsm(_h(handle), blockCount, M, N, K, _addr(A), lda,
+                                        _c(alpha), _addr(B), ldb, _c(beta),
+                                        _addr(Cm), ldc, _stream(stream)))
+
+
+def ztsmm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
+          handle=None, stream=None) -> bool:
+    return _check(lib().skinny_ztsmm(_h(handle), blockCount, M, N, K, _addr(A), lda,
+                                     _z(alpha), _addr(B), ldb, _z(beta),
+                                     _addr(Cm), ldc, _stream(stream)))
+
+
+def ctsmm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
+          handle=None, stream=None) -> bool:
+    return _check(lib().skinny_ctsmm(_h(handle), blockCount, M, N, K, _addr(A), lda,
+                                     _c(alpha), _addr(B), ldb, _c(beta),
+                                     _addr(Cm), ldc, _stream(stream)))
+
+
 def _dtype_key(t) -> str:
     import torch
-    return {torch.float64: "DR", torch.float32: "FR"}[t.dtype]
+    return {torch.float64: "DR", torch.float32: "FR",
+            torch.complex128: "DC", torch.complex64: "FC"}[t.dtype]
 
 
 def tsmttsm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
             handle=None, stream=None) -> bool:
     """C = alpha A^T B + beta C, the routine picked from C's dtype."""
-    fn = {"DR": dtsmttsm, "FR": stsmttsm}[_dtype_key(Cm)]
+    fn = {"DR": dtsmttsm, "FR": stsmttsm, "DC": ztsmttsm, "FC": ctsmttsm}[_dtype_key(Cm)]
     return fn(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc, handle, stream)
 
 
 def tsmm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
          handle=None, stream=None) -> bool:
     """C = alpha A B + beta C, the routine picked from C's dtype."""
-    fn = {"DR": dtsmm, "FR": stsmm}[_dtype_key(Cm)]
+    fn = {"DR": dtsmm, "FR": stsmm, "DC": ztsmm, "FC": ctsmm}[_dtype_key(Cm)]
     return fn(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc, handle, stream)
 
 
 # ---------------------------------------------------------------------------
 # registries (versions.hpp:47-177): the variants a test or perf sweep runs
 # ---------------------------------------------------------------------------
-types = ("FR", "DR")
+types = ("FR", "DR", "FC", "DC")
 flops_per_cell = {"FC": 8, "DC": 8, "FR": 2, "DR": 2, "HR": 9}  # types.hpp:17-57
 
 MatmulFunction = Callable[..., bool]
 
 
 def get_enabled_tsmttsm_versions(type_key: str = "DR") -> List[Tuple[MatmulFunction, str]]:
-    table = {"DR": dtsmttsm, "FR": stsmttsm}
+    table = {"DR": dtsmttsm, "FR": stsmttsm, "DC": ztsmttsm, "FC": ctsmttsm}
     return [(table[type_key], "SKINNY")]
 
 
 def get_enabled_tsmm_versions(type_key: str = "DR") -> List[Tuple[MatmulFunction, str]]:
-    table = {"DR": dtsmm, "FR": stsmm}
+    table = {"DR": dtsmm, "FR": stsmm, "DC": ztsmm, "FC": ctsmm}
     return [(table[type_key], "SKINNY")]
 
 

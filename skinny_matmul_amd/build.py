@@ -28,6 +28,7 @@ SOURCES = [
     "runtime.hip",
     "tsmttsm.hip",
     "tsmm.hip",
+    "tsmm_lds.hip",
     "capi.hip",
     "microbench.hip",
 ]

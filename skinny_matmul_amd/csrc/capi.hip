@@ -72,6 +72,61 @@ skinny_status_t tsmm_real(const char* fn, skinny_handle_t h, size_t blockCount,
                              beta, C, ldc, (hipStream_t)stream);
 }
 
+template <typename T, typename CT>
+skinny_status_t tsmttsm_cplx(const char* fn, skinny_handle_t h,
+                             size_t blockCount, int64_t M, int64_t N, int64_t K,
+                             const CT* A, int64_t lda, CT alpha, const CT* B,
+                             int64_t ldb, CT beta, CT* C, int64_t ldc,
+                             skinny_stream_t stream) {
+  skinny_status_t s = check_common(fn, M, N, K, A, B, C);
+  if (s != SKINNY_OK) return s;
+  if (lda < std::max<int64_t>(1, M) || ldb < std::max<int64_t>(1, N) ||
+      ldc < std::max<int64_t>(1, M))
+    return fail(SKINNY_INVALID_ARG, std::string(fn) + ": leading dimension too small");
+  if (M == 0 || N == 0) return SKINNY_OK;
+  if (M > 512 || N > 512)
+    return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": M, N <= 512 only");
+  h = resolve(h);
+  if (!h) return fail(SKINNY_HIP_ERROR, std::string(fn) + ": no device handle");
+  TsmttsmPlan p = plan_tsmttsm(h, blockCount, 2 * M, 2 * N, K, (int)sizeof(T),
+                               (int)sizeof(T));
+  return launch_tsmttsm<T>(h, p, 2 * M, 2 * N, K, (const T*)A, 2 * lda,
+                           alpha.x, alpha.y, (const T*)B, 2 * ldb, beta.x,
+                           beta.y, (T*)C, ldc, true, (hipStream_t)stream);
+}
+
+template <typename T, typename CT>
+skinny_status_t tsmm_cplx(const char* fn, skinny_handle_t h, size_t blockCount,
+                          int64_t M, int64_t N, int64_t K, const CT* A,
+                          int64_t lda, CT alpha, const CT* B, int64_t ldb,
+                          CT beta, CT* C, int64_t ldc, skinny_stream_t stream) {
+  skinny_status_t s = check_common(fn, M, N, K, A, B, C);
+  if (s != SKINNY_OK) return s;
+  if (lda < std::max<int64_t>(1, M) || ldb < std::max<int64_t>(1, M) ||
+      ldc < std::max<int64_t>(1, N))
+    return fail(SKINNY_INVALID_ARG, std::string(fn) + ": leading dimension too small");
+  if (K == 0 || N == 0) return SKINNY_OK;
+  if (M > 64 || N > 64)
+    return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": M, N <= 64 only");
+  const bool inplace = (const void*)A == (const void*)C;
+  if (inplace && lda != ldc)
+    return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": in place needs lda == ldc");
+  if (!inplace && M > 0) {
+    const char* a0 = (const char*)A;
+    const char* a1 = a0 + sizeof(CT) * ((K - 1) * lda + M);
+    const char* c0 = (const char*)C;
+    const char* c1 = c0 + sizeof(CT) * ((K - 1) * ldc + N);
+    if (a0 < c1 && c0 < a1)
+      return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": partially overlapping A and C");
+  }
+  h = resolve(h);
+  if (!h) return fail(SKINNY_HIP_ERROR, std::string(fn) + ": no device handle");
+  return launch_tsmm_lds<T>(h, blockCount, 2 * M, 2 * N, K, (const T*)A,
+                            2 * lda, alpha.x, alpha.y, (const T*)B, ldb,
+                            beta.x, beta.y, (T*)C, 2 * ldc, true,
+                            (hipStream_t)stream);
+}
+
 }  // namespace
 
 extern "C" {
@@ -133,6 +188,54 @@ skinny_status_t skinny_stsmm(skinny_handle_t h, size_t blockCount, int64_t M,
                              float beta, float* C, int64_t ldc,
                              skinny_stream_t stream) {
   return tsmm_real<float>("skinny_stsmm", h, blockCount, M, N, K, A, lda,
+                          alpha, B, ldb, beta, C, ldc, stream);
+}
+
+skinny_status_t skinny_ztsmttsm(skinny_handle_t h, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K,
+                                const skinny_double_complex* A, int64_t lda,
+                                skinny_double_complex alpha,
+                                const skinny_double_complex* B, int64_t ldb,
+                                skinny_double_complex beta,
+                                skinny_double_complex* C, int64_t ldc,
+                                skinny_stream_t stream) {
+  return tsmttsm_cplx<double>("skinny_ztsmttsm", h, blockCount, M, N, K, A,
+                              lda, alpha, B, ldb, beta, C, ldc, stream);
+}
+
+skinny_status_t skinny_ctsmttsm(skinny_handle_t h, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K,
+                                const skinny_float_complex* A, int64_t lda,
+                                skinny_float_complex alpha,
+                                const skinny_float_complex* B, int64_t ldb,
+                                skinny_float_complex beta,
+                                skinny_float_complex* C, int64_t ldc,
+                                skinny_stream_t stream) {
+  return tsmttsm_cplx<float>("skinny_ctsmttsm", h, blockCount, M, N, K, A, lda,
+                             alpha, B, ldb, beta, C, ldc, stream);
+}
+
+skinny_status_t skinny_ztsmm(skinny_handle_t h, size_t blockCount, int64_t M,
+                             int64_t N, int64_t K,
+                             const skinny_double_complex* A, int64_t lda,
+                             skinny_double_complex alpha,
+                             const skinny_double_complex* B, int64_t ldb,
+                             skinny_double_complex beta,
+                             skinny_double_complex* C, int64_t ldc,
+                             skinny_stream_t stream) {
+  return tsmm_cplx<double>("skinny_ztsmm", h, blockCount, M, N, K, A, lda,
+                           alpha, B, ldb, beta, C, ldc, stream);
+}
+
+skinny_status_t skinny_ctsmm(skinny_handle_t h, size_t blockCount, int64_t M,
+                             int64_t N, int64_t K,
+                             const skinny_float_complex* A, int64_t lda,
+                             skinny_float_complex alpha,
+                             const skinny_float_complex* B, int64_t ldb,
+                             skinny_float_complex beta,
+                             skinny_float_complex* C, int64_t ldc,
+                             skinny_stream_t stream) {
+  return tsmm_cplx<float>("skinny_ctsmm", h, blockCount, M, N, K, A, lda,
                           alpha, B, ldb, beta, C, ldc, stream);
 }
 

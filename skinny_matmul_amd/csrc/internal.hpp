@@ -61,6 +61,15 @@ struct TsmttsmPlan {
 TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
                          int64_t N, int64_t K, int elem_bytes, int partial_bytes);
 
+// complex (cplx = true): A, B, C point at interleaved (re, im) data and M, N,
+// lda, ldb are the real-view sizes (2x the complex ones); ldc stays complex.
+template <typename T>
+skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
+                               int64_t M, int64_t N, int64_t K, const T* A,
+                               int64_t lda, T alpha, T alpha_i, const T* B,
+                               int64_t ldb, T beta, T beta_i, T* C,
+                               int64_t ldc, bool cplx, hipStream_t stream);
+
 template <typename T>
 skinny_status_t launch_tsmttsm_real(skinny_handle_t h, const TsmttsmPlan& p,
                                     int64_t M, int64_t N, int64_t K,
@@ -77,5 +86,14 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
                                  int64_t M, int64_t N, int64_t K, const T* A,
                                  int64_t lda, T alpha, const T* B, int64_t ldb,
                                  T beta, T* C, int64_t ldc, hipStream_t stream);
+
+// B staged in LDS: complex (cplx, real-view M, N, lda, ldc; ldb in complex
+// units) and real widths 65..128.
+template <typename T>
+skinny_status_t launch_tsmm_lds(skinny_handle_t h, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K, const T* A,
+                                int64_t lda, T alpha, T alpha_i, const T* B,
+                                int64_t ldb, T beta, T beta_i, T* C,
+                                int64_t ldc, bool cplx, hipStream_t stream);
 
 }  // namespace skinny

@@ -334,8 +334,9 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
                                  int64_t M, int64_t N, int64_t K, const T* A,
                                  int64_t lda, T alpha, const T* B, int64_t ldb,
                                  T beta, T* C, int64_t ldc, hipStream_t stream) {
-  if (M > 64)
-    return fail(SKINNY_UNSUPPORTED, "tsmm: M > 64 not supported");
+  if (M > 64 || N > 64)
+    return launch_tsmm_lds<T>(h, blockCount, M, N, K, A, lda, alpha, T(0), B,
+                              ldb, beta, T(0), C, ldc, false, stream);
   TmArgs<T> a;
   a.A = A;
   a.B = B;

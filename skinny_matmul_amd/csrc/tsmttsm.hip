@@ -35,6 +35,8 @@ struct TsArgs {
   int64_t K, lda, ldb, ldc;
   int M, N;
   T alpha, beta;
+  T alpha_i, beta_i;  // imaginary parts (complex view only)
+  int cplx;           // 1: A, B, C are complex, viewed as interleaved reals
   T* part1;      // [tile][G][PS]
   T* part2;      // [tile][ngroups][PS]
   unsigned* cnt; // [tile][ngroups] then [tile]
@@ -58,27 +60,71 @@ __device__ __forceinline__ T epilogue(T s, T alpha, T beta, const T* c) {
   return r;
 }
 
+template <typename T>
+__device__ __forceinline__ T sum_entry(const T* src, int count, int PS, int e) {
+  T s = T(0);
+  const T* p = src + e;
+  int q = 0;
+  for (; q + 4 <= count; q += 4) {
+    T v0 = p[(int64_t)(q + 0) * PS];
+    T v1 = p[(int64_t)(q + 1) * PS];
+    T v2 = p[(int64_t)(q + 2) * PS];
+    T v3 = p[(int64_t)(q + 3) * PS];
+    s += v0;
+    s += v1;
+    s += v2;
+    s += v3;
+  }
+  for (; q < count; q++) s += p[(int64_t)q * PS];
+  return s;
+}
+
+// Complex result from the interleaved-real product R = A_r^T B_r:
+//   C(m, n) = alpha * [(R00 - R11) + i (R01 + R10)] + beta * C(m, n)
+// with Rab = R(2m + a, 2n + b); operation order of cu_complex.h (scale, then
+// axpy: beta * C + alpha * s), plain transpose (gen_cublas.cuh:30-33).
+template <typename T>
+__device__ __forceinline__ void complex_epilogue(const TsArgs<T>& a, T r00,
+                                                 T r11, T r01, T r10, T* c) {
+#pragma clang fp contract(off)
+  T re = r00 - r11;
+  T im = r01 + r10;
+  T o_re = a.alpha * re - a.alpha_i * im;
+  T o_im = a.alpha * im + a.alpha_i * re;
+  if (a.beta != T(0) || a.beta_i != T(0)) {
+    T c_re = c[0], c_im = c[1];
+    T b_re = a.beta * c_re - a.beta_i * c_im;
+    T b_im = a.beta * c_im + a.beta_i * c_re;
+    o_re = b_re + o_re;
+    o_im = b_im + o_im;
+  }
+  c[0] = o_re;
+  c[1] = o_im;
+}
+
 // Sum `count` partials of PS entries starting at src, in order, into dst or
 // (final) into C through the epilogue.
 template <typename T, int P>
 __device__ void reduce_partials(const TsArgs<T>& a, const T* src, int count,
                                 int PS, T* dst, bool final, int m0, int n0,
                                 int Mloc, int Nloc) {
-  for (int e = threadIdx.x; e < PS; e += blockDim.x) {
-    T s = T(0);
-    const T* p = src + e;
-    int q = 0;
-    for (; q + 4 <= count; q += 4) {
-      T v0 = p[(int64_t)(q + 0) * PS];
-      T v1 = p[(int64_t)(q + 1) * PS];
-      T v2 = p[(int64_t)(q + 2) * PS];
-      T v3 = p[(int64_t)(q + 3) * PS];
-      s += v0;
-      s += v1;
-      s += v2;
-      s += v3;
+  if (final && a.cplx) {
+    const int oMc = a.outM / 2, oNc = a.outN / 2;
+    for (int e = threadIdx.x; e < oMc * oNc; e += blockDim.x) {
+      const int mc = e % oMc, nc = e / oMc;
+      if (2 * mc >= Mloc || 2 * nc >= Nloc) continue;
+      const int e00 = (2 * nc) * a.outM + 2 * mc;
+      const T r00 = sum_entry(src, count, PS, e00);
+      const T r10 = sum_entry(src, count, PS, e00 + 1);
+      const T r01 = sum_entry(src, count, PS, e00 + a.outM);
+      const T r11 = sum_entry(src, count, PS, e00 + a.outM + 1);
+      T* c = a.C + 2 * ((int64_t)(n0 / 2 + nc) * a.ldc + (m0 / 2 + mc));
+      complex_epilogue(a, r00, r11, r01, r10, c);
     }
-    for (; q < count; q++) s += p[(int64_t)q * PS];
+    return;
+  }
+  for (int e = threadIdx.x; e < PS; e += blockDim.x) {
+    const T s = sum_entry(src, count, PS, e);
     if (!final) {
       dst[e] = s;
     } else {
@@ -418,11 +464,28 @@ static bool dispatch(const TsmttsmPlan& p, const TsArgs<T>& a, int blocks,
 }
 
 template <typename T>
+skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
+                               int64_t M, int64_t N, int64_t K, const T* A,
+                               int64_t lda, T alpha, T alpha_i, const T* B,
+                               int64_t ldb, T beta, T beta_i, T* C,
+                               int64_t ldc, bool cplx, hipStream_t stream);
+
+template <typename T>
 skinny_status_t launch_tsmttsm_real(skinny_handle_t h, const TsmttsmPlan& p,
                                     int64_t M, int64_t N, int64_t K,
                                     const T* A, int64_t lda, T alpha,
                                     const T* B, int64_t ldb, T beta, T* C,
                                     int64_t ldc, hipStream_t stream) {
+  return launch_tsmttsm<T>(h, p, M, N, K, A, lda, alpha, T(0), B, ldb, beta,
+                           T(0), C, ldc, false, stream);
+}
+
+template <typename T>
+skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
+                               int64_t M, int64_t N, int64_t K, const T* A,
+                               int64_t lda, T alpha, T alpha_i, const T* B,
+                               int64_t ldb, T beta, T beta_i, T* C,
+                               int64_t ldc, bool cplx, hipStream_t stream) {
   const int ntiles = p.tilesM * p.tilesN;
   if ((size_t)ntiles * (p.ngroups + 1) > (size_t)kMaxCounters)
     return fail(SKINNY_UNSUPPORTED, "tsmttsm: too many partial groups");
@@ -440,6 +503,9 @@ skinny_status_t launch_tsmttsm_real(skinny_handle_t h, const TsmttsmPlan& p,
   a.N = (int)N;
   a.alpha = alpha;
   a.beta = beta;
+  a.alpha_i = alpha_i;
+  a.beta_i = beta_i;
+  a.cplx = cplx ? 1 : 0;
   a.part1 = static_cast<T*>(ws_scratch(h));
   a.part2 = a.part1 + (size_t)ntiles * p.G * p.outM * p.outN;
   a.cnt = ws_counters(h);
@@ -462,6 +528,14 @@ skinny_status_t launch_tsmttsm_real(skinny_handle_t h, const TsmttsmPlan& p,
   return SKINNY_OK;
 }
 
+template skinny_status_t launch_tsmttsm<double>(
+    skinny_handle_t, const TsmttsmPlan&, int64_t, int64_t, int64_t,
+    const double*, int64_t, double, double, const double*, int64_t, double,
+    double, double*, int64_t, bool, hipStream_t);
+template skinny_status_t launch_tsmttsm<float>(
+    skinny_handle_t, const TsmttsmPlan&, int64_t, int64_t, int64_t,
+    const float*, int64_t, float, float, const float*, int64_t, float, float,
+    float*, int64_t, bool, hipStream_t);
 template skinny_status_t launch_tsmttsm_real<double>(
     skinny_handle_t, const TsmttsmPlan&, int64_t, int64_t, int64_t,
     const double*, int64_t, double, const double*, int64_t, double, double*,

@@ -1,0 +1,160 @@
+"""Complex TSMTTSM / TSMM (DC, FC) and real TSMM widths 65..128 on the GPU.
+
+Integer-valued complex data keeps every product and sum exact, so results
+are compared bitwise with the oracle (the reference compares bitwise against
+cuBLAS Zgemm/Cgemm, test.cu:174, gen_cublas.cuh:142-151, tsmm/cublas.cuh:
+204-213).  Complex TSMTTSM is a plain transpose (CUBLAS_OP_T, no conjugate).
+"""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+from gpu_util import EPS64, dev, host, note, rel_fro
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _cz(v):
+    return np.array([complex(a, b) for a, b in v], dtype=np.complex128)
+
+
+def cint(orc, n, seed, stream):
+    re = orc.integers(n, seed, 2 * stream)
+    im = orc.integers(n, seed, 2 * stream + 1)
+    return (re + 1j * im).astype(np.complex128)
+
+
+def test_golden_complex(gpu):
+    with open(os.path.join(GOLD, "complex_int.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        for dt in (np.complex128, np.complex64):
+            for bc in (0, 3):
+                note(f"complex golden {c['op']} {np.dtype(dt).name} M={c['M']} N={c['N']} bc={bc}")
+                A, B, C = (_cz(c[k]).astype(dt) for k in ("A", "B", "C0"))
+                dA, dB, dC = dev(A), dev(B), dev(C)
+                fn = gpu.tsmttsm if c["op"] == "ztsmttsm" else gpu.tsmm
+                ok = fn(bc, c["M"], c["N"], c["K"], dA, c["lda"], complex(*c["alpha"]), dB,
+                        c["ldb"], complex(*c["beta"]), dC, c["ldc"])
+                assert ok
+                assert np.array_equal(host(dC), _cz(c["C"]).astype(dt))
+
+
+def _fuzz_tsmttsm(gpu, orc, rng, M, N, dt, Kmax):
+    self_ = M == N and rng.random() < 0.5
+    lda = M + rng.randint(0, 4)
+    ldb = lda if self_ else N + rng.randint(0, 4)
+    ldc = M + rng.randint(0, 4)
+    K = rng.randint(1, Kmax)
+    bc = rng.randint(1, 200)
+    alpha = complex(rng.randint(-2, 2), rng.randint(-2, 2))
+    beta = complex(rng.randint(-1, 1), rng.randint(-1, 1))
+    seed = rng.randint(0, 1 << 30)
+    note(f"ztsmttsm fuzz {np.dtype(dt).name} M={M} N={N} K={K} lda={lda} ldb={ldb} ldc={ldc} "
+         f"bc={bc} self={self_} alpha={alpha} beta={beta} seed={seed}")
+    A = cint(orc, K * lda, seed, 0).astype(dt)
+    B = A if self_ else cint(orc, K * ldb, seed, 1).astype(dt)
+    C = cint(orc, N * ldc, seed, 2).astype(dt)
+    want = C.copy()
+    orc.tsmttsm(M, N, K, A, lda, alpha, B, ldb, beta, want, ldc)
+    dA = dev(A)
+    dB = dA if self_ else dev(B)
+    dC = dev(C)
+    assert gpu.tsmttsm(bc, M, N, K, dA, lda, alpha, dB, ldb, beta, dC, ldc)
+    assert np.array_equal(host(dC), want), (M, N, K, bc, self_)
+
+
+def _fuzz_tsmm(gpu, orc, rng, M, N, dt, Kmax):
+    inplace = rng.random() < 0.5
+    ldb = M + rng.randint(0, 4)
+    if inplace:
+        ldc = max(N + rng.randint(0, 4), M)
+        lda = ldc
+    else:
+        lda = M + rng.randint(0, 4)
+        ldc = N + rng.randint(0, 4)
+    K = rng.randint(1, Kmax)
+    bc = rng.randint(1, 200)
+    alpha = complex(rng.randint(-2, 2), rng.randint(-2, 2))
+    beta = complex(rng.randint(-1, 1), rng.randint(-1, 1))
+    seed = rng.randint(0, 1 << 30)
+    note(f"ztsmm fuzz {np.dtype(dt).name} M={M} N={N} K={K} lda={lda} ldb={ldb} ldc={ldc} "
+         f"bc={bc} inplace={inplace} alpha={alpha} beta={beta} seed={seed}")
+    B = cint(orc, N * ldb, seed, 1).astype(dt)
+    C = cint(orc, K * ldc, seed, 2).astype(dt)
+    A = C.copy() if inplace else cint(orc, K * lda, seed, 0).astype(dt)
+    want = C.copy()
+    orc.tsmm(M, N, K, want if inplace else A, lda, alpha, B, ldb, beta, want, ldc)
+    dC = dev(C)
+    dA = dC if inplace else dev(A)
+    dB = dev(B)
+    assert gpu.tsmm(bc, M, N, K, dA, lda, alpha, dB, ldb, beta, dC, ldc)
+    assert np.array_equal(host(dC), want), (M, N, K, bc, inplace)
+
+
+@pytest.mark.parametrize("M", [1, 2, 3, 4, 7, 8, 13, 16, 24, 32, 33, 48, 64])
+def test_fuzz_ztsmttsm(gpu, orc, M):
+    rng = random.Random(300 + M)
+    for _ in range(3):
+        _fuzz_tsmttsm(gpu, orc, rng, M, M, np.complex128, 8000)
+    _fuzz_tsmttsm(gpu, orc, rng, M, rng.randint(1, 64), np.complex128, 4000)
+
+
+@pytest.mark.parametrize("M", [1, 2, 3, 4, 5, 8, 12, 16, 20, 31, 32, 40, 64])
+def test_fuzz_ztsmm(gpu, orc, M):
+    rng = random.Random(400 + M)
+    for _ in range(3):
+        _fuzz_tsmm(gpu, orc, rng, M, M, np.complex128, 8000)
+    _fuzz_tsmm(gpu, orc, rng, M, rng.randint(1, 64), np.complex128, 4000)
+
+
+@pytest.mark.parametrize("M,N", [(1, 1), (4, 4), (8, 8), (16, 16), (32, 32), (5, 9)])
+def test_fuzz_complex64(gpu, orc, M, N):
+    rng = random.Random(M * 13 + N)
+    for _ in range(2):
+        _fuzz_tsmttsm(gpu, orc, rng, M, N, np.complex64, 8000)
+        _fuzz_tsmm(gpu, orc, rng, M, N, np.complex64, 8000)
+
+
+@pytest.mark.parametrize("M", [8, 32, 64])
+def test_ztsmm_uniform_tolerance(gpu, orc, M):
+    K = 1 << 14
+    note(f"ztsmm uniform M=N={M} K={K}")
+    A = (orc.uniform(K * M, 6, 0) + 1j * orc.uniform(K * M, 6, 1)).astype(np.complex128)
+    B = (orc.uniform(M * M, 6, 2) + 1j * orc.uniform(M * M, 6, 3)).astype(np.complex128)
+    want = np.zeros(K * M, dtype=np.complex128)
+    orc.tsmm(M, M, K, A, M, 1.0, B, M, 0.0, want, M)
+    dC = dev(np.zeros(K * M, dtype=np.complex128))
+    assert gpu.tsmm(0, M, M, K, dev(A), M, 1.0, dev(B), M, 0.0, dC, M)
+    assert rel_fro(host(dC), want) <= 8 * M * EPS64
+
+
+@pytest.mark.parametrize("M,N", [(65, 65), (96, 40), (128, 128), (20, 100), (128, 3)])
+def test_real_tsmm_wide(gpu, orc, M, N):
+    rng = random.Random(M + 7 * N)
+    for _ in range(2):
+        inplace = rng.random() < 0.5 and True
+        ldb = M + rng.randint(0, 4)
+        if inplace:
+            ldc = max(N + rng.randint(0, 4), M)
+            lda = ldc
+        else:
+            lda = M + rng.randint(0, 4)
+            ldc = N + rng.randint(0, 4)
+        K = rng.randint(1, 5000)
+        beta = float(rng.randint(0, 1))
+        seed = rng.randint(0, 1 << 30)
+        note(f"dtsmm wide M={M} N={N} K={K} inplace={inplace} beta={beta} seed={seed}")
+        B = orc.integers(N * ldb, seed, 1)
+        C = orc.integers(K * ldc, seed, 2)
+        A = C.copy() if inplace else orc.integers(K * lda, seed, 0)
+        want = C.copy()
+        orc.tsmm(M, N, K, want if inplace else A, lda, 1.0, B, ldb, beta, want, ldc)
+        dC = dev(C)
+        dA = dC if inplace else dev(A)
+        assert gpu.dtsmm(0, M, N, K, dA, lda, 1.0, dev(B), ldb, beta, dC, ldc)
+        assert np.array_equal(host(dC), want)
