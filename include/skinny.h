@@ -126,6 +126,25 @@ skinny_status_t skinny_ctsmttsm(skinny_handle_t h, size_t blockCount,
                                 skinny_float_complex* C, int64_t ldc,
                                 skinny_stream_t stream);
 
+/* HR (types.hpp:49-57): double inputs and output, double-double (PseudoQuad)
+ * accumulation through FMA TwoProd and the accurate dd addition; replaces the
+ * iT = PseudoQuad instantiations (genv3.cuh:46-79, specsmall.cuh) and their
+ * epilogue convert(scale2(renormalize(sum), alpha)) + beta C
+ * (genv3.cuh:19-26).  M, N <= 1024. */
+skinny_status_t skinny_htsmttsm(skinny_handle_t h, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K,
+                                const double* A, int64_t lda, double alpha,
+                                const double* B, int64_t ldb, double beta,
+                                double* C, int64_t ldc, skinny_stream_t stream);
+/* the same accumulation with the double-double result kept:
+ * C = alpha * sum + beta * C evaluated in dd, C an array of (s, t) pairs */
+skinny_status_t skinny_htsmttsm_dd(skinny_handle_t h, size_t blockCount,
+                                   int64_t M, int64_t N, int64_t K,
+                                   const double* A, int64_t lda, double alpha,
+                                   const double* B, int64_t ldb, double beta,
+                                   skinny_dd* C, int64_t ldc,
+                                   skinny_stream_t stream);
+
 /* ---- TSMM: C = alpha A B + beta C -------------------------------------- */
 /* replaces tsmm_fix1..4/fix_fb/fix_blend/var1/varip1..3/varip_blend/
  * fix_ip_ghost/var_ip_ghost and tsmm_cublas<T> (tsmm/fix3.cuh:41-65,

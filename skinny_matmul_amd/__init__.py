@@ -25,7 +25,7 @@ from typing import Callable, List, Tuple
 __all__ = [
     "SkinnyError", "lib", "library_path", "available", "Handle",
     "tsmttsm", "tsmm", "dtsmttsm", "stsmttsm", "dtsmm", "stsmm",
-    "ztsmttsm", "ctsmttsm", "ztsmm", "ctsmm",
+    "ztsmttsm", "ctsmttsm", "ztsmm", "ctsmm", "htsmttsm", "htsmttsm_dd",
     "fill_uniform", "fill_uniform_f", "fill_int",
     "get_enabled_tsmttsm_versions", "get_enabled_tsmm_versions",
     "types", "flops_per_cell",
@@ -86,6 +86,8 @@ _SIGS = {
     "skinny_ctsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, _FComplex, vp, i64, _FComplex, vp, i64, vp]),
     "skinny_ztsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, _DComplex, vp, i64, _DComplex, vp, i64, vp]),
     "skinny_ctsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, _FComplex, vp, i64, _FComplex, vp, i64, vp]),
+    "skinny_htsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, vp]),
+    "skinny_htsmttsm_dd": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, vp]),
     "skinny_fill_uniform": (C.c_int, [vp, i64, u64, u64, i64, vp]),
     "skinny_fill_uniform_f": (C.c_int, [vp, i64, u64, u64, i64, vp]),
     "skinny_fill_int": (C.c_int, [vp, i64, u64, u64, i64, C.c_int, C.c_int, vp]),
@@ -210,6 +212,22 @@ def stsmm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
                                      _addr(Cm), ldc, _stream(stream)))
 
 
+def htsmttsm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
+             handle=None, stream=None) -> bool:
+    """HR: double in/out, double-double accumulation (types.hpp:49-57)."""
+    return _check(lib().skinny_htsmttsm(_h(handle), blockCount, M, N, K, _addr(A), lda,
+                                        float(alpha), _addr(B), ldb, float(beta),
+                                        _addr(Cm), ldc, _stream(stream)))
+
+
+def htsmttsm_dd(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cdd, ldc,
+                handle=None, stream=None) -> bool:
+    """HR with the dd result kept: Cdd holds (s, t) pairs (2 * ldc * N doubles)."""
+    return _check(lib().skinny_htsmttsm_dd(_h(handle), blockCount, M, N, K, _addr(A), lda,
+                                           float(alpha), _addr(B), ldb, float(beta),
+                                           _addr(Cdd), ldc, _stream(stream)))
+
+
 def _z(v) -> _DComplex:
     v = complex(v)
     return _DComplex(v.real, v.imag)
@@ -271,19 +289,20 @@ def tsmm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
 # ---------------------------------------------------------------------------
 # registries (versions.hpp:47-177): the variants a test or perf sweep runs
 # ---------------------------------------------------------------------------
-types = ("FR", "DR", "FC", "DC")
+types = ("FR", "DR", "FC", "DC", "HR")
 flops_per_cell = {"FC": 8, "DC": 8, "FR": 2, "DR": 2, "HR": 9}  # types.hpp:17-57
 
 MatmulFunction = Callable[..., bool]
 
 
 def get_enabled_tsmttsm_versions(type_key: str = "DR") -> List[Tuple[MatmulFunction, str]]:
-    table = {"DR": dtsmttsm, "FR": stsmttsm, "DC": ztsmttsm, "FC": ctsmttsm}
+    table = {"DR": dtsmttsm, "FR": stsmttsm, "DC": ztsmttsm, "FC": ctsmttsm,
+             "HR": htsmttsm}
     return [(table[type_key], "SKINNY")]
 
 
 def get_enabled_tsmm_versions(type_key: str = "DR") -> List[Tuple[MatmulFunction, str]]:
-    table = {"DR": dtsmm, "FR": stsmm, "DC": ztsmm, "FC": ctsmm}
+    table = {"DR": dtsmm, "FR": stsmm, "DC": ztsmm, "FC": ctsmm, "HR": dtsmm}
     return [(table[type_key], "SKINNY")]
 
 

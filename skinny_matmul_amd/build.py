@@ -27,6 +27,7 @@ ARCH = os.environ.get("SKINNY_ARCH", "gfx950")
 SOURCES = [
     "runtime.hip",
     "tsmttsm.hip",
+    "htsmttsm.hip",
     "tsmm.hip",
     "tsmm_lds.hip",
     "capi.hip",

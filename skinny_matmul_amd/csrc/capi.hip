@@ -127,6 +127,26 @@ skinny_status_t tsmm_cplx(const char* fn, skinny_handle_t h, size_t blockCount,
                             (hipStream_t)stream);
 }
 
+skinny_status_t htsmttsm_any(const char* fn, skinny_handle_t h,
+                             size_t blockCount, int64_t M, int64_t N, int64_t K,
+                             const double* A, int64_t lda, double alpha,
+                             const double* B, int64_t ldb, double beta,
+                             double* C, bool dd_out, int64_t ldc,
+                             skinny_stream_t stream) {
+  skinny_status_t s = check_common(fn, M, N, K, A, B, C);
+  if (s != SKINNY_OK) return s;
+  if (lda < std::max<int64_t>(1, M) || ldb < std::max<int64_t>(1, N) ||
+      ldc < std::max<int64_t>(1, M))
+    return fail(SKINNY_INVALID_ARG, std::string(fn) + ": leading dimension too small");
+  if (M == 0 || N == 0) return SKINNY_OK;
+  if (M > 1024 || N > 1024)
+    return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": M, N <= 1024 only");
+  h = resolve(h);
+  if (!h) return fail(SKINNY_HIP_ERROR, std::string(fn) + ": no device handle");
+  return launch_htsmttsm(h, blockCount, M, N, K, A, lda, alpha, B, ldb, beta, C,
+                         dd_out, ldc, (hipStream_t)stream);
+}
+
 }  // namespace
 
 extern "C" {
@@ -237,6 +257,26 @@ skinny_status_t skinny_ctsmm(skinny_handle_t h, size_t blockCount, int64_t M,
                              skinny_stream_t stream) {
   return tsmm_cplx<float>("skinny_ctsmm", h, blockCount, M, N, K, A, lda,
                           alpha, B, ldb, beta, C, ldc, stream);
+}
+
+skinny_status_t skinny_htsmttsm(skinny_handle_t h, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K,
+                                const double* A, int64_t lda, double alpha,
+                                const double* B, int64_t ldb, double beta,
+                                double* C, int64_t ldc,
+                                skinny_stream_t stream) {
+  return htsmttsm_any("skinny_htsmttsm", h, blockCount, M, N, K, A, lda, alpha,
+                      B, ldb, beta, C, false, ldc, stream);
+}
+
+skinny_status_t skinny_htsmttsm_dd(skinny_handle_t h, size_t blockCount,
+                                   int64_t M, int64_t N, int64_t K,
+                                   const double* A, int64_t lda, double alpha,
+                                   const double* B, int64_t ldb, double beta,
+                                   skinny_dd* C, int64_t ldc,
+                                   skinny_stream_t stream) {
+  return htsmttsm_any("skinny_htsmttsm_dd", h, blockCount, M, N, K, A, lda,
+                      alpha, B, ldb, beta, (double*)C, true, ldc, stream);
 }
 
 }  // extern "C"

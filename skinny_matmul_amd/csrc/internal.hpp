@@ -87,6 +87,14 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
                                  int64_t lda, T alpha, const T* B, int64_t ldb,
                                  T beta, T* C, int64_t ldc, hipStream_t stream);
 
+// HR: double in, PseudoQuad accumulation; dd_out writes (s, t) pairs to C
+skinny_status_t launch_htsmttsm(skinny_handle_t h, size_t blockCount,
+                                int64_t M, int64_t N, int64_t K,
+                                const double* A, int64_t lda, double alpha,
+                                const double* B, int64_t ldb, double beta,
+                                double* C, bool dd_out, int64_t ldc,
+                                hipStream_t stream);
+
 // B staged in LDS: complex (cplx, real-view M, N, lda, ldc; ldb in complex
 // units) and real widths 65..128.
 template <typename T>
