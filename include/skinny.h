@@ -193,6 +193,38 @@ skinny_status_t skinny_fill_int(double* out, int64_t n, uint64_t seed,
                                 uint64_t stream_id, int64_t offset, int lo,
                                 int hi, skinny_stream_t stream);
 
+/* ---- SVQB orthogonalisation (svqb/svqb.cu:72-130) ---------------------- */
+/* One block: X is K x M row-major (ldx >= M), M <= 64, updated in place. */
+typedef struct {
+  int64_t M, K;
+  double* X;
+  int64_t ldx;
+} skinny_svqb_block;
+/* An executable hipGraph owning its buffers (see skinny_dsvqb_graph). */
+typedef struct skinny_graph_s* skinny_graph_t;
+/* `sweeps` SVQB sweeps over `nblocks` independent blocks: per sweep the Gram
+ * S = X^T X (TSMTTSM), the FP64 eigensolve of D S D on the device (one
+ * workgroup per block) giving T = D U L^-1/2, and X <- X T in place (TSMM).
+ * Synchronises `stream` before returning (its per-call buffers are freed). */
+skinny_status_t skinny_dsvqb(skinny_handle_t h, int nblocks,
+                             const skinny_svqb_block* blocks, int sweeps,
+                             skinny_stream_t stream);
+/* Capture the same `sweeps` into a hipGraph (no memsets, no host syncs, no
+ * allocations inside); replay with skinny_graph_launch.  The graph owns a
+ * private handle and buffers, so its device pointers stay valid. */
+skinny_status_t skinny_dsvqb_graph(skinny_handle_t h, int nblocks,
+                                   const skinny_svqb_block* blocks, int sweeps,
+                                   skinny_stream_t stream,
+                                   skinny_graph_t* graph);
+skinny_status_t skinny_graph_launch(skinny_graph_t graph,
+                                    skinny_stream_t stream);
+skinny_status_t skinny_graph_destroy(skinny_graph_t graph);
+/* ||X^T X - I||_F with the double-double Gram (svqb.cu:184-211); synchronous,
+ * result on the host. */
+skinny_status_t skinny_dorth_error(skinny_handle_t h, int64_t M, int64_t K,
+                                   const double* X, int64_t ldx, double* fro,
+                                   skinny_stream_t stream);
+
 /* ---- microbenchmarks: this chip's own ceilings ------------------------ */
 /* HBM stream over arrays of `bytes_per_array` (allocated internally, far
  * above the Infinity Cache when >= 1 GiB); kind 0 read, 1 copy, 2 triad,

@@ -71,6 +71,10 @@ def lib():
             "orc_xdot": (None, [i64, vp, i64, vp, i64, vp, vp]),
             "orc_xtsmttsm": (None, [i64, i64, i64, vp, i64, vp, i64, vp, vp, i64]),
             "orc_xtsmttsm_entries": (None, [i64, vp, i64, vp, i64, i64, vp, vp, vp, vp]),
+            "orc_jacobi_eig": (C.c_int, [i64, vp, i64, vp, vp, i64, C.c_int]),
+            "orc_svqb_transform": (None, [i64, vp, i64, vp, i64]),
+            "orc_svqb_sweep": (None, [i64, i64, vp, i64]),
+            "orc_gram_error": (None, [i64, i64, vp, i64, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -252,3 +256,46 @@ def dd_renormalize(x):
 
 def max_threads() -> int:
     return int(lib().orc_max_threads())
+
+
+# SVQB ----------------------------------------------------------------------
+def jacobi_eig(S: np.ndarray, max_sweeps: int = 40):
+    """Eigen-decomposition of a symmetric matrix: (w, V), V columns."""
+    M = S.shape[0]
+    A = np.array(S, dtype=np.float64, order="F").copy(order="F")
+    Af = A.ravel(order="F").copy()
+    w = np.zeros(M)
+    V = np.zeros(M * M)
+    sweeps = lib().orc_jacobi_eig(M, _p(Af), M, _p(w), _p(V), M, max_sweeps)
+    return w, V.reshape(M, M, order="F"), sweeps
+
+
+def svqb_transform(S: np.ndarray) -> np.ndarray:
+    M = S.shape[0]
+    Sf = np.array(S, dtype=np.float64).ravel(order="F").copy()
+    T = np.zeros(M * M)
+    lib().orc_svqb_transform(M, _p(Sf), M, _p(T), M)
+    return T.reshape(M, M, order="F")
+
+
+def svqb_sweep(M, K, X, ldx):
+    lib().orc_svqb_sweep(M, K, _p(X), ldx)
+    return X
+
+
+def gram_error(M, K, X, ldx):
+    f = np.zeros(1)
+    m = np.zeros(1)
+    lib().orc_gram_error(M, K, _p(X), ldx, _p(f), _p(m))
+    return float(f[0]), float(m[0])
+
+
+def svqb_test_block(M, K, seed, cond_exp=6.0):
+    """X = Q diag(logspace(0, -cond_exp, M)) W^T, Q (K x M) and W (M x M)
+    orthonormal from a seeded QR; row-major K x M."""
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((K, M)))
+    W, _ = np.linalg.qr(rng.standard_normal((M, M)))
+    sig = np.logspace(0, -cond_exp, M)
+    X = (Q * sig) @ W.T
+    return np.ascontiguousarray(X).ravel()

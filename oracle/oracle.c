@@ -613,3 +613,100 @@ void orc_xtsmttsm_entries(int64_t K, const double* A, int64_t lda,
   for (int64_t e = 0; e < nentries; e++)
     orc_xdot(K, A + ms[e], lda, B + ns[e], ldb, &hi[e], &lo[e]);
 }
+
+/* ------------------------------------------------------------------------ */
+/* SVQB, svqb/svqb.cu:72-130, FP64 throughout                               */
+/* ------------------------------------------------------------------------ */
+/* Cyclic Jacobi eigensolver for a symmetric M x M column-major S (destroyed;
+ * its diagonal ends as the eigenvalues).  Textbook row-cyclic ordering
+ * (Golub & Van Loan, Matrix Computations, 4th ed., Alg. 8.5.3) with the
+ * stable rotation t = sign(tau) / (|tau| + sqrt(1 + tau^2)).  V receives the
+ * eigenvectors as columns.  Returns the sweeps used. */
+int orc_jacobi_eig(int64_t M, double* S, int64_t lds, double* w, double* V,
+                   int64_t ldv, int max_sweeps) {
+  for (int64_t j = 0; j < M; j++)
+    for (int64_t i = 0; i < M; i++) V[j * ldv + i] = (i == j) ? 1.0 : 0.0;
+  int sweep = 0;
+  for (; sweep < max_sweeps; sweep++) {
+    double off = 0.0, dia = 0.0;
+    for (int64_t j = 0; j < M; j++)
+      for (int64_t i = 0; i < M; i++) {
+        double v = S[j * lds + i];
+        if (i == j) dia += v * v; else off += v * v;
+      }
+    if (off <= 1e-30 * dia) break;
+    for (int64_t p = 0; p < M - 1; p++)
+      for (int64_t q = p + 1; q < M; q++) {
+        double apq = S[q * lds + p];
+        if (apq == 0.0) continue;
+        double app = S[p * lds + p], aqq = S[q * lds + q];
+        double tau = (aqq - app) / (2.0 * apq);
+        double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+        for (int64_t j = 0; j < M; j++) { /* rows p, q */
+          double a = S[j * lds + p], b = S[j * lds + q];
+          S[j * lds + p] = c * a - s * b;
+          S[j * lds + q] = s * a + c * b;
+        }
+        for (int64_t i = 0; i < M; i++) { /* columns p, q */
+          double a = S[p * lds + i], b = S[q * lds + i];
+          S[p * lds + i] = c * a - s * b;
+          S[q * lds + i] = s * a + c * b;
+          double va = V[p * ldv + i], vb = V[q * ldv + i];
+          V[p * ldv + i] = c * va - s * vb;
+          V[q * ldv + i] = s * va + c * vb;
+        }
+        S[q * lds + p] = 0.0;
+        S[p * lds + q] = 0.0;
+      }
+  }
+  for (int64_t i = 0; i < M; i++) w[i] = S[i * lds + i];
+  return sweep;
+}
+
+/* T = D U L^-1/2 from the Gram S (svqb.cu:99-119); T column-major M x M */
+void orc_svqb_transform(int64_t M, const double* S, int64_t lds, double* T,
+                        int64_t ldt) {
+  double* A = (double*)malloc(sizeof(double) * M * M);
+  double* V = (double*)malloc(sizeof(double) * M * M);
+  double* w = (double*)malloc(sizeof(double) * M);
+  double* d = (double*)malloc(sizeof(double) * M);
+  for (int64_t i = 0; i < M; i++) {
+    double sii = S[i * lds + i];
+    d[i] = (fabs(sii) > 1.0e-15) ? 1.0 / sqrt(sii) : 0.0;
+  }
+  for (int64_t j = 0; j < M; j++)
+    for (int64_t i = 0; i < M; i++) A[j * M + i] = S[j * lds + i] * (d[i] * d[j]);
+  orc_jacobi_eig(M, A, M, w, V, M, 40);
+  for (int64_t n = 0; n < M; n++)
+    for (int64_t m = 0; m < M; m++)
+      T[n * ldt + m] = V[n * M + m] * (d[m] * (1.0 / sqrt(fmax(1.0e-15, w[n]))));
+  free(A); free(V); free(w); free(d);
+}
+
+/* one sweep: X (K x M row-major) <- X T with T from X^T X */
+void orc_svqb_sweep(int64_t M, int64_t K, double* X, int64_t ldx) {
+  double* S = (double*)calloc(M * M, sizeof(double));
+  double* T = (double*)malloc(sizeof(double) * M * M);
+  orc_dtsmttsm(M, M, K, X, ldx, 1.0, X, ldx, 0.0, S, M);
+  orc_svqb_transform(M, S, M, T, M);
+  orc_dtsmm(M, M, K, X, ldx, 1.0, T, M, 0.0, X, ldx);
+  free(S); free(T);
+}
+
+/* ||X^T X - I||_F (svqb.cu:184-211) and max |X^T X - I| (svqb.cu:213-239),
+ * the Gram accumulated exactly */
+void orc_gram_error(int64_t M, int64_t K, const double* X, int64_t ldx,
+                    double* fro, double* maxabs) {
+  double l2 = 0.0, mx = 0.0;
+  for (int64_t n = 0; n < M; n++)
+    for (int64_t m = 0; m < M; m++) {
+      double hi, lo;
+      orc_xdot(K, X + m, ldx, X + n, ldx, &hi, &lo);
+      double e = (m == n ? 1.0 : 0.0) - hi - lo;
+      l2 += e * e;
+      if (fabs(e) > mx) mx = fabs(e);
+    }
+  *fro = sqrt(l2);
+  *maxabs = mx;
+}

@@ -120,6 +120,15 @@ void orc_xtsmttsm_entries(int64_t K, const double* A, int64_t lda,
                           const int64_t* ms, const int64_t* ns, double* hi,
                           double* lo);
 
+/* ---- SVQB restatement (svqb/svqb.cu:72-130) ----------------------------- */
+int orc_jacobi_eig(int64_t M, double* S, int64_t lds, double* w, double* V,
+                   int64_t ldv, int max_sweeps);
+void orc_svqb_transform(int64_t M, const double* S, int64_t lds, double* T,
+                        int64_t ldt);
+void orc_svqb_sweep(int64_t M, int64_t K, double* X, int64_t ldx);
+void orc_gram_error(int64_t M, int64_t K, const double* X, int64_t ldx,
+                    double* fro, double* maxabs);
+
 #ifdef __cplusplus
 }
 #endif

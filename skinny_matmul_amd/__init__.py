@@ -59,6 +59,11 @@ class _FComplex(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float)]
 
 
+class SvqbBlock(C.Structure):
+    """skinny_svqb_block: X is K x M row-major with leading dimension ldx."""
+    _fields_ = [("M", C.c_int64), ("K", C.c_int64), ("X", C.c_void_p), ("ldx", C.c_int64)]
+
+
 def library_path() -> str:
     return _LIB_PATH
 
@@ -91,6 +96,11 @@ _SIGS = {
     "skinny_fill_uniform": (C.c_int, [vp, i64, u64, u64, i64, vp]),
     "skinny_fill_uniform_f": (C.c_int, [vp, i64, u64, u64, i64, vp]),
     "skinny_fill_int": (C.c_int, [vp, i64, u64, u64, i64, C.c_int, C.c_int, vp]),
+    "skinny_dsvqb": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
+    "skinny_dsvqb_graph": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, C.POINTER(vp)]),
+    "skinny_graph_launch": (C.c_int, [vp, vp]),
+    "skinny_graph_destroy": (C.c_int, [vp]),
+    "skinny_dorth_error": (C.c_int, [vp, i64, i64, vp, i64, C.POINTER(C.c_double), vp]),
     "skinny_mb_stream": (C.c_int, [C.c_int, sz, C.c_int, C.POINTER(C.c_double), vp]),
     "skinny_mb_peak": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double), vp]),
 }
@@ -322,6 +332,54 @@ def fill_uniform_f(out, n, seed, stream_id, offset=0, stream=None) -> None:
 def fill_int(out, n, seed, stream_id, lo=-2, hi=2, offset=0, stream=None) -> None:
     _check(lib().skinny_fill_int(_addr(out), n, seed, stream_id, offset, lo, hi,
                                  _stream(stream)), False)
+
+
+def _blocks(blocks):
+    """[(M, K, X, ldx), ...] -> ctypes array of skinny_svqb_block"""
+    arr = (SvqbBlock * len(blocks))()
+    for i, (M, K, X, ldx) in enumerate(blocks):
+        arr[i] = SvqbBlock(M, K, _addr(X), ldx)
+    return arr
+
+
+def dsvqb(blocks, sweeps=2, handle=None, stream=None) -> None:
+    """SVQB sweeps over blocks [(M, K, X, ldx), ...], in place (svqb.cu:72-130)."""
+    arr = _blocks(blocks)
+    _check(lib().skinny_dsvqb(_h(handle), len(blocks), C.cast(arr, vp), sweeps,
+                              _stream(stream)), False)
+
+
+class Graph:
+    """An executable hipGraph of SVQB sweeps (skinny_dsvqb_graph)."""
+
+    def __init__(self, blocks, sweeps=2, stream=None):
+        arr = _blocks(blocks)
+        g = vp()
+        _check(lib().skinny_dsvqb_graph(0, len(blocks), C.cast(arr, vp), sweeps,
+                                        _stream(stream), C.byref(g)), False)
+        self._g = g.value
+
+    def launch(self, stream=None) -> None:
+        _check(lib().skinny_graph_launch(self._g, _stream(stream)), False)
+
+    def close(self) -> None:
+        if self._g:
+            lib().skinny_graph_destroy(self._g)
+            self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def orth_error(M, K, X, ldx, handle=None, stream=None) -> float:
+    """||X^T X - I||_F with a double-double Gram (svqb.cu:184-211)."""
+    out = C.c_double()
+    _check(lib().skinny_dorth_error(_h(handle), M, K, _addr(X), ldx, C.byref(out),
+                                    _stream(stream)), False)
+    return out.value
 
 
 STREAM_KINDS = {"read": 0, "copy": 1, "triad": 2, "write": 3, "update": 4}

@@ -32,6 +32,7 @@ SOURCES = [
     "tsmm_lds.hip",
     "capi.hip",
     "microbench.hip",
+    "svqb.hip",
 ]
 
 CFLAGS = [
