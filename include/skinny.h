@@ -193,6 +193,35 @@ skinny_status_t skinny_fill_int(double* out, int64_t n, uint64_t seed,
                                 uint64_t stream_id, int64_t offset, int lo,
                                 int hi, skinny_stream_t stream);
 
+/* ---- persistent batched products --------------------------------------- */
+/* Replaces multi_dispatch.cuh:3-90 (one launch per compile-time (M, N)) with
+ * one persistent launch over a work queue of row ranges of many independent
+ * FP64 products of uneven (K, M, N), 1 <= M, N <= 64.  Ops must not read
+ * each other's outputs.  TSMM ops may be in place (C == A, lda == ldc). */
+typedef struct {
+  int op; /* 0: TSMTTSM  C = alpha A^T B + beta C;  1: TSMM  C = alpha A B + beta C */
+  int64_t M, N, K;
+  const double* A;
+  int64_t lda;
+  double alpha;
+  const double* B;
+  int64_t ldb;
+  double beta;
+  double* C;
+  int64_t ldc;
+} skinny_batch_op;
+typedef struct skinny_batch_s* skinny_batch_t;
+/* Plan (allocates and uploads the queue; item_bytes <= 0 picks 8 MB items).
+ * TSMTTSM results are reproducible: each op's partials are summed in item
+ * order whatever the schedule. */
+skinny_status_t skinny_dbatch_create(skinny_handle_t h, int nops,
+                                     const skinny_batch_op* ops,
+                                     int64_t item_bytes, skinny_batch_t* batch);
+/* One launch; no allocation, memset or host synchronisation (graph-safe). */
+skinny_status_t skinny_dbatch_run(skinny_batch_t batch, skinny_stream_t stream);
+skinny_status_t skinny_dbatch_destroy(skinny_batch_t batch);
+int64_t skinny_dbatch_items(skinny_batch_t batch);
+
 /* ---- SVQB orthogonalisation (svqb/svqb.cu:72-130) ---------------------- */
 /* One block: X is K x M row-major (ldx >= M), M <= 64, updated in place. */
 typedef struct {

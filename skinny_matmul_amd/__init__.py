@@ -59,6 +59,14 @@ class _FComplex(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float)]
 
 
+class BatchOp(C.Structure):
+    """skinny_batch_op (op 0: TSMTTSM, 1: TSMM)."""
+    _fields_ = [("op", C.c_int), ("M", C.c_int64), ("N", C.c_int64), ("K", C.c_int64),
+                ("A", C.c_void_p), ("lda", C.c_int64), ("alpha", C.c_double),
+                ("B", C.c_void_p), ("ldb", C.c_int64), ("beta", C.c_double),
+                ("C", C.c_void_p), ("ldc", C.c_int64)]
+
+
 class SvqbBlock(C.Structure):
     """skinny_svqb_block: X is K x M row-major with leading dimension ldx."""
     _fields_ = [("M", C.c_int64), ("K", C.c_int64), ("X", C.c_void_p), ("ldx", C.c_int64)]
@@ -96,6 +104,10 @@ _SIGS = {
     "skinny_fill_uniform": (C.c_int, [vp, i64, u64, u64, i64, vp]),
     "skinny_fill_uniform_f": (C.c_int, [vp, i64, u64, u64, i64, vp]),
     "skinny_fill_int": (C.c_int, [vp, i64, u64, u64, i64, C.c_int, C.c_int, vp]),
+    "skinny_dbatch_create": (C.c_int, [vp, C.c_int, vp, i64, C.POINTER(vp)]),
+    "skinny_dbatch_run": (C.c_int, [vp, vp]),
+    "skinny_dbatch_destroy": (C.c_int, [vp]),
+    "skinny_dbatch_items": (i64, [vp]),
     "skinny_dsvqb": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
     "skinny_dsvqb_graph": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, C.POINTER(vp)]),
     "skinny_graph_launch": (C.c_int, [vp, vp]),
@@ -332,6 +344,43 @@ def fill_uniform_f(out, n, seed, stream_id, offset=0, stream=None) -> None:
 def fill_int(out, n, seed, stream_id, lo=-2, hi=2, offset=0, stream=None) -> None:
     _check(lib().skinny_fill_int(_addr(out), n, seed, stream_id, offset, lo, hi,
                                  _stream(stream)), False)
+
+
+class Batch:
+    """A persistent batched launch over many independent FP64 products.
+
+    ops: iterable of (op, M, N, K, A, lda, alpha, B, ldb, beta, C, ldc) with
+    op "tsmttsm" or "tsmm" (or 0 / 1)."""
+
+    def __init__(self, ops, item_bytes=0, handle=None):
+        ops = list(ops)
+        arr = (BatchOp * len(ops))()
+        for i, (op, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc) in enumerate(ops):
+            kind = {"tsmttsm": 0, "tsmm": 1}.get(op, op)
+            arr[i] = BatchOp(kind, M, N, K, _addr(A), lda, alpha, _addr(B), ldb, beta,
+                             _addr(Cm), ldc)
+        b = vp()
+        _check(lib().skinny_dbatch_create(_h(handle), len(ops), C.cast(arr, vp), item_bytes,
+                                          C.byref(b)), False)
+        self._b = b.value
+
+    @property
+    def items(self) -> int:
+        return int(lib().skinny_dbatch_items(self._b))
+
+    def run(self, stream=None) -> None:
+        _check(lib().skinny_dbatch_run(self._b, _stream(stream)), False)
+
+    def close(self) -> None:
+        if self._b:
+            lib().skinny_dbatch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _blocks(blocks):

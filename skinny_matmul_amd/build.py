@@ -33,6 +33,7 @@ SOURCES = [
     "capi.hip",
     "microbench.hip",
     "svqb.hip",
+    "batch.hip",
 ]
 
 CFLAGS = [

@@ -55,6 +55,7 @@ struct TsmttsmPlan {
   int GS = 1;         // first-level reduction group size
   int ngroups = 1;
   int outM = 1, outN = 1;  // entries of one partial (outM x outN)
+  int uvar = 0;            // unroll variant: 0 base, 1 doubled
   size_t scratch_bytes = 0;
 };
 

@@ -24,6 +24,7 @@
 //    are reset by their last user, so no memset is needed between calls.
 #include "internal.hpp"
 #include "mfma.hpp"
+#include "tsmttsm_core.hpp"
 
 namespace skinny {
 
@@ -137,20 +138,14 @@ __device__ void reduce_partials(const TsArgs<T>& a, const T* src, int count,
   }
 }
 
-template <typename T, int P, int MT, int NT, bool SELF>
+template <typename T, int P, int MT, int NT, int U, bool SELF>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu((MT * NT >= 9) ? 2 : 1)))
     tsmttsm_kernel(TsArgs<T> a) {
-  using V4 = typename Mfma<T>::v4;
-  constexpr int U = Unroll<MT, NT, P>::value;
-  constexpr int RS = 4 * P;   // rows per MFMA step
-  constexpr int CH = RS * U;  // rows per chunk
-  constexpr int DM = 16 * MT, DN = 16 * NT;
-  constexpr int NACC = MT * NT * 4;
-
-  const int lane = threadIdx.x & 63;
+  using Core = TsCore<T, P, MT, NT, U, SELF>;
+  using V4 = typename Core::V4;
+  constexpr int DM = Core::DM, DN = Core::DN;
   const int wave = threadIdx.x >> 6;
-  const int li = lane & 15, lg = lane >> 4;
 
   const int ntiles = a.tilesM * a.tilesN;
   const int tile = blockIdx.x % ntiles;
@@ -160,162 +155,15 @@ __global__ void __launch_bounds__(256)
   const int Mloc = min(a.M - m0, DM);
   const int Nloc = min(a.N - n0, DN);
 
-  // lane -> (row within step, column) of its A and B operands
-  int rA = 0, cA = li, rB = 0, cB = li;
-  bool vA = true, vB = true;
-  if (P > 1) {
-    rA = li / a.M;
-    cA = li % a.M;
-    vA = rA < P;
-    rB = li / a.N;
-    cB = li % a.N;
-    vB = rB < P;
-  }
-  const int rowA = rA + P * lg;
-  const int rowB = rB + P * lg;
-  bool va[MT], vb[NT];
-#pragma unroll
-  for (int t = 0; t < MT; t++) va[t] = vA && (P > 1 || 16 * t + li < Mloc);
-#pragma unroll
-  for (int u = 0; u < NT; u++) vb[u] = vB && (P > 1 || 16 * u + li < Nloc);
-
-  const T* Ab = a.A + (int64_t)rowA * a.lda + m0 + cA;
-  const T* Bb = a.B + (int64_t)rowB * a.ldb + n0 + cB;
-  const int64_t stepA = (int64_t)RS * a.lda;
-  const int64_t stepB = (int64_t)RS * a.ldb;
-
   V4 acc[MT][NT];
-#pragma unroll
-  for (int t = 0; t < MT; t++)
-#pragma unroll
-    for (int u = 0; u < NT; u++) acc[t][u] = V4{0, 0, 0, 0};
+  Core::zero(acc);
+  Core::accumulate(a.A, a.lda, a.B, a.ldb, a.M, a.N, m0, n0, Mloc, Nloc, a.K,
+                   (int64_t)kp * kWaves + wave, (int64_t)a.G * kWaves, acc);
 
-  const int64_t K = a.K;
-  const int64_t nfull = K / CH;
-  const int64_t wstride = (int64_t)a.G * kWaves;
-  const int64_t w0 = (int64_t)kp * kWaves + wave;
-
-  for (int64_t c = w0; c < nfull; c += wstride) {
-    const T* pa = Ab + c * CH * a.lda;
-    const T* pb = Bb + c * CH * a.ldb;
-    T av[U][MT], bv[U][NT];
-#pragma unroll
-    for (int s = 0; s < U; s++) {
-#pragma unroll
-      for (int t = 0; t < MT; t++)
-        av[s][t] = va[t] ? pa[s * stepA + 16 * t] : T(0);
-      if (!SELF) {
-#pragma unroll
-        for (int u = 0; u < NT; u++)
-          bv[s][u] = vb[u] ? pb[s * stepB + 16 * u] : T(0);
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < U; s++)
-#pragma unroll
-      for (int t = 0; t < MT; t++)
-#pragma unroll
-        for (int u = 0; u < NT; u++)
-          acc[t][u] = Mfma<T>::mac(av[s][t], SELF ? av[s][u] : bv[s][u],
-                                   acc[t][u]);
-  }
-
-  // ragged tail: the one partial chunk, taken by the wave whose turn it is
-  if (nfull * CH < K && (nfull % wstride) == w0) {
-    const int64_t r0 = nfull * CH;
-#pragma unroll
-    for (int s = 0; s < U; s++) {
-      const bool okA = r0 + s * RS + rowA < K;
-      const bool okB = r0 + s * RS + rowB < K;
-      T av[MT], bv[NT];
-#pragma unroll
-      for (int t = 0; t < MT; t++)
-        av[t] = (va[t] && okA) ? Ab[(r0 + s * RS) * a.lda + 16 * t] : T(0);
-#pragma unroll
-      for (int u = 0; u < NT; u++) {
-        if (SELF)
-          bv[u] = av[u];
-        else
-          bv[u] = (vb[u] && okB) ? Bb[(r0 + s * RS) * a.ldb + 16 * u] : T(0);
-      }
-#pragma unroll
-      for (int t = 0; t < MT; t++)
-#pragma unroll
-        for (int u = 0; u < NT; u++)
-          acc[t][u] = Mfma<T>::mac(av[t], bv[u], acc[t][u]);
-    }
-  }
-
-  // ---- workgroup reduction through LDS: ((w0 + w2) + (w1 + w3)) ----------
-  __shared__ T red[2][NACC][64];
-  if (wave >= 2) {
-#pragma unroll
-    for (int t = 0; t < MT; t++)
-#pragma unroll
-      for (int u = 0; u < NT; u++)
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-          red[wave - 2][(t * NT + u) * 4 + v][lane] = acc[t][u][v];
-  }
-  __syncthreads();
-  if (wave < 2) {
-#pragma unroll
-    for (int t = 0; t < MT; t++)
-#pragma unroll
-      for (int u = 0; u < NT; u++)
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-          acc[t][u][v] += red[wave][(t * NT + u) * 4 + v][lane];
-  }
-  __syncthreads();
-  if (wave == 1) {
-#pragma unroll
-    for (int t = 0; t < MT; t++)
-#pragma unroll
-      for (int u = 0; u < NT; u++)
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-          red[0][(t * NT + u) * 4 + v][lane] = acc[t][u][v];
-  }
-  __syncthreads();
-  if (wave == 0) {
-#pragma unroll
-    for (int t = 0; t < MT; t++)
-#pragma unroll
-      for (int u = 0; u < NT; u++)
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-          acc[t][u][v] += red[0][(t * NT + u) * 4 + v][lane];
-  }
-  __syncthreads();
-  // D tile as a DM x DN row-major matrix (MFMA D layout: lane l, element v
-  // holds D[drow(l / 16, v)][l % 16])
-  T* Dm = &red[1][0][0];
-  if (wave == 0) {
-#pragma unroll
-    for (int t = 0; t < MT; t++)
-#pragma unroll
-      for (int u = 0; u < NT; u++)
-#pragma unroll
-        for (int v = 0; v < 4; v++)
-          Dm[(16 * t + Mfma<T>::drow(lg, v)) * DN + 16 * u + li] = acc[t][u][v];
-  }
-  __syncthreads();
-
+  __shared__ T red[Core::RED_ELEMS];
   const int PS = a.outM * a.outN;
   T* mine = a.part1 + ((int64_t)tile * a.G + kp) * PS;
-  for (int e = threadIdx.x; e < PS; e += blockDim.x) {
-    int ml = e % a.outM, nl = e / a.outM;
-    T s;
-    if (P > 1) {
-      s = T(0);
-#pragma unroll
-      for (int r = 0; r < P; r++) s += Dm[(r * a.M + ml) * DN + r * a.N + nl];
-    } else {
-      s = Dm[ml * DN + nl];
-    }
-    mine[e] = s;
-  }
+  Core::wg_partial(acc, red, mine, a.M, a.N, a.outM, a.outN);
 
   // ---- grid reduction: last arrival per group, then last group ----------
   // Arrival publishes this workgroup's partial with a release fence at agent
@@ -430,28 +278,35 @@ TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
 }
 
 template <typename T, int P, int MT, int NT, bool SELF>
-static void launch_one(const TsArgs<T>& a, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL((tsmttsm_kernel<T, P, MT, NT, SELF>), dim3(blocks),
-                     dim3(256), 0, s, a);
+static void launch_one(const TsArgs<T>& a, int blocks, int uvar,
+                       hipStream_t s) {
+  constexpr int U0 = Unroll<MT, NT, P>::value;
+  if (uvar == 0)
+    hipLaunchKernelGGL((tsmttsm_kernel<T, P, MT, NT, U0, SELF>), dim3(blocks),
+                       dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((tsmttsm_kernel<T, P, MT, NT, 2 * U0, SELF>),
+                       dim3(blocks), dim3(256), 0, s, a);
 }
 
 template <typename T, bool SELF>
 static bool dispatch(const TsmttsmPlan& p, const TsArgs<T>& a, int blocks,
                      hipStream_t s) {
+  const int uv = p.uvar;
   if (p.P > 1) {
     switch (p.P) {
-      case 16: launch_one<T, 16, 1, 1, SELF>(a, blocks, s); return true;
-      case 8: launch_one<T, 8, 1, 1, SELF>(a, blocks, s); return true;
-      case 5: launch_one<T, 5, 1, 1, SELF>(a, blocks, s); return true;
-      case 4: launch_one<T, 4, 1, 1, SELF>(a, blocks, s); return true;
-      case 3: launch_one<T, 3, 1, 1, SELF>(a, blocks, s); return true;
-      case 2: launch_one<T, 2, 1, 1, SELF>(a, blocks, s); return true;
+      case 16: launch_one<T, 16, 1, 1, SELF>(a, blocks, uv, s); return true;
+      case 8: launch_one<T, 8, 1, 1, SELF>(a, blocks, uv, s); return true;
+      case 5: launch_one<T, 5, 1, 1, SELF>(a, blocks, uv, s); return true;
+      case 4: launch_one<T, 4, 1, 1, SELF>(a, blocks, uv, s); return true;
+      case 3: launch_one<T, 3, 1, 1, SELF>(a, blocks, uv, s); return true;
+      case 2: launch_one<T, 2, 1, 1, SELF>(a, blocks, uv, s); return true;
       default: return false;
     }
   }
 #define SK_CASE(mt, nt)                                   \
   case (mt)*8 + (nt):                                     \
-    launch_one<T, 1, mt, nt, SELF>(a, blocks, s);         \
+    launch_one<T, 1, mt, nt, SELF>(a, blocks, uv, s);     \
     return true;
   switch (p.MT * 8 + p.NT) {
     SK_CASE(1, 1) SK_CASE(1, 2) SK_CASE(1, 3) SK_CASE(1, 4)
