@@ -98,6 +98,8 @@ def main() -> int:
     ap.add_argument("--logK", type=int, default=26)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the untimed autotuning of each shape")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -140,6 +142,12 @@ def main() -> int:
         assert ok, (op, M)
 
     step_bytes = sum(op_bytes(op, M, M, K) for op, M in ops)
+    tuned = {}
+    if not args.no_tune:
+        # untimed: the library's autotuner picks unroll and grid per shape
+        for op, M in ops:
+            tuned[f"{op}_{M}"] = round(sk.autotune(op, M, M, K, reps=3), 3)
+        log(f"autotuned: {sk.tune_export().strip()}")
     for _ in range(args.warmup):
         for op, M in ops:
             run(op, M)
@@ -198,7 +206,8 @@ def main() -> int:
         "data": "synthetic U[-1,1) from a counter RNG (seed 1), generated in HBM",
         "config": {
             "workload": (f"TSMTTSM C=A^T B then TSMM C=A B, fp64, M=N in {list(WIDTHS)}, "
-                         f"K=2^{args.logK}, alpha=1, beta=0, out of place, tuned blockCount"),
+                         f"K=2^{args.logK}, alpha=1, beta=0, out of place, "
+                         f"{'default' if args.no_tune else 'autotuned'} blockCount"),
             "K": K,
             "widths": list(WIDTHS),
             "calls_per_step": len(ops),

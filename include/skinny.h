@@ -193,6 +193,22 @@ skinny_status_t skinny_fill_int(double* out, int64_t n, uint64_t seed,
                                 uint64_t stream_id, int64_t offset, int lo,
                                 int hi, skinny_stream_t stream);
 
+/* ---- autotuner ----------------------------------------------------------- */
+/* Time the unroll variants and grid sizes (0.5 .. 8 workgroups per CU) of
+ * one FP64 shape class (op 0: TSMTTSM, 1: TSMM; dtype 0: double; M, N <= 64;
+ * K class = floor(log2 K)) on scratch data and remember the fastest in the
+ * handle: later calls with blockCount == 0 use it.  Replaces the hand-fitted
+ * blends (tsmm/fix_blend.cuh:13-28, varip_blend.cuh:13-26) and perf.cu's
+ * per-shape grid sweep (perf.cu:186-204).  Synchronous. */
+skinny_status_t skinny_autotune(skinny_handle_t h, int op, int dtype,
+                                int64_t M, int64_t N, int64_t K, int reps,
+                                double* best_ms);
+/* The table as text lines "op dtype M N log2K unroll_variant blockCount";
+ * returns the size needed including the terminating NUL. */
+size_t skinny_tune_export(skinny_handle_t h, char* buf, size_t len);
+skinny_status_t skinny_tune_import(skinny_handle_t h, const char* text);
+void skinny_tune_clear(skinny_handle_t h);
+
 /* ---- persistent batched products --------------------------------------- */
 /* Replaces multi_dispatch.cuh:3-90 (one launch per compile-time (M, N)) with
  * one persistent launch over a work queue of row ranges of many independent

@@ -104,6 +104,10 @@ _SIGS = {
     "skinny_fill_uniform": (C.c_int, [vp, i64, u64, u64, i64, vp]),
     "skinny_fill_uniform_f": (C.c_int, [vp, i64, u64, u64, i64, vp]),
     "skinny_fill_int": (C.c_int, [vp, i64, u64, u64, i64, C.c_int, C.c_int, vp]),
+    "skinny_autotune": (C.c_int, [vp, C.c_int, C.c_int, i64, i64, i64, C.c_int, C.POINTER(C.c_double)]),
+    "skinny_tune_export": (sz, [vp, C.c_char_p, sz]),
+    "skinny_tune_import": (C.c_int, [vp, C.c_char_p]),
+    "skinny_tune_clear": (None, [vp]),
     "skinny_dbatch_create": (C.c_int, [vp, C.c_int, vp, i64, C.POINTER(vp)]),
     "skinny_dbatch_run": (C.c_int, [vp, vp]),
     "skinny_dbatch_destroy": (C.c_int, [vp]),
@@ -344,6 +348,29 @@ def fill_uniform_f(out, n, seed, stream_id, offset=0, stream=None) -> None:
 def fill_int(out, n, seed, stream_id, lo=-2, hi=2, offset=0, stream=None) -> None:
     _check(lib().skinny_fill_int(_addr(out), n, seed, stream_id, offset, lo, hi,
                                  _stream(stream)), False)
+
+
+def autotune(op, M, N, K, reps=3, handle=None) -> float:
+    """Tune one FP64 shape class; returns the best time in ms."""
+    kind = {"tsmttsm": 0, "tsmm": 1}.get(op, op)
+    out = C.c_double()
+    _check(lib().skinny_autotune(_h(handle), kind, 0, M, N, K, reps, C.byref(out)), False)
+    return out.value
+
+
+def tune_export(handle=None) -> str:
+    n = lib().skinny_tune_export(_h(handle), None, 0)
+    buf = C.create_string_buffer(n)
+    lib().skinny_tune_export(_h(handle), buf, n)
+    return buf.value.decode()
+
+
+def tune_import(text: str, handle=None) -> None:
+    _check(lib().skinny_tune_import(_h(handle), text.encode()), False)
+
+
+def tune_clear(handle=None) -> None:
+    lib().skinny_tune_clear(_h(handle))
 
 
 class Batch:

@@ -34,6 +34,7 @@ SOURCES = [
     "microbench.hip",
     "svqb.hip",
     "batch.hip",
+    "tune.hip",
 ]
 
 CFLAGS = [

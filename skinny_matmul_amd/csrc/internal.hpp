@@ -5,10 +5,26 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 
 #include "skinny.h"
+
+namespace skinny {
+struct TuneKey {
+  int op, dtype, M, N, kclass;  // kclass = floor(log2 K)
+  bool operator<(const TuneKey& o) const {
+    return std::tie(op, dtype, M, N, kclass) <
+           std::tie(o.op, o.dtype, o.M, o.N, o.kclass);
+  }
+};
+struct TuneEntry {
+  int uvar;  // unroll variant (TSMTTSM)
+  int G;     // workgroups along K
+};
+}  // namespace skinny
 
 struct skinny_handle_s {
   int device = 0;
@@ -18,6 +34,9 @@ struct skinny_handle_s {
   void* ws = nullptr;
   size_t ws_bytes = 0;
   std::mutex mu;
+  // autotuned choices for blockCount == 0
+  std::map<skinny::TuneKey, skinny::TuneEntry> tune;
+  std::mutex tune_mu;
 };
 
 namespace skinny {
@@ -44,6 +63,10 @@ inline void* ws_scratch(skinny_handle_t h) {
 inline skinny_handle_t resolve(skinny_handle_t h) {
   return h ? h : skinny_default_handle();
 }
+
+// autotuner table lookup (op 0 TSMTTSM, 1 TSMM; dtype 0 double)
+bool tune_lookup(skinny_handle_t h, int op, int dtype, int64_t M, int64_t N,
+                 int64_t K, TuneEntry* out);
 
 // ---- TSMTTSM ------------------------------------------------------------
 struct TsmttsmPlan {

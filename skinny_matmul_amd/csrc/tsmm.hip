@@ -296,8 +296,8 @@ static void launch_mfma_s(const TmArgs<T>& a, int blocks, int nt, bool beta0,
 
 int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
                       int elem_bytes) {
-  (void)N;
-  (void)elem_bytes;
+  TuneEntry te;
+  if (elem_bytes == 8 && tune_lookup(h, 1, 0, M, N, K, &te)) return te.G;
   // measured on MI355X at K = 2^26 (fp64): the one-row-per-thread kernel for
   // M = 1 wants four workgroups per CU, everything else two
   int64_t want = (K + 255) / 256;

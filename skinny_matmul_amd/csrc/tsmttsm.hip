@@ -61,25 +61,6 @@ __device__ __forceinline__ T epilogue(T s, T alpha, T beta, const T* c) {
   return r;
 }
 
-template <typename T>
-__device__ __forceinline__ T sum_entry(const T* src, int count, int PS, int e) {
-  T s = T(0);
-  const T* p = src + e;
-  int q = 0;
-  for (; q + 4 <= count; q += 4) {
-    T v0 = p[(int64_t)(q + 0) * PS];
-    T v1 = p[(int64_t)(q + 1) * PS];
-    T v2 = p[(int64_t)(q + 2) * PS];
-    T v3 = p[(int64_t)(q + 3) * PS];
-    s += v0;
-    s += v1;
-    s += v2;
-    s += v3;
-  }
-  for (; q < count; q++) s += p[(int64_t)q * PS];
-  return s;
-}
-
 // Complex result from the interleaved-real product R = A_r^T B_r:
 //   C(m, n) = alpha * [(R00 - R11) + i (R01 + R10)] + beta * C(m, n)
 // with Rab = R(2m + a, 2n + b); operation order of cu_complex.h (scale, then
@@ -103,37 +84,83 @@ __device__ __forceinline__ void complex_epilogue(const TsArgs<T>& a, T r00,
   c[1] = o_im;
 }
 
-// Sum `count` partials of PS entries starting at src, in order, into dst or
-// (final) into C through the epilogue.
-template <typename T, int P>
-__device__ void reduce_partials(const TsArgs<T>& a, const T* src, int count,
-                                int PS, T* dst, bool final, int m0, int n0,
-                                int Mloc, int Nloc) {
-  if (final && a.cplx) {
+// Sum `count` partials of PS entries (src[q * PS + e]) with the whole
+// workgroup: entry e is summed by a team of TPE threads (thread j of the team
+// takes partials j, j + TPE, ...; the team is then combined by a fixed LDS
+// tree), so a small output pays one round trip instead of `count`.  The order
+// depends only on (PS, count): results are reproducible.  The sums go to
+// dst (group level) or to `fin` in LDS (final level, PS <= ldsn / 2).
+template <typename T>
+__device__ void team_sum(const T* src, int count, int PS, T* dst, T* lds,
+                         int ldsn, bool to_lds) {
+  int pad = 1;
+  while (pad < PS) pad <<= 1;
+  int tpe = 256 / pad;
+  if (tpe < 1) tpe = 1;
+  int cpow = 1;
+  while (cpow < count) cpow <<= 1;
+  if (tpe > cpow) tpe = cpow;
+  const int per_pass = 256 / tpe;
+  T* tree = lds;
+  T* fin = lds + ldsn / 2;
+  const int tid = threadIdx.x;
+  const int j = tid % tpe;
+  for (int e0 = 0; e0 < PS; e0 += per_pass) {
+    const int e = e0 + tid / tpe;
+    T s = T(0);
+    if (e < PS) {
+      const T* p = src + e;
+      int q = j;
+      for (; q + 3 * tpe < count; q += 4 * tpe) {
+        const T v0 = p[(int64_t)q * PS];
+        const T v1 = p[(int64_t)(q + tpe) * PS];
+        const T v2 = p[(int64_t)(q + 2 * tpe) * PS];
+        const T v3 = p[(int64_t)(q + 3 * tpe) * PS];
+        s += v0;
+        s += v1;
+        s += v2;
+        s += v3;
+      }
+      for (; q < count; q += tpe) s += p[(int64_t)q * PS];
+    }
+    tree[tid] = s;
+    __syncthreads();
+    for (int w = tpe >> 1; w > 0; w >>= 1) {
+      if (j < w) tree[tid] += tree[tid + w];
+      __syncthreads();
+    }
+    if (j == 0 && e < PS) {
+      if (to_lds)
+        fin[e] = tree[tid];
+      else
+        dst[e] = tree[tid];
+    }
+    __syncthreads();
+  }
+}
+
+// Final epilogue from the summed entries in LDS (real or complex view).
+template <typename T>
+__device__ void final_epilogue(const TsArgs<T>& a, const T* fin, int m0,
+                               int n0, int Mloc, int Nloc) {
+  if (a.cplx) {
     const int oMc = a.outM / 2, oNc = a.outN / 2;
     for (int e = threadIdx.x; e < oMc * oNc; e += blockDim.x) {
       const int mc = e % oMc, nc = e / oMc;
       if (2 * mc >= Mloc || 2 * nc >= Nloc) continue;
       const int e00 = (2 * nc) * a.outM + 2 * mc;
-      const T r00 = sum_entry(src, count, PS, e00);
-      const T r10 = sum_entry(src, count, PS, e00 + 1);
-      const T r01 = sum_entry(src, count, PS, e00 + a.outM);
-      const T r11 = sum_entry(src, count, PS, e00 + a.outM + 1);
       T* c = a.C + 2 * ((int64_t)(n0 / 2 + nc) * a.ldc + (m0 / 2 + mc));
-      complex_epilogue(a, r00, r11, r01, r10, c);
+      complex_epilogue(a, fin[e00], fin[e00 + a.outM + 1], fin[e00 + a.outM],
+                       fin[e00 + 1], c);
     }
     return;
   }
+  const int PS = a.outM * a.outN;
   for (int e = threadIdx.x; e < PS; e += blockDim.x) {
-    const T s = sum_entry(src, count, PS, e);
-    if (!final) {
-      dst[e] = s;
-    } else {
-      int ml = e % a.outM, nl = e / a.outM;
-      if (ml < Mloc && nl < Nloc) {
-        T* c = a.C + (int64_t)(n0 + nl) * a.ldc + (m0 + ml);
-        *c = epilogue(s, a.alpha, a.beta, c);
-      }
+    const int ml = e % a.outM, nl = e / a.outM;
+    if (ml < Mloc && nl < Nloc) {
+      T* c = a.C + (int64_t)(n0 + nl) * a.ldc + (m0 + ml);
+      *c = epilogue(fin[e], a.alpha, a.beta, c);
     }
   }
 }
@@ -187,13 +214,15 @@ __global__ void __launch_bounds__(256)
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
   const T* gsrc = a.part1 + ((int64_t)tile * a.G + (int64_t)grp * a.GS) * PS;
+  constexpr int LDSN = TsCore<T, P, MT, NT, U, SELF>::RED_ELEMS;
   if (a.ngroups == 1) {
-    reduce_partials<T, P>(a, gsrc, gsz, PS, nullptr, true, m0, n0, Mloc, Nloc);
+    team_sum<T>(gsrc, gsz, PS, nullptr, red, LDSN, true);
+    final_epilogue<T>(a, red + LDSN / 2, m0, n0, Mloc, Nloc);
     if (threadIdx.x == 0) *cnt1 = 0u;
     return;
   }
   T* gdst = a.part2 + ((int64_t)tile * a.ngroups + grp) * PS;
-  reduce_partials<T, P>(a, gsrc, gsz, PS, gdst, false, m0, n0, Mloc, Nloc);
+  team_sum<T>(gsrc, gsz, PS, gdst, red, LDSN, false);
   if (threadIdx.x == 0) *cnt1 = 0u;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
@@ -205,8 +234,9 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   if (!s_last) return;
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  reduce_partials<T, P>(a, a.part2 + (int64_t)tile * a.ngroups * PS,
-                        a.ngroups, PS, nullptr, true, m0, n0, Mloc, Nloc);
+  team_sum<T>(a.part2 + (int64_t)tile * a.ngroups * PS, a.ngroups, PS, nullptr,
+              red, LDSN, true);
+  final_epilogue<T>(a, red + LDSN / 2, m0, n0, Mloc, Nloc);
   if (threadIdx.x == 0) *cnt2 = 0u;
 }
 
@@ -248,8 +278,12 @@ TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
   int U = p.P > 1 ? 8 : ((p.MT + p.NT) >= 8 ? 2 : ((p.MT + p.NT) >= 4 ? 4 : 8));
   int64_t CH = 4LL * p.P * U;
   int64_t G;
+  TuneEntry te;
   if (blockCount > 0) {
     G = (int64_t)blockCount;
+  } else if (elem_bytes == 8 && tune_lookup(h, 0, 0, M, N, K, &te)) {
+    G = te.G;
+    p.uvar = te.uvar;
   } else {
     // measured on MI355X at K = 2^26 (fp64): one workgroup per CU for tiles
     // up to 16 x 16, two for the 32 x 32 and 64 x 64 tiles
@@ -262,7 +296,8 @@ TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
   }
   if (G > 65536) G = 65536;
   p.G = (int)G;
-  if (p.G <= 16) {
+  // one level when each thread reads at most 8 partials, else two levels
+  if (p.G <= 16 || (int64_t)p.G * p.outM * p.outN <= 256 * 8) {
     p.GS = p.G;
     p.ngroups = 1;
   } else {
