@@ -126,6 +126,24 @@ skinny_status_t skinny_ctsmttsm(skinny_handle_t h, size_t blockCount,
                                 skinny_float_complex* C, int64_t ldc,
                                 skinny_stream_t stream);
 
+/* Ozaki split on the INT8 matrix cores (no reference counterpart): FP64
+ * inputs split into `slices` (3..5) signed 7-bit limbs against a running
+ * per-column power-of-two scale, exact INT32 accumulation per limb level,
+ * FP64 reconstruction when a column's scale grows or every 16384 rows.
+ * Relative error about 2^(-7 slices) of sum |A||B|; with blockCount == 1 the
+ * computation is one fixed sequence.  M, N <= 1024. */
+skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
+                                      int64_t M, int64_t N, int64_t K,
+                                      const double* A, int64_t lda,
+                                      double alpha, const double* B,
+                                      int64_t ldb, double beta, double* C,
+                                      int64_t ldc, int slices,
+                                      skinny_stream_t stream);
+/* Lane layout of the INT8 32x32x32 matrix-core result (diagnostic):
+ * host_out[lane * 16 + r] = 32 * row, host_out[1024 + lane * 16 + r] =
+ * 32 * column of result element r of lane `lane`.  Synchronous. */
+skinny_status_t skinny_probe_i8_layout(int* host_out);
+
 /* HR (types.hpp:49-57): double inputs and output, double-double (PseudoQuad)
  * accumulation through FMA TwoProd and the accurate dd addition; replaces the
  * iT = PseudoQuad instantiations (genv3.cuh:46-79, specsmall.cuh) and their

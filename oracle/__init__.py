@@ -71,6 +71,7 @@ def lib():
             "orc_xdot": (None, [i64, vp, i64, vp, i64, vp, vp]),
             "orc_xtsmttsm": (None, [i64, i64, i64, vp, i64, vp, i64, vp, vp, i64]),
             "orc_xtsmttsm_entries": (None, [i64, vp, i64, vp, i64, i64, vp, vp, vp, vp]),
+            "orc_ozaki_tsmttsm": (None, [i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, C.c_int]),
             "orc_jacobi_eig": (C.c_int, [i64, vp, i64, vp, vp, i64, C.c_int]),
             "orc_svqb_transform": (None, [i64, vp, i64, vp, i64]),
             "orc_svqb_sweep": (None, [i64, i64, vp, i64]),
@@ -298,4 +299,22 @@ def svqb_test_block(M, K, seed, cond_exp=6.0):
     W, _ = np.linalg.qr(rng.standard_normal((M, M)))
     sig = np.logspace(0, -cond_exp, M)
     X = (Q * sig) @ W.T
+    return np.ascontiguousarray(X).ravel()
+
+
+def ozaki_tsmttsm(M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc, slices):
+    """Ozaki-split TSMTTSM as the device runs it with blockCount = 1."""
+    lib().orc_ozaki_tsmttsm(M, N, K, _p(A), lda, float(alpha), _p(B), ldb, float(beta),
+                            _p(Cm), ldc, slices)
+    return Cm
+
+
+def c3_columns(M, K, seed):
+    """C3 inputs: column j = (v + 1e-8 u_j) 2^e_j, v and u_j ~ U[-1,1) from the
+    counter RNG, e_j integer in [-20, 20]; row-major K x M."""
+    v = uniform(K, seed, 0)
+    e = (integers(M, seed, 999, -20, 20)).astype(int)
+    X = np.empty((K, M))
+    for j in range(M):
+        X[:, j] = np.ldexp(v + 1e-8 * uniform(K, seed, 1 + j), int(e[j]))
     return np.ascontiguousarray(X).ravel()

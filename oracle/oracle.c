@@ -710,3 +710,123 @@ void orc_gram_error(int64_t M, int64_t K, const double* X, int64_t ldx,
   *fro = sqrt(l2);
   *maxabs = mx;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Ozaki split TSMTTSM, restated for blockCount = 1 (one workgroup per       */
+/* 128 x 128 tile walking all 32-row chunks in order): the bit-exact twin of */
+/* the device path.  Algorithm: Ozaki, Ogita, Oishi, Rump, "Error-free       */
+/* transformations of matrix multiplication by using fast routines of matrix */
+/* multiplication and its applications", Numer. Algorithms 59 (2012), with   */
+/* truncation-based 7-bit limbs and a running per-column scale.              */
+/* ------------------------------------------------------------------------ */
+#define OZ_TILE 128
+#define OZ_ROWS 32
+#define OZ_FLUSH 16384
+#define OZ_EMIN (-4000)
+
+static void oz_flush_tile(int S, int64_t* acc, double* cacc, const int* ea,
+                          const int* eb) {
+  for (int n = 0; n < OZ_TILE; n++)
+    for (int m = 0; m < OZ_TILE; m++) {
+      int64_t* a = acc + ((int64_t)n * OZ_TILE + m) * S;
+      if (ea[m] != OZ_EMIN && eb[n] != OZ_EMIN) {
+        double v = ldexp((double)a[S - 1], -7 * (S + 1));
+        for (int g = S; g >= 2; g--) v = v + ldexp((double)a[g - 2], -7 * g);
+        cacc[n * OZ_TILE + m] = cacc[n * OZ_TILE + m] + ldexp(v, ea[m] + eb[n]);
+      }
+      for (int g = 0; g < S; g++) a[g] = 0;
+    }
+}
+
+void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
+                       int64_t lda, double alpha, const double* B, int64_t ldb,
+                       double beta, double* C, int64_t ldc, int S) {
+  int64_t* acc = (int64_t*)malloc(sizeof(int64_t) * OZ_TILE * OZ_TILE * S);
+  double* cacc = (double*)malloc(sizeof(double) * OZ_TILE * OZ_TILE);
+  signed char* da = (signed char*)malloc((size_t)S * OZ_ROWS * OZ_TILE);
+  signed char* db = (signed char*)malloc((size_t)S * OZ_ROWS * OZ_TILE);
+  for (int64_t m0 = 0; m0 < M; m0 += OZ_TILE)
+    for (int64_t n0 = 0; n0 < N; n0 += OZ_TILE) {
+      int erun[2][OZ_TILE], ecur[2][OZ_TILE];
+      for (int c = 0; c < OZ_TILE; c++) erun[0][c] = erun[1][c] = OZ_EMIN;
+      memset(acc, 0, sizeof(int64_t) * OZ_TILE * OZ_TILE * S);
+      memset(cacc, 0, sizeof(double) * OZ_TILE * OZ_TILE);
+      int since = 0;
+      for (int64_t r0 = 0; r0 < K; r0 += OZ_ROWS) {
+        int flush = (since + OZ_ROWS > OZ_FLUSH);
+        for (int mat = 0; mat < 2; mat++)
+          for (int c = 0; c < OZ_TILE; c++) {
+            int64_t gc = (mat == 0 ? m0 : n0) + c;
+            int valid = gc < (mat == 0 ? M : N);
+            double mx = 0.0;
+            for (int i = 0; i < OZ_ROWS && valid; i++) {
+              int64_t r = r0 + i;
+              if (r >= K) break;
+              double x = mat == 0 ? A[r * lda + gc] : B[r * ldb + gc];
+              if (fabs(x) > mx) mx = fabs(x);
+            }
+            int e = OZ_EMIN;
+            if (mx > 0.0) frexp(mx, &e);
+            ecur[mat][c] = e;
+            if (e > erun[mat][c]) flush = 1;
+          }
+        if (flush) {
+          oz_flush_tile(S, acc, cacc, erun[0], erun[1]);
+          since = 0;
+          for (int mat = 0; mat < 2; mat++)
+            for (int c = 0; c < OZ_TILE; c++)
+              if (ecur[mat][c] > erun[mat][c]) erun[mat][c] = ecur[mat][c];
+        }
+        /* limbs */
+        for (int mat = 0; mat < 2; mat++) {
+          signed char* d = mat == 0 ? da : db;
+          for (int c = 0; c < OZ_TILE; c++) {
+            int64_t gc = (mat == 0 ? m0 : n0) + c;
+            int valid = gc < (mat == 0 ? M : N);
+            int E = erun[mat][c];
+            for (int i = 0; i < OZ_ROWS; i++) {
+              int64_t r = r0 + i;
+              double x = (valid && r < K) ? (mat == 0 ? A[r * lda + gc] : B[r * ldb + gc]) : 0.0;
+              double y = (E == OZ_EMIN) ? 0.0 : ldexp(x, -E);
+              for (int s2 = 0; s2 < S; s2++) {
+                y = y * 128.0;
+                double dd = trunc(y);
+                y = y - dd;
+                d[((size_t)s2 * OZ_TILE + c) * OZ_ROWS + i] = (signed char)(int)dd;
+              }
+            }
+          }
+        }
+        /* exact integer products per level g = s + t (1-based limbs) */
+        for (int g = 2; g <= S + 1; g++)
+          for (int s1 = 1; s1 <= S; s1++) {
+            int t1 = g - s1;
+            if (t1 < 1 || t1 > S) continue;
+            const signed char* pa = da + (size_t)(s1 - 1) * OZ_TILE * OZ_ROWS;
+            const signed char* pb = db + (size_t)(t1 - 1) * OZ_TILE * OZ_ROWS;
+            for (int n = 0; n < OZ_TILE; n++)
+              for (int m = 0; m < OZ_TILE; m++) {
+                int64_t sum = 0;
+                const signed char* x = pa + (size_t)m * OZ_ROWS;
+                const signed char* y = pb + (size_t)n * OZ_ROWS;
+                for (int i = 0; i < OZ_ROWS; i++) sum += (int64_t)x[i] * y[i];
+                acc[((int64_t)n * OZ_TILE + m) * S + (g - 2)] += sum;
+              }
+          }
+        since += OZ_ROWS;
+      }
+      oz_flush_tile(S, acc, cacc, erun[0], erun[1]);
+      for (int64_t n = 0; n < OZ_TILE && n0 + n < N; n++)
+        for (int64_t m = 0; m < OZ_TILE && m0 + m < M; m++) {
+          double sres = cacc[n * OZ_TILE + m];
+          double* c = &C[(n0 + n) * ldc + (m0 + m)];
+          double r = alpha * sres;
+          if (beta != 0.0) r = beta * *c + r;
+          *c = r;
+        }
+    }
+  free(acc);
+  free(cacc);
+  free(da);
+  free(db);
+}

@@ -120,6 +120,11 @@ void orc_xtsmttsm_entries(int64_t K, const double* A, int64_t lda,
                           const int64_t* ms, const int64_t* ns, double* hi,
                           double* lo);
 
+/* ---- Ozaki split (bit-exact twin of the device path at blockCount 1) ----- */
+void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
+                       int64_t lda, double alpha, const double* B, int64_t ldb,
+                       double beta, double* C, int64_t ldc, int slices);
+
 /* ---- SVQB restatement (svqb/svqb.cu:72-130) ----------------------------- */
 int orc_jacobi_eig(int64_t M, double* S, int64_t lds, double* w, double* V,
                    int64_t ldv, int max_sweeps);

@@ -99,6 +99,8 @@ _SIGS = {
     "skinny_ctsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, _FComplex, vp, i64, _FComplex, vp, i64, vp]),
     "skinny_ztsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, _DComplex, vp, i64, _DComplex, vp, i64, vp]),
     "skinny_ctsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, _FComplex, vp, i64, _FComplex, vp, i64, vp]),
+    "skinny_dtsmttsm_ozaki": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, C.c_int, vp]),
+    "skinny_probe_i8_layout": (C.c_int, [vp]),
     "skinny_htsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, vp]),
     "skinny_htsmttsm_dd": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, vp]),
     "skinny_fill_uniform": (C.c_int, [vp, i64, u64, u64, i64, vp]),
@@ -236,6 +238,22 @@ def stsmm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
     return _check(lib().skinny_stsmm(_h(handle), blockCount, M, N, K, _addr(A), lda,
                                      float(alpha), _addr(B), ldb, float(beta),
                                      _addr(Cm), ldc, _stream(stream)))
+
+
+def dtsmttsm_ozaki(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
+                   slices=4, handle=None, stream=None) -> bool:
+    """FP64 TSMTTSM through INT8 limbs on the matrix cores (Ozaki split)."""
+    return _check(lib().skinny_dtsmttsm_ozaki(_h(handle), blockCount, M, N, K, _addr(A), lda,
+                                              float(alpha), _addr(B), ldb, float(beta),
+                                              _addr(Cm), ldc, slices, _stream(stream)))
+
+
+def probe_i8_layout():
+    """(rows, cols) of each (lane, element) of the INT8 32x32x32 MFMA result."""
+    import numpy as np
+    out = np.zeros(2048, dtype=np.int32)
+    _check(lib().skinny_probe_i8_layout(out.ctypes.data_as(vp)), False)
+    return out[:1024].reshape(64, 16) // 32, out[1024:].reshape(64, 16) // 32
 
 
 def htsmttsm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,

@@ -35,6 +35,7 @@ SOURCES = [
     "svqb.hip",
     "batch.hip",
     "tune.hip",
+    "ozaki.hip",
 ]
 
 CFLAGS = [

@@ -41,9 +41,12 @@ struct skinny_handle_s {
 
 namespace skinny {
 
-// bytes at the start of the workspace reserved for arrival counters
+// bytes at the start of the workspace reserved for arrival counters; its
+// last kZeroBytes are never written and read as zeros (a zero row for loads
+// past the end of a matrix)
 constexpr size_t kCounterBytes = 64 * 1024;
-constexpr int kMaxCounters = kCounterBytes / sizeof(unsigned int);
+constexpr size_t kZeroBytes = 8 * 1024;
+constexpr int kMaxCounters = (kCounterBytes - kZeroBytes) / sizeof(unsigned int);
 
 void set_error(const std::string& msg);
 skinny_status_t fail(skinny_status_t s, const std::string& msg);
@@ -55,6 +58,9 @@ skinny_status_t ensure_workspace(skinny_handle_t h, size_t scratch_bytes);
 
 inline unsigned int* ws_counters(skinny_handle_t h) {
   return static_cast<unsigned int*>(h->ws);
+}
+inline const double* ws_zeros(skinny_handle_t h) {
+  return reinterpret_cast<const double*>(static_cast<char*>(h->ws) + kCounterBytes - kZeroBytes);
 }
 inline void* ws_scratch(skinny_handle_t h) {
   return static_cast<char*>(h->ws) + kCounterBytes;

@@ -1,0 +1,647 @@
+// Ozaki-split FP64 TSMTTSM on the INT8 matrix cores (no reference
+// counterpart; an alternative to the FP64 kernel for wide blocks).
+//
+// C = alpha A^T B + beta C with A (K x M), B (K x N) FP64.  The rows are taken
+// in chunks of 32 (one MFMA k-depth):
+//  * Each column c has a running scale exponent E_c: every |x| seen so far is
+//    below 2^E_c.  An element x is split into S signed 7-bit limbs,
+//      y = x 2^-E;  for s = 1..S: y = 128 y; d_s = trunc(y); y = y - d_s,
+//    so x = 2^E (sum_s d_s 2^-7s + r 2^-7S), |d_s| <= 127, |r| < 1; every
+//    step is exact in FP64 (Ozaki et al. 2012, error-free splitting).
+//  * For each limb-sum level g = s + t (2 .. S + 1; pairs with s + t > S + 1
+//    are dropped, as in the truncated Ozaki scheme) the products
+//    sum_k d_s(k, m) d'_t(k, n) accumulate exactly in INT32 through
+//    v_mfma_i32_32x32x32_i8, one accumulator per level.
+//  * The INT32 accumulators are converted to FP64 ("flushed") only when a
+//    column's magnitude grows past its scale (new E_c) or before INT32 could
+//    overflow (every 16384 rows); the flush forms
+//      V = sum_{g = S + 1 .. 2} acc_g 2^-7g   (in this order, FP64),
+//      C_acc(m, n) += V 2^(E_m + E_n).
+//    For data of steady magnitude the conversion cost is amortised over
+//    thousands of rows, so the INT8 units and HBM set the pace.
+//  * Workgroups (4 waves, one 64 x 64 region of C each, the level
+//    accumulators in the matrix-core registers) own a 128 x 128 tile of C
+//    and a grid-stride set of chunks; their FP64 partials are summed by the
+//    same two-level last-arrival reduction as the FP64 kernel.  With
+//    blockCount == 1 the whole computation is one fixed sequence, restated
+//    bit for bit on the host by the oracle.
+//  * Per chunk the loop is software-pipelined: the next chunk's loads are in
+//    flight while this chunk is split (VALU, packed 4 rows to a word) into
+//    one LDS buffer and the previous chunk's limbs in the other buffer are
+//    multiplied, the two interleaved instruction by instruction.
+#include <cmath>
+#include <utility>
+
+#include "internal.hpp"
+#include "team_sum.hpp"
+
+namespace skinny {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+typedef unsigned v2u __attribute__((ext_vector_type(2)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
+// Raw buffer resources: the per-access constant parts of an address go into
+// the scalar offset, so unrolled loops over rows and accumulator entries need
+// one vector address register instead of one 64-bit pointer per access.
+// Word 3 = 0x00020000 is the gfx9 raw-buffer format (32-bit data, no swizzle);
+// loads past num_records return 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t oz_rsrc(const void* p,
+                                                          int64_t bytes) {
+  const int n = bytes > 0x7fffffff ? 0x7fffffff : (int)bytes;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, n,
+                                           0x00020000);
+}
+
+__device__ __forceinline__ double oz_ld(__amdgpu_buffer_rsrc_t r, int voff,
+                                        int soff) {
+  return __builtin_bit_cast(double,
+                            __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+__device__ __forceinline__ void oz_st(double v, __amdgpu_buffer_rsrc_t r,
+                                      int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, voff,
+                                        soff, 0);
+}
+
+// 32x32x32 INT8 MFMA: lane l supplies 16 k-bytes of A row l % 32 (k half
+// l / 32) and of B column l % 32; element r of the result in lane l is
+// D[i8_drow(l, r)][l % 32].
+__device__ __forceinline__ int i8_drow(int lane, int r) {
+  return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+}
+
+constexpr int kOzTile = 128;
+constexpr int kOzRows = 32;
+constexpr int kOzThreads = 256;
+constexpr int kOzFlushRows = 16384;
+constexpr int kOzEmin = -4000;
+
+struct OzArgs {
+  const double* A;
+  const double* B;
+  double* C;
+  int64_t K, lda, ldb, ldc;
+  int M, N;
+  double alpha, beta;
+  double* part1;
+  double* part2;
+  int* stage;  // per wave: S * 4096 INT32 accumulator staging
+  const double* zeros;  // >= 1024 zero doubles
+  unsigned* cnt;
+  int G, GS, ngroups, tilesM, tilesN;
+};
+
+// Empty asm that requires the accumulators in the matrix-core (AGPR)
+// registers: placed at the top of the chunk loop it keeps the register
+// allocator from carrying them around the loop as VGPR copies (which the
+// flush path otherwise provokes, spilling them every iteration).
+template <int S>
+__device__ __forceinline__ void oz_pin(v16i (&acc)[S][4]) {
+#pragma unroll
+  for (int g = 0; g < S; g++)
+#pragma unroll
+    for (int tt = 0; tt < 4; tt++) asm volatile("" : "+a"(acc[g][tt]));
+}
+
+// Convert the INT32 level accumulators to FP64 and add them to this wave's
+// 64 x 64 region of the workgroup's partial tile in global memory (entry
+// (m, n) at n * 128 + m, owned by exactly one lane), then clear them.  The
+// accumulators go straight from the matrix-core registers to the wave's INT32
+// staging area and are converted from there one entry at a time, so the
+// conversion holds no copy of them.  `part` is the resource of the region's
+// first entry, `lofs` the lane's byte offset in it.
+template <int S>
+__device__ __forceinline__ void oz_flush(v16i (&acc)[S][4],
+                                         __amdgpu_buffer_rsrc_t stage,
+                                         __amdgpu_buffer_rsrc_t part, int lofs,
+                                         const int* ea_w, const int* eb_w,
+                                         int lane) {
+#pragma unroll
+  for (int g = 0; g < S; g++)
+#pragma unroll
+    for (int tt = 0; tt < 4; tt++) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const v4i w = {acc[g][tt][4 * q], acc[g][tt][4 * q + 1],
+                       acc[g][tt][4 * q + 2], acc[g][tt][4 * q + 3]};
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, w), stage,
+                                               16 * lane, 1024 * ((g * 4 + tt) * 4 + q), 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[g][tt][r] = 0;
+    }
+#pragma unroll 1
+  for (int e = 0; e < 64; e++) {
+    const int tt = e >> 4, r = e & 15;
+    const int dm = 32 * (tt >> 1) + i8_drow(lane, r);
+    const int dn = 32 * (tt & 1) + (lane & 31);
+    const int ea = ea_w[dm], eb = eb_w[dn];
+    if (ea != kOzEmin && eb != kOzEmin) {
+      const int lo = 16 * lane + 4 * (r & 3);
+      auto level = [&](int g) {
+        return (int)__builtin_amdgcn_raw_buffer_load_b32(
+            stage, lo, 1024 * ((g * 4 + tt) * 4 + (r >> 2)), 0);
+      };
+      double v = ldexp((double)level(S - 1), -7 * (S + 1));
+#pragma unroll
+      for (int g = S; g >= 2; g--) v = v + ldexp((double)level(g - 2), -7 * g);
+      const int so = 8 * (32 * (tt & 1) * kOzTile + 32 * (tt >> 1) +
+                          8 * (r >> 2) + (r & 3));
+      oz_st(oz_ld(part, lofs, so) + ldexp(v, ea + eb), part, lofs, so);
+    }
+  }
+}
+
+// Signed 7-bit limbs of x against scale 2^E: with q = floor(|x| 2^(7S - E))
+// (|x| < 2^E, so q < 2^(7S)), limb s is sign(x) * bits [7(S - s), 7(S - s) + 7)
+// of q: the same digits as the sequence y = 128 y; d = trunc(y); y -= d.
+template <int S>
+__device__ __forceinline__ void oz_limbs(double x, int E, int (&d)[S]) {
+  const double ax = fabs(x);
+  const bool neg = x < 0.0;
+  constexpr int S4 = S < 4 ? S : 4;
+  const double t = ldexp(ax, 7 * S4 - E);
+  const unsigned q = (unsigned)t;  // truncation; t < 2^28
+#pragma unroll
+  for (int s = 0; s < S4; s++) {
+    const int v = (int)((q >> (7 * (S4 - 1 - s))) & 127u);
+    d[s] = neg ? -v : v;
+  }
+  if (S > 4) {
+    const double rem = (t - (double)q) * 128.0;  // exact, in [0, 128)
+    const int v = (int)rem;
+    d[S - 1] = neg ? -v : v;
+  }
+}
+
+// Limb words of four consecutive rows of a column: w[s] holds limb s + 1 of
+// row j in byte j (two's complement), the digits of oz_limbs.  The bytes are
+// built unsigned and negated together: for a byte u <= 127,
+// ((0x80 - u) ^ 0x80) == -u mod 256 with no borrow into the next byte; the
+// sign bytes come from v_perm's sign-replicating selectors (9: bit 31 of
+// the second operand, 11: bit 31 of the first).
+//
+// Returns the largest q >> (7 min(S, 4)): nonzero iff some |x| >= 2^E (the
+// conversion saturates), i.e. iff a row outgrew the scale.
+template <int S>
+__device__ __forceinline__ unsigned oz_pack4(const double (&xv)[4], int E,
+                                             int (&w)[S]) {
+  constexpr int S4 = S < 4 ? S : 4;
+  unsigned q[4], r5[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const double t = ldexp(fabs(xv[j]), 7 * S4 - E);
+    q[j] = (unsigned)t;  // truncation; t < 2^28
+    r5[j] = S > 4 ? (unsigned)((t - (double)q[j]) * 128.0) : 0u;
+  }
+  const unsigned m =
+      __builtin_amdgcn_perm((unsigned)__double2hiint(xv[1]), (unsigned)__double2hiint(xv[0]), 0x0C0C0B09u) |
+      __builtin_amdgcn_perm((unsigned)__double2hiint(xv[3]), (unsigned)__double2hiint(xv[2]), 0x0B090C0Cu);
+#pragma unroll
+  for (int s = 0; s < S; s++) {
+    unsigned u = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const unsigned d = s < S4 ? (q[j] >> (7 * (S4 - 1 - s))) & 127u : r5[j];
+      u |= d << (8 * j);
+    }
+    const unsigned nb = (0x80808080u - u) ^ 0x80808080u;
+    w[s] = (int)((nb & m) | (u & ~m));
+  }
+  return max(max(q[0], q[1]), max(q[2], q[3])) >> (7 * S4);
+}
+
+// Calls f(integral_constant<int, I>) for I = 0 .. N - 1 in order: unrolled
+// at compile time, so indices into register arrays stay constant.
+template <class F, int... I>
+__device__ __forceinline__ void oz_static_for(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+
+// The k-th INT8 product of a chunk (tile tt, level g, limb s of A, limb
+// g - s of B), tile-major; 2 S (S + 1) products per chunk and wave.
+struct OzMf {
+  int tt, g, s;
+};
+template <int S>
+constexpr int oz_nmfma() {
+  return 2 * S * (S + 1);
+}
+template <int S>
+constexpr OzMf oz_mfma_of(int k) {
+  int n = 0;
+  for (int tt = 0; tt < 4; tt++)
+    for (int g = 2; g <= S + 1; g++)
+      for (int s = 1; s <= S; s++) {
+        const int t = g - s;
+        if (t < 1 || t > S) continue;
+        if (n == k) return OzMf{tt, g, s};
+        n++;
+      }
+  return OzMf{0, 0, 0};
+}
+
+template <int S>
+__global__ void __launch_bounds__(kOzThreads)
+    __attribute__((amdgpu_waves_per_eu(1, 1))) ozaki_kernel(OzArgs a) {
+  // limbs of two chunks (double buffer): [buffer][A/B][limb][column][row]
+  __shared__ __attribute__((aligned(16))) signed char sl[2][2][S][kOzTile][kOzRows];
+  __shared__ int erun[2][kOzTile];
+  __shared__ int s_last;
+  __shared__ double tree[2 * kOzThreads];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave >> 1, wn = wave & 1;  // this wave's 64 x 64 region
+  const int ntiles = a.tilesM * a.tilesN;
+  const int tile = blockIdx.x % ntiles;
+  const int kp = blockIdx.x / ntiles;
+  const int m0 = (tile % a.tilesM) * kOzTile;
+  const int n0 = (tile / a.tilesM) * kOzTile;
+
+  // waves 0-1 stream the 128 columns of A's tile, waves 2-3 those of B;
+  // thread `col` holds its column's 32 rows of the current chunk
+  const int mat = wave >> 1, col = tid & 127;
+  const int width = mat == 0 ? a.M : a.N;
+  const int gcol = (mat == 0 ? m0 : n0) + col;
+  const bool cvalid = gcol < width;
+  const int voff = 8 * (cvalid ? gcol : 0);
+  const double* mbase = mat == 0 ? a.A : a.B;
+  const int64_t ld = mat == 0 ? a.lda : a.ldb;
+  const int ld8 = (int)(8 * ld);
+
+  erun[mat][col] = kOzEmin;
+  v16i acc[S][4];
+#pragma unroll
+  for (int g = 0; g < S; g++)
+#pragma unroll
+    for (int tt = 0; tt < 4; tt++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[g][tt][r] = 0;
+  // this workgroup's FP64 partial tile, accumulated by the (rare) flushes
+  constexpr int PS = kOzTile * kOzTile;
+  double* mine = a.part1 + ((int64_t)tile * a.G + kp) * PS;
+  const __amdgpu_buffer_rsrc_t part =
+      oz_rsrc(mine + 64 * wn * kOzTile + 64 * wm,
+              8 * (int64_t)(PS - 64 * wn * kOzTile - 64 * wm));
+  const int lofs = 8 * ((lane & 31) * kOzTile + 4 * (lane >> 5));
+#pragma unroll
+  for (int tt = 0; tt < 4; tt++)
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+      oz_st(0.0, part, lofs,
+            8 * (32 * (tt & 1) * kOzTile + 32 * (tt >> 1) + 8 * (r >> 2) + (r & 3)));
+  const __amdgpu_buffer_rsrc_t stage = oz_rsrc(
+      a.stage + ((int64_t)blockIdx.x * (kOzThreads / 64) + wave) * (S * 4096),
+      (int64_t)S * 4096 * 4);
+  const int* ea_w = &erun[0][64 * wm];
+  const int* eb_w = &erun[1][64 * wn];
+  __syncthreads();
+
+  const int64_t nch = (a.K + kOzRows - 1) / kOzRows;
+  double x[kOzRows], xn[kOzRows];  // this chunk and the next one in flight
+  // rows [32 c, 32 c + 32) of this thread's column; rows at or past K read
+  // the zero area instead (the choice of resource is scalar), so every chunk
+  // issues the same 32 loads back to back.  Padding columns read column 0
+  // and are zeroed through their scale.
+  const __amdgpu_buffer_rsrc_t rz = oz_rsrc(a.zeros, 8 * 1024);
+  auto load_chunk = [&](int64_t c, double (&dst)[kOzRows]) {
+    const int64_t r0 = c * kOzRows;
+    const int rows = (int)min<int64_t>(kOzRows, a.K - r0);
+    const __amdgpu_buffer_rsrc_t rs =
+        oz_rsrc(mbase + r0 * ld, 8 * ((int64_t)(rows - 1) * ld + width));
+#pragma unroll
+    for (int i = 0; i < kOzRows; i++)
+      dst[i] = i < rows ? oz_ld(rs, voff, i * ld8) : oz_ld(rz, voff, 0);
+  };
+  // frexp exponent of the chunk's largest |x| (max |x| < 2^e), kOzEmin for
+  // an all-zero chunk or a padding column
+  auto chunk_exp = [&]() {
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < kOzRows; i++) mx = fmax(mx, fabs(x[i]));
+    int e = kOzEmin;
+    if (cvalid && mx > 0.0) (void)frexp(mx, &e);
+    return e;
+  };
+  // the scale the limbs are taken against: 2^E for a live column; an
+  // all-zero column (E = kOzEmin) uses 2^-1074, below which nothing is
+  // nonzero, so any nonzero value reads as growth; padding columns use
+  // 2^2000, which turns every value into zero limbs
+  auto split_scale = [&]() {
+    return cvalid ? max(erun[mat][col], -1074) : 2000;
+  };
+  // split chunk c (in x) into S limbs and stage them in LDS buffer b
+  auto limbs = [&](int b) {
+    const int Es = split_scale();
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+      int words[S][4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const double xv[4] = {x[16 * half + 4 * q], x[16 * half + 4 * q + 1],
+                              x[16 * half + 4 * q + 2], x[16 * half + 4 * q + 3]};
+        int w[S];
+        (void)oz_pack4<S>(xv, Es, w);
+#pragma unroll
+        for (int s2 = 0; s2 < S; s2++) words[s2][q] = w[s2];
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < S; s2++) {
+        const v4i pk = {words[s2][0], words[s2][1], words[s2][2], words[s2][3]};
+        *reinterpret_cast<v4i*>(&sl[b][mat][s2][col][16 * half]) = pk;
+      }
+    }
+  };
+  // INT8 products of the chunk staged in buffer b, exact INT32 accumulation
+  // per level g = s + t
+  const int kh = 16 * (lane >> 5);
+  auto multiply = [&](int b) {
+#pragma unroll
+    for (int tt = 0; tt < 4; tt++) {
+      const int mcol = 64 * wm + 32 * (tt >> 1) + (lane & 31);
+      const int ncol = 64 * wn + 32 * (tt & 1) + (lane & 31);
+#pragma unroll
+      for (int g = 2; g <= S + 1; g++) {
+#pragma unroll
+        for (int s1 = 1; s1 <= S; s1++) {
+          const int t1 = g - s1;
+          if (t1 < 1 || t1 > S) continue;
+          const v4i av = *reinterpret_cast<const v4i*>(&sl[b][0][s1 - 1][mcol][kh]);
+          const v4i bv = *reinterpret_cast<const v4i*>(&sl[b][1][t1 - 1][ncol][kh]);
+          acc[g - 2][tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc[g - 2][tt], 0, 0, 0);
+        }
+      }
+    }
+  };
+  // Steady-state step: split the chunk in x into LDS buffer bs while the
+  // chunk staged in buffer bm is multiplied.  The splits of 8 groups of 4
+  // rows and the 2 S (S + 1) products are issued interleaved, in fixed units
+  // separated by scheduling fences, so the VALU work runs while the matrix
+  // core is busy.  Returns nonzero if a value outgrew its column's scale.
+  auto split_multiply = [&](int bm, int bs) {
+    v4i av[2][S], bv[2][S];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int s2 = 0; s2 < S; s2++) {
+        av[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][0][s2][64 * wm + 32 * h + (lane & 31)][kh]);
+        bv[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][1][s2][64 * wn + 32 * h + (lane & 31)][kh]);
+      }
+    const int Es = split_scale();
+    unsigned over = 0;
+    int words[S][4];
+    oz_static_for(
+        [&](auto U_) {
+          constexpr int U = decltype(U_)::value;  // rows 4 U .. 4 U + 3
+          {
+            const double xv[4] = {x[4 * U], x[4 * U + 1], x[4 * U + 2], x[4 * U + 3]};
+            int w[S];
+            over |= oz_pack4<S>(xv, Es, w);
+#pragma unroll
+            for (int s2 = 0; s2 < S; s2++) words[s2][U % 4] = w[s2];
+          }
+          constexpr int NM = oz_nmfma<S>();
+          constexpr int k0 = U * NM / 8, k1 = (U + 1) * NM / 8;
+          oz_static_for(
+              [&](auto K_) {
+                constexpr OzMf m = oz_mfma_of<S>(k0 + decltype(K_)::value);
+                acc[m.g - 2][m.tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                    av[m.tt >> 1][m.s - 1], bv[m.tt & 1][m.g - m.s - 1],
+                    acc[m.g - 2][m.tt], 0, 0, 0);
+              },
+              std::make_integer_sequence<int, k1 - k0>{});
+          if constexpr (U % 4 == 3) {
+#pragma unroll
+            for (int s2 = 0; s2 < S; s2++) {
+              const v4i pk = {words[s2][0], words[s2][1], words[s2][2], words[s2][3]};
+              *reinterpret_cast<v4i*>(&sl[bs][mat][s2][col][16 * (U / 4)]) = pk;
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        },
+        std::make_integer_sequence<int, kOzRows / 4>{});
+    return over;
+  };
+
+  // Software pipeline: while chunk c is split (VALU) into one LDS buffer,
+  // the previous chunk's limbs in the other buffer are multiplied (matrix
+  // cores) and chunk c + G is being loaded.  A flush happens before chunk c
+  // when c raises a column's scale or when its 32 rows would take the
+  // accumulators past kOzFlushRows rows; `since` counts the rows staged
+  // since the last flush.  Growth shows up while c is being split: the
+  // previous chunk has been multiplied by then, and c is split again after
+  // the flush.
+  int64_t ch = kp;
+  load_chunk(min(ch, nch - 1), x);
+  int pb = 0;
+  while (ch < nch) {
+    // (re)start at chunk ch: accumulators empty, nothing staged
+    {
+      const int e = chunk_exp();
+      const int eold = erun[mat][col];
+      erun[mat][col] = max(eold, e);  // only this thread touches the entry
+    }
+    __syncthreads();
+    load_chunk(min(ch + a.G, nch - 1), xn);
+    limbs(pb);
+    int since = kOzRows;
+#pragma unroll
+    for (int i = 0; i < kOzRows; i++) x[i] = xn[i];
+    __syncthreads();
+    bool staged = true;  // buffer pb holds a chunk not yet multiplied
+    for (ch += a.G; ch < nch; ch += a.G) {
+      oz_pin<S>(acc);
+      if (since + kOzRows > kOzFlushRows) break;  // uniform
+      load_chunk(min(ch + a.G, nch - 1), xn);
+      const unsigned over = split_multiply(pb, pb ^ 1);
+      if (__syncthreads_or(over != 0)) {
+        staged = false;  // buffer pb was multiplied; ch is split again
+        break;
+      }
+      since += kOzRows;
+#pragma unroll
+      for (int i = 0; i < kOzRows; i++) x[i] = xn[i];
+      pb ^= 1;
+    }
+    // drain: multiply the staged chunk, convert the accumulators
+    if (staged) multiply(pb);
+    oz_flush<S>(acc, stage, part, lofs, ea_w, eb_w, lane);
+    __syncthreads();
+  }
+
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  const int grp = kp / a.GS;
+  const int gsz = min(a.GS, a.G - grp * a.GS);
+  unsigned* cnt1 = a.cnt + tile * a.ngroups + grp;
+  unsigned* cnt2 = a.cnt + ntiles * a.ngroups + tile;
+  if (tid == 0) {
+    unsigned t = __hip_atomic_fetch_add(cnt1, 1u, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (t == (unsigned)(gsz - 1));
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  const int Mloc = min(a.M - m0, kOzTile), Nloc = min(a.N - n0, kOzTile);
+  auto apply = [&](int e, double s) {
+    const int ml = e % kOzTile, nl = e / kOzTile;
+    if (ml < Mloc && nl < Nloc) {
+#pragma clang fp contract(off)
+      double* c = a.C + (int64_t)(n0 + nl) * a.ldc + (m0 + ml);
+      double r = a.alpha * s;
+      if (a.beta != 0.0) r = a.beta * *c + r;
+      *c = r;
+    }
+  };
+  const double* gsrc = a.part1 + ((int64_t)tile * a.G + (int64_t)grp * a.GS) * PS;
+  if (a.ngroups == 1) {
+    team_reduce<double>(gsrc, gsz, PS, tree, apply);
+    if (tid == 0) *cnt1 = 0u;
+    return;
+  }
+  double* gdst = a.part2 + ((int64_t)tile * a.ngroups + grp) * PS;
+  team_reduce<double>(gsrc, gsz, PS, tree, [&](int e, double s) { gdst[e] = s; });
+  if (tid == 0) *cnt1 = 0u;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (tid == 0) {
+    unsigned t = __hip_atomic_fetch_add(cnt2, 1u, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (t == (unsigned)(a.ngroups - 1));
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  team_reduce<double>(a.part2 + (int64_t)tile * a.ngroups * PS, a.ngroups, PS,
+                      tree, apply);
+  if (tid == 0) *cnt2 = 0u;
+}
+
+// Lane-layout probe of the INT8 MFMA (diagnostic): out[0..1023] = row index
+// of element (lane, r) (as 32 * row), out[1024..2047] = column index (32 *
+// column), from A = lane row index / B = ones and the reverse.
+__global__ void i8_layout_probe(int* out) {
+  const int lane = threadIdx.x;
+  const int i = lane & 31;
+  const int rowv = i | (i << 8) | (i << 16) | (i << 24);
+  const int ones = 0x01010101;
+  const v4i a1 = {rowv, rowv, rowv, rowv}, b1 = {ones, ones, ones, ones};
+  v16i d1 = {};
+  d1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(a1, b1, d1, 0, 0, 0);
+  v16i d2 = {};
+  d2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(b1, a1, d2, 0, 0, 0);
+  for (int r = 0; r < 16; r++) {
+    out[lane * 16 + r] = d1[r];
+    out[1024 + lane * 16 + r] = d2[r];
+  }
+}
+
+static int isqrt_u(int x) {
+  int r = 1;
+  while (r * r < x) r++;
+  return r;
+}
+
+}  // namespace skinny
+
+using namespace skinny;
+
+extern "C" {
+
+skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
+                                      int64_t M, int64_t N, int64_t K,
+                                      const double* A, int64_t lda,
+                                      double alpha, const double* B,
+                                      int64_t ldb, double beta, double* C,
+                                      int64_t ldc, int slices,
+                                      skinny_stream_t stream) {
+  if (M < 0 || N < 0 || K < 0)
+    return fail(SKINNY_INVALID_ARG, "ozaki: negative size");
+  if (M == 0 || N == 0) return SKINNY_OK;
+  if (!C || (K > 0 && (!A || !B)))
+    return fail(SKINNY_INVALID_ARG, "ozaki: null matrix");
+  if (lda < M || ldb < N || ldc < M)
+    return fail(SKINNY_INVALID_ARG, "ozaki: leading dimension too small");
+  if (slices < 3 || slices > 5)
+    return fail(SKINNY_UNSUPPORTED, "ozaki: 3..5 slices");
+  if (M > 1024 || N > 1024)
+    return fail(SKINNY_UNSUPPORTED, "ozaki: M, N <= 1024");
+  if (lda > (1 << 22) || ldb > (1 << 22))
+    return fail(SKINNY_UNSUPPORTED, "ozaki: lda, ldb <= 2^22");
+  h = resolve(h);
+  if (!h) return fail(SKINNY_HIP_ERROR, "ozaki: no device handle");
+  OzArgs a;
+  a.tilesM = (int)((M + kOzTile - 1) / kOzTile);
+  a.tilesN = (int)((N + kOzTile - 1) / kOzTile);
+  const int ntiles = a.tilesM * a.tilesN;
+  int64_t G = (int64_t)blockCount;
+  if (G <= 0) {
+    G = std::max<int64_t>(1, h->cu_count / ntiles);
+    G = std::min<int64_t>(G, std::max<int64_t>(1, K / (kOzRows * 64)));
+  }
+  if (G > 65536) G = 65536;
+  a.G = (int)G;
+  if (a.G <= 16) {
+    a.GS = a.G;
+    a.ngroups = 1;
+  } else {
+    a.GS = isqrt_u(a.G);
+    a.ngroups = (a.G + a.GS - 1) / a.GS;
+  }
+  if ((size_t)ntiles * (a.ngroups + 1) > (size_t)kMaxCounters)
+    return fail(SKINNY_UNSUPPORTED, "ozaki: too many partial groups");
+  const size_t PS = (size_t)kOzTile * kOzTile;
+  size_t bytes = (size_t)ntiles * a.G * PS * sizeof(double);
+  if (a.ngroups > 1) bytes += (size_t)ntiles * a.ngroups * PS * sizeof(double);
+  const size_t part_bytes = bytes;
+  bytes += (size_t)ntiles * a.G * (kOzThreads / 64) * slices * 4096 * sizeof(int);
+  skinny_status_t st = ensure_workspace(h, bytes);
+  if (st != SKINNY_OK) return st;
+  a.A = A;
+  a.B = B;
+  a.C = C;
+  a.K = K;
+  a.lda = lda;
+  a.ldb = ldb;
+  a.ldc = ldc;
+  a.M = (int)M;
+  a.N = (int)N;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.part1 = static_cast<double*>(ws_scratch(h));
+  a.part2 = a.part1 + (size_t)ntiles * a.G * PS;
+  a.stage = reinterpret_cast<int*>(static_cast<char*>(ws_scratch(h)) + part_bytes);
+  a.zeros = ws_zeros(h);
+  a.cnt = ws_counters(h);
+  const int blocks = a.G * ntiles;
+  hipStream_t s = (hipStream_t)stream;
+  switch (slices) {
+    case 3: hipLaunchKernelGGL(ozaki_kernel<3>, dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+    case 4: hipLaunchKernelGGL(ozaki_kernel<4>, dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+    default: hipLaunchKernelGGL(ozaki_kernel<5>, dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "ozaki launch");
+  return SKINNY_OK;
+}
+
+skinny_status_t skinny_probe_i8_layout(int* host_out) {
+  if (!host_out) return fail(SKINNY_INVALID_ARG, "probe: null out");
+  int* d = nullptr;
+  hipError_t e = hipMalloc(&d, 2048 * sizeof(int));
+  if (e != hipSuccess) return hip_fail(e, "probe alloc");
+  hipLaunchKernelGGL(i8_layout_probe, dim3(1), dim3(64), 0, 0, d);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(host_out, d, 2048 * sizeof(int), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess) return hip_fail(e, "probe");
+  return SKINNY_OK;
+}
+
+}  // extern "C"

@@ -25,6 +25,7 @@
 #include "internal.hpp"
 #include "mfma.hpp"
 #include "tsmttsm_core.hpp"
+#include "team_sum.hpp"
 
 namespace skinny {
 
@@ -82,61 +83,6 @@ __device__ __forceinline__ void complex_epilogue(const TsArgs<T>& a, T r00,
   }
   c[0] = o_re;
   c[1] = o_im;
-}
-
-// Sum `count` partials of PS entries (src[q * PS + e]) with the whole
-// workgroup: entry e is summed by a team of TPE threads (thread j of the team
-// takes partials j, j + TPE, ...; the team is then combined by a fixed LDS
-// tree), so a small output pays one round trip instead of `count`.  The order
-// depends only on (PS, count): results are reproducible.  The sums go to
-// dst (group level) or to `fin` in LDS (final level, PS <= ldsn / 2).
-template <typename T>
-__device__ void team_sum(const T* src, int count, int PS, T* dst, T* lds,
-                         int ldsn, bool to_lds) {
-  int pad = 1;
-  while (pad < PS) pad <<= 1;
-  int tpe = 256 / pad;
-  if (tpe < 1) tpe = 1;
-  int cpow = 1;
-  while (cpow < count) cpow <<= 1;
-  if (tpe > cpow) tpe = cpow;
-  const int per_pass = 256 / tpe;
-  T* tree = lds;
-  T* fin = lds + ldsn / 2;
-  const int tid = threadIdx.x;
-  const int j = tid % tpe;
-  for (int e0 = 0; e0 < PS; e0 += per_pass) {
-    const int e = e0 + tid / tpe;
-    T s = T(0);
-    if (e < PS) {
-      const T* p = src + e;
-      int q = j;
-      for (; q + 3 * tpe < count; q += 4 * tpe) {
-        const T v0 = p[(int64_t)q * PS];
-        const T v1 = p[(int64_t)(q + tpe) * PS];
-        const T v2 = p[(int64_t)(q + 2 * tpe) * PS];
-        const T v3 = p[(int64_t)(q + 3 * tpe) * PS];
-        s += v0;
-        s += v1;
-        s += v2;
-        s += v3;
-      }
-      for (; q < count; q += tpe) s += p[(int64_t)q * PS];
-    }
-    tree[tid] = s;
-    __syncthreads();
-    for (int w = tpe >> 1; w > 0; w >>= 1) {
-      if (j < w) tree[tid] += tree[tid + w];
-      __syncthreads();
-    }
-    if (j == 0 && e < PS) {
-      if (to_lds)
-        fin[e] = tree[tid];
-      else
-        dst[e] = tree[tid];
-    }
-    __syncthreads();
-  }
 }
 
 // Final epilogue from the summed entries in LDS (real or complex view).

@@ -1,0 +1,63 @@
+"""Pin the oracle's Ozaki-split restatement on the CPU.
+
+Checks: exactness on data the limbs represent exactly (small integers, so
+every product and the reconstruction are exact), error against the exact
+superaccumulator decreasing with the limb count and within the scheme's
+bound (and within the 8 K eps tolerance at K = 2^24 from 4 limbs on) on
+U[-1,1) and on cancelling columns, and alpha/beta handling with growing
+magnitudes.
+"""
+import numpy as np
+import pytest
+
+
+def test_integers_exact(orc):
+    M, N, K = 7, 5, 700
+    A = orc.integers(K * M, 1, 0, -100, 100)
+    B = orc.integers(K * N, 1, 1, -100, 100)
+    want = np.zeros(M * N)
+    orc.tsmttsm(M, N, K, A, M, 1.0, B, N, 0.0, want, M)
+    for S in (3, 4, 5):
+        got = np.zeros(M * N)
+        orc.ozaki_tsmttsm(M, N, K, A, M, 1.0, B, N, 0.0, got, M, S)
+        assert np.array_equal(got, want), S
+
+
+@pytest.mark.parametrize("data", ["uniform", "c3"])
+def test_error_vs_exact(orc, data):
+    M = N = 24
+    K = 3000
+    if data == "uniform":
+        A = orc.uniform(K * M, 2, 0)
+        B = orc.uniform(K * N, 2, 1)
+    else:
+        A = orc.c3_columns(M, K, 3)
+        B = A
+    hi, lo = orc.xtsmttsm(M, N, K, A, M, B, N)
+    errs = []
+    for S in (3, 4, 5):
+        got = np.zeros(M * N)
+        orc.ozaki_tsmttsm(M, N, K, A, M, 1.0, B, N, 0.0, got, M, S)
+        errs.append(np.linalg.norm((got - hi) - lo) / np.linalg.norm(hi))
+    assert errs[0] > errs[1] > errs[2]
+    # truncated scheme: limb truncation and the dropped levels s + t > S + 1
+    # each contribute about S 2^(-7 S) of sum |a||b|
+    for S, err in zip((3, 4, 5), errs):
+        assert err <= 8 * (S + 1) * 2.0 ** (-7 * S), (S, err)
+    # the C3 tolerance, 8 K eps at K = 2^24 (3e-8), holds from 4 limbs on
+    assert errs[1] <= 8 * 2 ** 24 * np.finfo(float).eps
+
+
+def test_alpha_beta_and_growth(orc):
+    M, N, K = 3, 4, 5000
+    A = orc.uniform(K * M, 4, 0)
+    B = orc.uniform(K * N, 4, 1)
+    # magnitude grows along k: exercises the rescale-and-flush path
+    scale = np.ldexp(1.0, (np.arange(K) // 700))
+    A = (A.reshape(K, M) * scale[:, None]).ravel()
+    C0 = orc.uniform(M * N, 4, 2)
+    got = C0.copy()
+    orc.ozaki_tsmttsm(M, N, K, A, M, 2.0, B, N, -0.5, got, M, 5)
+    want = C0.copy()
+    orc.tsmttsm(M, N, K, A, M, 2.0, B, N, -0.5, want, M)
+    assert np.linalg.norm(got - want) <= 1e-9 * np.linalg.norm(want)
