@@ -134,6 +134,14 @@ def lib():
         if not os.path.exists(_LIB_PATH):
             raise SkinnyError(HIP_ERROR, f"{_LIB_PATH} is not built: run "
                               "`python -m skinny_matmul_amd.build`")
+        # torch (when installed) carries its own copy of the HIP runtime; load
+        # it first so that libskinny binds to that same copy.  Loading the
+        # system runtime first and torch afterwards puts two HIP runtimes in
+        # one process, and device discovery then fails.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(_LIB_PATH)
         for name, (res, args) in _SIGS.items():
             fn = getattr(L, name)
@@ -161,7 +169,10 @@ def _addr(x) -> int:
 
 def _stream(stream) -> int:
     if stream is None:
-        import torch
+        import sys
+        torch = sys.modules.get("torch")
+        if torch is None or not torch.cuda.is_initialized():
+            return 0  # the null stream
         return int(torch.cuda.current_stream().cuda_stream)
     if isinstance(stream, int):
         return stream

@@ -300,7 +300,7 @@ skinny_status_t skinny_dbatch_create(skinny_handle_t h, int nops,
   if (!out || nops < 1 || !ops)
     return fail(SKINNY_INVALID_ARG, "batch: no ops");
   h = resolve(h);
-  if (!h) return fail(SKINNY_HIP_ERROR, "batch: no device handle");
+  if (!h) return no_handle("batch");
   if (item_bytes <= 0) item_bytes = 8LL << 20;
   std::vector<BatchOpDev> dops(nops);
   std::vector<BatchItemDev> items;

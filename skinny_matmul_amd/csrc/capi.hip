@@ -33,7 +33,7 @@ skinny_status_t tsmttsm_real(const char* fn, skinny_handle_t h,
   if (M > 1024 || N > 1024)
     return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": M, N <= 1024 only");
   h = resolve(h);
-  if (!h) return fail(SKINNY_HIP_ERROR, std::string(fn) + ": no device handle");
+  if (!h) return no_handle(fn);
   TsmttsmPlan p = plan_tsmttsm(h, blockCount, M, N, K, (int)sizeof(T), (int)sizeof(T));
   return launch_tsmttsm_real<T>(h, p, M, N, K, A, lda, alpha, B, ldb, beta, C,
                                 ldc, (hipStream_t)stream);
@@ -64,7 +64,7 @@ skinny_status_t tsmm_real(const char* fn, skinny_handle_t h, size_t blockCount,
       return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": partially overlapping A and C");
   }
   h = resolve(h);
-  if (!h) return fail(SKINNY_HIP_ERROR, std::string(fn) + ": no device handle");
+  if (!h) return no_handle(fn);
   if (M == 0) {
     // C = beta * C: the M-loop is empty, the kernel handles it
   }
@@ -87,7 +87,7 @@ skinny_status_t tsmttsm_cplx(const char* fn, skinny_handle_t h,
   if (M > 512 || N > 512)
     return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": M, N <= 512 only");
   h = resolve(h);
-  if (!h) return fail(SKINNY_HIP_ERROR, std::string(fn) + ": no device handle");
+  if (!h) return no_handle(fn);
   TsmttsmPlan p = plan_tsmttsm(h, blockCount, 2 * M, 2 * N, K, (int)sizeof(T),
                                (int)sizeof(T));
   return launch_tsmttsm<T>(h, p, 2 * M, 2 * N, K, (const T*)A, 2 * lda,
@@ -120,7 +120,7 @@ skinny_status_t tsmm_cplx(const char* fn, skinny_handle_t h, size_t blockCount,
       return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": partially overlapping A and C");
   }
   h = resolve(h);
-  if (!h) return fail(SKINNY_HIP_ERROR, std::string(fn) + ": no device handle");
+  if (!h) return no_handle(fn);
   return launch_tsmm_lds<T>(h, blockCount, 2 * M, 2 * N, K, (const T*)A,
                             2 * lda, alpha.x, alpha.y, (const T*)B, ldb,
                             beta.x, beta.y, (T*)C, 2 * ldc, true,
@@ -142,7 +142,7 @@ skinny_status_t htsmttsm_any(const char* fn, skinny_handle_t h,
   if (M > 1024 || N > 1024)
     return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": M, N <= 1024 only");
   h = resolve(h);
-  if (!h) return fail(SKINNY_HIP_ERROR, std::string(fn) + ": no device handle");
+  if (!h) return no_handle(fn);
   return launch_htsmttsm(h, blockCount, M, N, K, A, lda, alpha, B, ldb, beta, C,
                          dd_out, ldc, (hipStream_t)stream);
 }

@@ -51,6 +51,9 @@ constexpr int kMaxCounters = (kCounterBytes - kZeroBytes) / sizeof(unsigned int)
 void set_error(const std::string& msg);
 skinny_status_t fail(skinny_status_t s, const std::string& msg);
 skinny_status_t hip_fail(hipError_t e, const char* what);
+// failure of a call that found no handle (the default handle's creation
+// error is included)
+skinny_status_t no_handle(const std::string& fn);
 
 // Make sure the handle's scratch region holds `scratch_bytes` (grows the
 // allocation, synchronising the device, only when it is too small).

@@ -135,7 +135,7 @@ skinny_status_t skinny_mb_stream(int kind, size_t bytes_per_array, int reps,
   if (!gbs || kind < 0 || kind > 4 || bytes_per_array < 4096 || reps < 1)
     return fail(SKINNY_INVALID_ARG, "skinny_mb_stream: bad arguments");
   skinny_handle_t h = skinny_default_handle();
-  if (!h) return fail(SKINNY_HIP_ERROR, "no device");
+  if (!h) return no_handle("microbench");
   hipStream_t st = (hipStream_t)stream;
   const int64_t n2 = (int64_t)(bytes_per_array / 16);
   const size_t nb = (size_t)n2 * 16;
@@ -186,7 +186,7 @@ skinny_status_t skinny_mb_peak(int kind, int reps, double* tflops,
   if (!tflops || kind < 0 || kind > 2 || reps < 1)
     return fail(SKINNY_INVALID_ARG, "skinny_mb_peak: bad arguments");
   skinny_handle_t h = skinny_default_handle();
-  if (!h) return fail(SKINNY_HIP_ERROR, "no device");
+  if (!h) return no_handle("microbench");
   hipStream_t st = (hipStream_t)stream;
   void* out = nullptr;
   hipError_t e = hipMalloc(&out, 4096);

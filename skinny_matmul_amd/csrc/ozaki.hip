@@ -575,7 +575,7 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
   if (lda > (1 << 22) || ldb > (1 << 22))
     return fail(SKINNY_UNSUPPORTED, "ozaki: lda, ldb <= 2^22");
   h = resolve(h);
-  if (!h) return fail(SKINNY_HIP_ERROR, "ozaki: no device handle");
+  if (!h) return no_handle("ozaki");
   OzArgs a;
   a.tilesM = (int)((M + kOzTile - 1) / kOzTile);
   a.tilesN = (int)((N + kOzTile - 1) / kOzTile);

@@ -11,6 +11,14 @@ static thread_local std::string g_last_error;
 
 void set_error(const std::string& msg) { g_last_error = msg; }
 
+// why the process-wide default handle could not be created (empty if it was)
+static std::string g_default_handle_error;
+
+skinny_status_t no_handle(const std::string& fn) {
+  return fail(SKINNY_HIP_ERROR, fn + ": no device handle (" +
+                                    g_default_handle_error + ")");
+}
+
 skinny_status_t fail(skinny_status_t s, const std::string& msg) {
   set_error(msg);
   return s;
@@ -167,7 +175,10 @@ skinny_handle_t skinny_default_handle(void) {
   static std::once_flag once;
   static skinny_handle_t h = nullptr;
   std::call_once(once, [] {
-    if (skinny_create(&h) != SKINNY_OK) h = nullptr;
+    if (skinny_create(&h) != SKINNY_OK) {
+      h = nullptr;
+      g_default_handle_error = g_last_error;
+    }
   });
   return h;
 }

@@ -291,7 +291,7 @@ skinny_status_t skinny_dsvqb(skinny_handle_t h, int nblocks,
   if (s != SKINNY_OK) return s;
   if (sweeps < 0) return fail(SKINNY_INVALID_ARG, "svqb: sweeps < 0");
   h = resolve(h);
-  if (!h) return fail(SKINNY_HIP_ERROR, "svqb: no device handle");
+  if (!h) return no_handle("svqb");
   s = ensure_workspace(h, max_gram_scratch(h, blocks, nblocks));
   if (s != SKINNY_OK) return s;
   SvqbPlanBuf b;
@@ -373,7 +373,7 @@ skinny_status_t skinny_dorth_error(skinny_handle_t h, int64_t M, int64_t K,
   if (!fro || M < 1 || K < 0 || !X || ldx < M)
     return fail(SKINNY_INVALID_ARG, "orth_error: bad arguments");
   h = resolve(h);
-  if (!h) return fail(SKINNY_HIP_ERROR, "orth_error: no device handle");
+  if (!h) return no_handle("orth_error");
   double* G = nullptr;
   hipError_t e = hipMalloc(&G, sizeof(double) * M * M);
   if (e != hipSuccess) return hip_fail(e, "orth_error alloc");

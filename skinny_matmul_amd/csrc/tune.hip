@@ -74,7 +74,7 @@ skinny_status_t skinny_autotune(skinny_handle_t h, int op, int dtype,
       N > 64)
     return fail(SKINNY_UNSUPPORTED, "autotune: fp64, 1 <= M, N <= 64");
   h = resolve(h);
-  if (!h) return fail(SKINNY_HIP_ERROR, "autotune: no device handle");
+  if (!h) return no_handle("autotune");
   if (reps < 1) reps = 3;
   hipStream_t st = nullptr;
   hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);

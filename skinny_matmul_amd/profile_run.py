@@ -2,6 +2,8 @@
 
     python -m skinny_matmul_amd.profile_run --op tsmm --M 64 --logK 26 --reps 3
     python -m skinny_matmul_amd.profile_run --op copy --logK 26 --reps 3
+    python -m skinny_matmul_amd.profile_run --op ozaki --M 128 --logK 24 --S 4
+    python -m skinny_matmul_amd.profile_run --op hr --M 64 --logK 22
 
 ``--op copy`` runs the HBM copy microbenchmark over arrays of the same byte
 size as A (the calibration kernel for FETCH_SIZE / WRITE_SIZE counters).
@@ -15,7 +17,8 @@ import sys
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--op", choices=("tsmm", "tsmttsm", "copy"), required=True)
+    ap.add_argument("--op", choices=("tsmm", "tsmttsm", "ozaki", "hr", "copy"), required=True)
+    ap.add_argument("--S", type=int, default=4, help="Ozaki limb count")
     ap.add_argument("--M", type=int, default=64)
     ap.add_argument("--logK", type=int, default=26)
     ap.add_argument("--reps", type=int, default=3)
@@ -42,6 +45,10 @@ def main() -> int:
     for _ in range(a.reps):
         if a.op == "tsmm":
             ok = sk.dtsmm(0, M, M, K, A, M, 1.0, Bs, M, 0.0, B, M)
+        elif a.op == "ozaki":
+            ok = sk.dtsmttsm_ozaki(0, M, M, K, A, M, 1.0, B, M, 0.0, Cs, M, slices=a.S)
+        elif a.op == "hr":
+            ok = sk.htsmttsm(0, M, M, K, A, M, 1.0, B, M, 0.0, Cs, M)
         else:
             ok = sk.dtsmttsm(0, M, M, K, A, M, 1.0, B, M, 0.0, Cs, M)
         assert ok

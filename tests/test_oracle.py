@@ -185,6 +185,17 @@ def test_rng_matches_python_restatement(orc):
     assert set(np.unique(ints)) == {-2.0, -1.0, 0.0, 1.0, 2.0}
 
 
+def test_rng_splitmix64_published_values(orc):
+    """orc_rng_bits(key, 0) is SplitMix64's output for state `key`: with key
+    = (n - 1) * golden it must give the published first outputs of
+    SplitMix64 seeded with 0 (Steele, Lea, Flood 2014; the xoshiro
+    reference seeding generator)."""
+    golden = 0x9E3779B97F4A7C15
+    want = (0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F)
+    for n, w in enumerate(want):
+        assert orc.lib().orc_rng_bits((n * golden) & ((1 << 64) - 1), 0) == w
+
+
 # ---------------------------------------------------------------------------
 # PseudoQuad error-free transformations, exact rationals
 # ---------------------------------------------------------------------------
