@@ -113,6 +113,41 @@ SKINNY_HD PseudoQuad dd_fma(PseudoQuad c, double a, double b) {
   return dd_add(c, two_prod(a, b));
 }
 
+// Double-double plus a third word u collecting the rounding remainders of
+// dd_add's two inexact additions (c and w): v is bitwise dd_add's result and
+// v + u carries the sum to the remainders' precision.  Combining partial sums
+// of very different magnitude (one near 2^124 and one near 2^62) puts the
+// smaller one into t, whose ulp rounds the small terms away before the large
+// ones cancel; u keeps them.  HR uses it to combine partial sums.
+struct DdAcc {
+  PseudoQuad v;
+  double u;
+};
+
+SKINNY_HD DdAcc dda_add(DdAcc x, PseudoQuad y) {
+SKINNY_NOCONTRACT
+  PseudoQuad S = two_sum(x.v.s, y.s);
+  PseudoQuad T = two_sum(x.v.t, y.t);
+  PseudoQuad Cc = two_sum(S.t, T.s);  // c of dd_add and its remainder
+  PseudoQuad V = fast2sum(S.s, Cc.s);
+  PseudoQuad W = two_sum(T.t, V.t);   // w of dd_add and its remainder
+  DdAcc r;
+  r.v = fast2sum(V.s, W.s);
+  r.u = x.u + (Cc.t + W.t);
+  return r;
+}
+
+SKINNY_HD DdAcc dda_add(DdAcc x, DdAcc y) {
+SKINNY_NOCONTRACT
+  DdAcc r = dda_add(x, y.v);
+  r.u = r.u + y.u;
+  return r;
+}
+
+SKINNY_HD PseudoQuad dda_value(DdAcc x) {
+  return dd_add(x.v, PseudoQuad{x.u, 0.0});
+}
+
 SKINNY_HD PseudoQuad dd_renorm(PseudoQuad a) { return fast2sum(a.s, a.t); }
 
 SKINNY_HD double dd_to_d(PseudoQuad a) {

@@ -3,7 +3,8 @@
 // Mirrors the reference operator boundary (versions.hpp:43-177, types.hpp:9-59):
 //
 //   * the type set FR / DR / FC / DC / HR with htype, dtype, dItype and
-//     flopsPerCell (types.hpp:9-57);
+//     flopsPerCell (types.hpp:9-57), as structs skinny::TypeFR .. TypeHR
+//     (the bare names are the selecting macros);
 //   * MatmulFunctionType, the std::function every variant is called through
 //     (versions.hpp:43-45);
 //   * templates skinny::tsmttsm<T, iT> and skinny::tsmm<T> with the reference
@@ -53,35 +54,35 @@ inline bool status_to_bool(skinny_status_t s) {
 }
 
 // ---- type set (types.hpp:9-57) --------------------------------------------
-struct FR {
+struct TypeFR {
   typedef float htype;
   typedef float dtype;
   typedef float dItype;
   static constexpr int flopsPerCell = 2;
   static const char* name() { return "FR"; }
 };
-struct DR {
+struct TypeDR {
   typedef double htype;
   typedef double dtype;
   typedef double dItype;
   static constexpr int flopsPerCell = 2;
   static const char* name() { return "DR"; }
 };
-struct FC {
+struct TypeFC {
   typedef std::complex<float> htype;
   typedef skinny_float_complex dtype;
   typedef skinny_float_complex dItype;
   static constexpr int flopsPerCell = 8;
   static const char* name() { return "FC"; }
 };
-struct DC {
+struct TypeDC {
   typedef std::complex<double> htype;
   typedef skinny_double_complex dtype;
   typedef skinny_double_complex dItype;
   static constexpr int flopsPerCell = 8;
   static const char* name() { return "DC"; }
 };
-struct HR {
+struct TypeHR {
   typedef double htype;
   typedef double dtype;
   typedef PseudoQuad dItype;
@@ -120,6 +121,43 @@ inline bool tsmttsm<float, float>(size_t bc, int M, int N, int K,
                                         ldb, beta, C, ldc, nullptr));
 }
 
+template <>
+inline bool tsmttsm<skinny_double_complex, skinny_double_complex>(
+    size_t bc, int M, int N, int K, const skinny_double_complex* A, int lda,
+    skinny_double_complex alpha, const skinny_double_complex* B, int ldb,
+    skinny_double_complex beta, skinny_double_complex* C, int ldc) {
+  return status_to_bool(skinny_ztsmttsm(nullptr, bc, M, N, K, A, lda, alpha, B,
+                                        ldb, beta, C, ldc, nullptr));
+}
+template <>
+inline bool tsmttsm<skinny_float_complex, skinny_float_complex>(
+    size_t bc, int M, int N, int K, const skinny_float_complex* A, int lda,
+    skinny_float_complex alpha, const skinny_float_complex* B, int ldb,
+    skinny_float_complex beta, skinny_float_complex* C, int ldc) {
+  return status_to_bool(skinny_ctsmttsm(nullptr, bc, M, N, K, A, lda, alpha, B,
+                                        ldb, beta, C, ldc, nullptr));
+}
+// HR: double data, PseudoQuad accumulation (types.hpp:49-57)
+template <>
+inline bool tsmttsm<double, PseudoQuad>(size_t bc, int M, int N, int K,
+                                        const double* A, int lda, double alpha,
+                                        const double* B, int ldb, double beta,
+                                        double* C, int ldc) {
+  return status_to_bool(skinny_htsmttsm(nullptr, bc, M, N, K, A, lda, alpha, B,
+                                        ldb, beta, C, ldc, nullptr));
+}
+
+// FP64 TSMTTSM through S INT8 limbs on the matrix cores (no reference
+// counterpart); same signature, so it registers as one more DR variant
+template <int S>
+inline bool tsmttsm_ozaki(size_t bc, int M, int N, int K, const double* A,
+                          int lda, double alpha, const double* B, int ldb,
+                          double beta, double* C, int ldc) {
+  return status_to_bool(skinny_dtsmttsm_ozaki(nullptr, bc, M, N, K, A, lda,
+                                              alpha, B, ldb, beta, C, ldc, S,
+                                              nullptr));
+}
+
 template <typename T>
 bool tsmm(size_t blockCount, int M, int N, int K, const T* A, int lda, T alpha,
           const T* B, int ldb, T beta, T* C, int ldc);
@@ -136,6 +174,23 @@ inline bool tsmm<float>(size_t bc, int M, int N, int K, const float* A,
                         int lda, float alpha, const float* B, int ldb,
                         float beta, float* C, int ldc) {
   return status_to_bool(skinny_stsmm(nullptr, bc, M, N, K, A, lda, alpha, B,
+                                     ldb, beta, C, ldc, nullptr));
+}
+
+template <>
+inline bool tsmm<skinny_double_complex>(
+    size_t bc, int M, int N, int K, const skinny_double_complex* A, int lda,
+    skinny_double_complex alpha, const skinny_double_complex* B, int ldb,
+    skinny_double_complex beta, skinny_double_complex* C, int ldc) {
+  return status_to_bool(skinny_ztsmm(nullptr, bc, M, N, K, A, lda, alpha, B,
+                                     ldb, beta, C, ldc, nullptr));
+}
+template <>
+inline bool tsmm<skinny_float_complex>(
+    size_t bc, int M, int N, int K, const skinny_float_complex* A, int lda,
+    skinny_float_complex alpha, const skinny_float_complex* B, int ldb,
+    skinny_float_complex beta, skinny_float_complex* C, int ldc) {
+  return status_to_bool(skinny_ctsmm(nullptr, bc, M, N, K, A, lda, alpha, B,
                                      ldb, beta, C, ldc, nullptr));
 }
 
@@ -161,20 +216,44 @@ using MatmulFunction = std::function<bool(
     const size_t, const int, const int, const int, const dtype*, const int,
     const dtype, const dtype*, const int, const dtype, dtype*, const int)>;
 
+template <typename dtype>
+using VersionList = std::vector<std::pair<MatmulFunction<dtype>, std::string>>;
+
+// Registries per type set (versions.hpp:47-177 selects them by macro; the
+// macro-selected globals below forward here).  "SKINNY" is the library's
+// variant for the type; DR TSMTTSM also lists the Ozaki INT8 split.
+template <class Types>
+VersionList<typename Types::dtype> tsmttsm_versions() {
+  typedef typename Types::dtype dt;
+  VersionList<dt> v;
+  v.push_back({tsmttsm<dt, typename Types::dItype>, "SKINNY"});
+  if constexpr (std::is_same<Types, TypeDR>::value) {
+    v.push_back({tsmttsm_ozaki<4>, "OZAKI4"});
+  }
+  return v;
+}
+template <class Types>
+VersionList<typename Types::dtype> tsmm_versions() {
+  typedef typename Types::dtype dt;
+  VersionList<dt> v;
+  v.push_back({tsmm<dt>, "SKINNY"});
+  return v;
+}
+
 }  // namespace skinny
 
 // ---- reference-style globals, selected by one type macro -------------------
 #if defined(FR) || defined(DR) || defined(FC) || defined(DC) || defined(HR)
 #if defined(FR)
-typedef skinny::FR skinny_types_t;
+typedef skinny::TypeFR skinny_types_t;
 #elif defined(DR)
-typedef skinny::DR skinny_types_t;
+typedef skinny::TypeDR skinny_types_t;
 #elif defined(FC)
-typedef skinny::FC skinny_types_t;
+typedef skinny::TypeFC skinny_types_t;
 #elif defined(DC)
-typedef skinny::DC skinny_types_t;
+typedef skinny::TypeDC skinny_types_t;
 #else
-typedef skinny::HR skinny_types_t;
+typedef skinny::TypeHR skinny_types_t;
 #endif
 typedef skinny_types_t::htype htype;
 typedef skinny_types_t::dtype dtype;
@@ -186,14 +265,10 @@ using MatmulFunctionType = skinny::MatmulFunction<dtype>;
 
 inline std::vector<std::pair<MatmulFunctionType, std::string>>
 getEnabledTSMTTSMVersions() {
-  std::vector<std::pair<MatmulFunctionType, std::string>> v;
-  v.push_back({skinny::tsmttsm<dtype, dItype>, "SKINNY"});
-  return v;
+  return skinny::tsmttsm_versions<skinny_types_t>();
 }
 inline std::vector<std::pair<MatmulFunctionType, std::string>>
 getEnabledTSMMVersions() {
-  std::vector<std::pair<MatmulFunctionType, std::string>> v;
-  v.push_back({skinny::tsmm<dtype>, "SKINNY"});
-  return v;
+  return skinny::tsmm_versions<skinny_types_t>();
 }
 #endif

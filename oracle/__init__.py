@@ -66,6 +66,9 @@ def lib():
             "orc_dd_slow_axpy2": (None, [vp, dbl, dbl, vp]),
             "orc_dd_scale2": (None, [vp, dbl, vp]),
             "orc_dd_renormalize": (None, [vp, vp]),
+            "orc_dda_add": (None, [vp, vp, vp]),
+            "orc_dda_add3": (None, [vp, vp, vp]),
+            "orc_dda_value": (None, [vp, vp]),
             "orc_htsmttsm": (None, [i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64]),
             "orc_htsmttsm_dd": (None, [i64, i64, i64, vp, i64, dbl, vp, i64, vp, vp, i64]),
             "orc_xdot": (None, [i64, vp, i64, vp, i64, vp, vp]),
@@ -246,6 +249,24 @@ def dd_slow_axpy2(c, a, b):
 def dd_scale2(x, b):
     r = np.zeros(2)
     lib().orc_dd_scale2(_p(np.array(x, dtype=np.float64)), float(b), _p(r))
+    return float(r[0]), float(r[1])
+
+
+def dda_add(x3, y):
+    """(s, t, u) + dd y (or + triple y, whose third word is added to u)."""
+    r = np.zeros(3)
+    if len(y) == 3:
+        lib().orc_dda_add3(_p(np.array(x3, dtype=np.float64)),
+                           _p(np.array(y, dtype=np.float64)), _p(r))
+    else:
+        lib().orc_dda_add(_p(np.array(x3, dtype=np.float64)),
+                          _p(np.array(y, dtype=np.float64)), _p(r))
+    return float(r[0]), float(r[1]), float(r[2])
+
+
+def dda_value(x3):
+    r = np.zeros(2)
+    lib().orc_dda_value(_p(np.array(x3, dtype=np.float64)), _p(r))
     return float(r[0]), float(r[1])
 
 

@@ -487,6 +487,52 @@ void orc_dd_renormalize(const double* x, double* r) {
   r[1] = o.t;
 }
 
+/* dd plus the rounding remainders of dd_accu's two inexact additions (c and
+ * w) in a third word u; (s, t) is bitwise dd_accu's result.  The HR kernel
+ * combines its partial sums with it (no reference counterpart). */
+typedef struct {
+  dd_t v;
+  double u;
+} dda_t;
+
+static inline dda_t dda_add(dda_t x, dd_t y) {
+  dd_t S = gen2sum(x.v.s, y.s);
+  dd_t T = gen2sum(x.v.t, y.t);
+  dd_t Cc = gen2sum(S.t, T.s);
+  dd_t V = fast2sum(S.s, Cc.s);
+  dd_t W = gen2sum(T.t, V.t);
+  dda_t r;
+  r.v = fast2sum(V.s, W.s);
+  r.u = x.u + (Cc.t + W.t);
+  return r;
+}
+
+void orc_dda_add(const double* x, const double* y, double* r) {
+  dda_t a = {{x[0], x[1]}, x[2]};
+  dd_t b = {y[0], y[1]};
+  dda_t o = dda_add(a, b);
+  r[0] = o.v.s;
+  r[1] = o.v.t;
+  r[2] = o.u;
+}
+/* a triple plus a triple: the dd part as above, then the third words */
+void orc_dda_add3(const double* x, const double* y, double* r) {
+  dda_t a = {{x[0], x[1]}, x[2]};
+  dd_t b = {y[0], y[1]};
+  dda_t o = dda_add(a, b);
+  o.u = o.u + y[2];
+  r[0] = o.v.s;
+  r[1] = o.v.t;
+  r[2] = o.u;
+}
+/* the triple's value as dd: dd_accu((s, t), (u, 0)) */
+void orc_dda_value(const double* x, double* r) {
+  dd_t a = {x[0], x[1]}, b = {x[2], 0.0};
+  dd_t o = dd_accu(a, b);
+  r[0] = o.s;
+  r[1] = o.t;
+}
+
 static dd_t hr_sum(int64_t K, const double* A, int64_t lda, const double* B,
                    int64_t ldb, int64_t m, int64_t n, double alpha) {
   dd_t s = {0.0, 0.0};
@@ -768,14 +814,25 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
             int e = OZ_EMIN;
             if (mx > 0.0) frexp(mx, &e);
             ecur[mat][c] = e;
-            if (e > erun[mat][c]) flush = 1;
+            if (e > erun[mat][c]) flush = 1; /* growth */
+            /* shrink: the chunk's largest limb integer q = floor(mx 2^(7 S4 -
+             * E)) is nonzero but 9 or more bits below the scale, so the
+             * scale restarts from this chunk (an outlier coarsens only its
+             * own chunk) */
+            if (erun[mat][c] != OZ_EMIN && mx > 0.0) {
+              int S4 = S < 4 ? S : 4;
+              double t = ldexp(mx, 7 * S4 - erun[mat][c]);
+              if (t >= 1.0 && t < ldexp(1.0, 7 * S4 - 9)) flush = 1;
+            }
           }
         if (flush) {
+          /* the accumulators are empty after a flush, so every column's
+           * scale restarts from this chunk: an outlier only coarsens the
+           * window it falls in */
           oz_flush_tile(S, acc, cacc, erun[0], erun[1]);
           since = 0;
           for (int mat = 0; mat < 2; mat++)
-            for (int c = 0; c < OZ_TILE; c++)
-              if (ecur[mat][c] > erun[mat][c]) erun[mat][c] = ecur[mat][c];
+            for (int c = 0; c < OZ_TILE; c++) erun[mat][c] = ecur[mat][c];
         }
         /* limbs */
         for (int mat = 0; mat < 2; mat++) {

@@ -92,6 +92,9 @@ void orc_dd_axpy2(const double* c, double a, double b, double* r);
 void orc_dd_slow_axpy2(const double* c, double a, double b, double* r);
 void orc_dd_scale2(const double* x, double b, double* r);
 void orc_dd_renormalize(const double* x, double* r);
+void orc_dda_add(const double* x3, const double* y2, double* r3);
+void orc_dda_add3(const double* x3, const double* y3, double* r3);
+void orc_dda_value(const double* x3, double* r2);
 
 /* HR: double inputs, PseudoQuad accumulation (slow_axpy2, the accurate
  * TwoProd + dd add of PseudoQuad.cuh:115-118) in sequential k order, then

@@ -12,8 +12,12 @@
 //    only double-accurate when the running sum cancels.
 //  * Row slots of a workgroup are combined in LDS in a fixed order, then the
 //    workgroup partials with the same two-level last-arrival reduction as the
-//    FP64 kernel, all in dd arithmetic, so the result is reproducible for a
-//    fixed blockCount.
+//    FP64 kernel, so the result is reproducible for a fixed blockCount.  The
+//    combine stages add dd partials with DdAcc (dd plus the rounding
+//    remainders of each addition) and store partials as (s, t, u) triples:
+//    partials of very different magnitude (2^124 next to 2^62 in the
+//    numeric_tests.cu:23-53 data) would otherwise round the small terms
+//    away, and the result would depend on blockCount.
 //  * The final step is the reference's: renormalize, scale by alpha (dd * d),
 //    then either convert to double and add beta * C, or keep the dd pair.
 #include "internal.hpp"
@@ -29,14 +33,19 @@ struct HrArgs {
   int M, N;
   double alpha, beta;
   int dd_out;
-  PseudoQuad* part1;  // [tile][G][PS]
-  PseudoQuad* part2;  // [tile][ngroups][PS]
+  double* part1;  // [tile][G][PS] (s, t, u) triples
+  double* part2;  // [tile][ngroups][PS] triples
   unsigned* cnt;
   int G, GS, ngroups, tilesM, tilesN, tileM, tileN;
 };
 
-__device__ __forceinline__ PseudoQuad ld_dd(const PseudoQuad* p) {
-  return *p;
+__device__ __forceinline__ DdAcc ld_tri(const double* p) {
+  return DdAcc{PseudoQuad{p[0], p[1]}, p[2]};
+}
+__device__ __forceinline__ void st_tri(double* p, DdAcc v) {
+  p[0] = v.v.s;
+  p[1] = v.v.t;
+  p[2] = v.u;
 }
 
 template <int TM, int TN>
@@ -98,16 +107,16 @@ __global__ void __launch_bounds__(256) htsmttsm_kernel(HrArgs a) {
     for (int j = 0; j < TN; j++) red[(i * TN + j) * 256 + tid] = acc[i][j];
   __syncthreads();
   const int PS = a.tileM * a.tileN;
-  PseudoQuad* mine = a.part1 + ((int64_t)tile * a.G + kp) * PS;
+  double* mine = a.part1 + ((int64_t)tile * a.G + kp) * PS * 3;
   for (int e = tid; e < PS; e += 256) {
     const int ml = e % a.tileM, nl = e / a.tileM;
-    PseudoQuad s{0.0, 0.0};
+    DdAcc s{{0.0, 0.0}, 0.0};
     if (ml < Mloc && nl < Nloc) {
       const int qq = (ml / TM) + (nl / TN) * QM;
       const int cell = (ml % TM) * TN + (nl % TN);
-      for (int sl = 0; sl < slots; sl++) s = dd_add(s, red[cell * 256 + sl * Q + qq]);
+      for (int sl = 0; sl < slots; sl++) s = dda_add(s, red[cell * 256 + sl * Q + qq]);
     }
-    mine[e] = s;
+    st_tri(mine + (int64_t)e * 3, s);
   }
 
   // ---- two-level last-arrival reduction ------------------------------------
@@ -127,14 +136,14 @@ __global__ void __launch_bounds__(256) htsmttsm_kernel(HrArgs a) {
   if (!s_last) return;
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
-  const PseudoQuad* src = a.part1 + ((int64_t)tile * a.G + (int64_t)grp * a.GS) * PS;
+  const double* src = a.part1 + ((int64_t)tile * a.G + (int64_t)grp * a.GS) * PS * 3;
   int count = gsz;
   if (a.ngroups > 1) {
-    PseudoQuad* dst = a.part2 + ((int64_t)tile * a.ngroups + grp) * PS;
+    double* dst = a.part2 + ((int64_t)tile * a.ngroups + grp) * PS * 3;
     for (int e = tid; e < PS; e += 256) {
-      PseudoQuad s{0.0, 0.0};
-      for (int g = 0; g < gsz; g++) s = dd_add(s, ld_dd(src + (int64_t)g * PS + e));
-      dst[e] = s;
+      DdAcc s{{0.0, 0.0}, 0.0};
+      for (int g = 0; g < gsz; g++) s = dda_add(s, ld_tri(src + ((int64_t)g * PS + e) * 3));
+      st_tri(dst + (int64_t)e * 3, s);
     }
     if (tid == 0) *cnt1 = 0u;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -147,16 +156,16 @@ __global__ void __launch_bounds__(256) htsmttsm_kernel(HrArgs a) {
     __syncthreads();
     if (!s_last) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    src = a.part2 + (int64_t)tile * a.ngroups * PS;
+    src = a.part2 + (int64_t)tile * a.ngroups * PS * 3;
     count = a.ngroups;
   }
   for (int e = tid; e < PS; e += 256) {
     const int ml = e % a.tileM, nl = e / a.tileM;
     if (ml >= Mloc || nl >= Nloc) continue;
-    PseudoQuad s{0.0, 0.0};
-    for (int g = 0; g < count; g++) s = dd_add(s, ld_dd(src + (int64_t)g * PS + e));
+    DdAcc s{{0.0, 0.0}, 0.0};
+    for (int g = 0; g < count; g++) s = dda_add(s, ld_tri(src + ((int64_t)g * PS + e) * 3));
     // genv3.cuh:25-26: beta * C + convert(scale2(renormalize(sum), alpha))
-    const PseudoQuad r = dd_scale(dd_renorm(s), a.alpha);
+    const PseudoQuad r = dd_scale(dd_renorm(dda_value(s)), a.alpha);
     const int64_t ci = (int64_t)(n0 + nl) * a.ldc + (m0 + ml);
     if (a.dd_out) {
       PseudoQuad o = r;
@@ -228,8 +237,8 @@ skinny_status_t launch_htsmttsm(skinny_handle_t h, size_t blockCount,
   if ((size_t)ntiles * (a.ngroups + 1) > (size_t)kMaxCounters)
     return fail(SKINNY_UNSUPPORTED, "htsmttsm: too many partial groups");
   const size_t PS = (size_t)a.tileM * a.tileN;
-  size_t bytes = (size_t)ntiles * a.G * PS * sizeof(PseudoQuad);
-  if (a.ngroups > 1) bytes += (size_t)ntiles * a.ngroups * PS * sizeof(PseudoQuad);
+  size_t bytes = (size_t)ntiles * a.G * PS * 3 * sizeof(double);
+  if (a.ngroups > 1) bytes += (size_t)ntiles * a.ngroups * PS * 3 * sizeof(double);
   skinny_status_t st = ensure_workspace(h, bytes);
   if (st != SKINNY_OK) return st;
   a.A = A;
@@ -244,8 +253,8 @@ skinny_status_t launch_htsmttsm(skinny_handle_t h, size_t blockCount,
   a.N = (int)N;
   a.alpha = alpha;
   a.beta = beta;
-  a.part1 = static_cast<PseudoQuad*>(ws_scratch(h));
-  a.part2 = a.part1 + (size_t)ntiles * a.G * PS;
+  a.part1 = static_cast<double*>(ws_scratch(h));
+  a.part2 = a.part1 + (size_t)ntiles * a.G * PS * 3;
   a.cnt = ws_counters(h);
   const int blocks = a.G * ntiles;
   switch (TMc) {

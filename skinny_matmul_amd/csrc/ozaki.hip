@@ -3,8 +3,9 @@
 //
 // C = alpha A^T B + beta C with A (K x M), B (K x N) FP64.  The rows are taken
 // in chunks of 32 (one MFMA k-depth):
-//  * Each column c has a running scale exponent E_c: every |x| seen so far is
-//    below 2^E_c.  An element x is split into S signed 7-bit limbs,
+//  * Each column c has a running scale exponent E_c: every |x| since the last
+//    flush is below 2^E_c (after a flush the scale restarts from the next
+//    chunk's largest |x|).  An element x is split into S signed 7-bit limbs,
 //      y = x 2^-E;  for s = 1..S: y = 128 y; d_s = trunc(y); y = y - d_s,
 //    so x = 2^E (sum_s d_s 2^-7s + r 2^-7S), |d_s| <= 127, |r| < 1; every
 //    step is exact in FP64 (Ozaki et al. 2012, error-free splitting).
@@ -13,8 +14,9 @@
 //    sum_k d_s(k, m) d'_t(k, n) accumulate exactly in INT32 through
 //    v_mfma_i32_32x32x32_i8, one accumulator per level.
 //  * The INT32 accumulators are converted to FP64 ("flushed") only when a
-//    column's magnitude grows past its scale (new E_c) or before INT32 could
-//    overflow (every 16384 rows); the flush forms
+//    column's magnitude grows past its scale, when a chunk's largest limb
+//    integer falls 9 or more bits below it (the chunk after an outlier), or
+//    before INT32 could overflow (every 16384 rows); the flush forms
 //      V = sum_{g = S + 1 .. 2} acc_g 2^-7g   (in this order, FP64),
 //      C_acc(m, n) += V 2^(E_m + E_n).
 //    For data of steady magnitude the conversion cost is amortised over
@@ -184,8 +186,8 @@ __device__ __forceinline__ void oz_limbs(double x, int E, int (&d)[S]) {
 // sign bytes come from v_perm's sign-replicating selectors (9: bit 31 of
 // the second operand, 11: bit 31 of the first).
 //
-// Returns the largest q >> (7 min(S, 4)): nonzero iff some |x| >= 2^E (the
-// conversion saturates), i.e. iff a row outgrew the scale.
+// Returns the largest q: q >= 2^(7 min(S, 4)) iff some |x| >= 2^E (the
+// conversion saturates), i.e. a row outgrew the scale.
 template <int S>
 __device__ __forceinline__ unsigned oz_pack4(const double (&xv)[4], int E,
                                              int (&w)[S]) {
@@ -211,7 +213,7 @@ __device__ __forceinline__ unsigned oz_pack4(const double (&xv)[4], int E,
     const unsigned nb = (0x80808080u - u) ^ 0x80808080u;
     w[s] = (int)((nb & m) | (u & ~m));
   }
-  return max(max(q[0], q[1]), max(q[2], q[3])) >> (7 * S4);
+  return max(max(q[0], q[1]), max(q[2], q[3]));
 }
 
 // Calls f(integral_constant<int, I>) for I = 0 .. N - 1 in order: unrolled
@@ -382,7 +384,9 @@ __global__ void __launch_bounds__(kOzThreads)
   // chunk staged in buffer bm is multiplied.  The splits of 8 groups of 4
   // rows and the 2 S (S + 1) products are issued interleaved, in fixed units
   // separated by scheduling fences, so the VALU work runs while the matrix
-  // core is busy.  Returns nonzero if a value outgrew its column's scale.
+  // core is busy.  Returns nonzero if the column's scale must restart: a
+  // value outgrew it, or the chunk's largest limb integer is nonzero but 9
+  // or more bits below it (the chunk after an outlier).
   auto split_multiply = [&](int bm, int bs) {
     v4i av[2][S], bv[2][S];
 #pragma unroll
@@ -393,7 +397,7 @@ __global__ void __launch_bounds__(kOzThreads)
         bv[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][1][s2][64 * wn + 32 * h + (lane & 31)][kh]);
       }
     const int Es = split_scale();
-    unsigned over = 0;
+    unsigned qmax = 0;
     int words[S][4];
     oz_static_for(
         [&](auto U_) {
@@ -401,7 +405,7 @@ __global__ void __launch_bounds__(kOzThreads)
           {
             const double xv[4] = {x[4 * U], x[4 * U + 1], x[4 * U + 2], x[4 * U + 3]};
             int w[S];
-            over |= oz_pack4<S>(xv, Es, w);
+            qmax = max(qmax, oz_pack4<S>(xv, Es, w));
 #pragma unroll
             for (int s2 = 0; s2 < S; s2++) words[s2][U % 4] = w[s2];
           }
@@ -425,7 +429,8 @@ __global__ void __launch_bounds__(kOzThreads)
           __builtin_amdgcn_sched_barrier(0);
         },
         std::make_integer_sequence<int, kOzRows / 4>{});
-    return over;
+    constexpr int S4 = S < 4 ? S : 4;
+    return (qmax >> (7 * S4)) != 0u || (qmax != 0u && qmax < (1u << (7 * S4 - 9)));
   };
 
   // Software pipeline: while chunk c is split (VALU) into one LDS buffer,
@@ -440,12 +445,10 @@ __global__ void __launch_bounds__(kOzThreads)
   load_chunk(min(ch, nch - 1), x);
   int pb = 0;
   while (ch < nch) {
-    // (re)start at chunk ch: accumulators empty, nothing staged
-    {
-      const int e = chunk_exp();
-      const int eold = erun[mat][col];
-      erun[mat][col] = max(eold, e);  // only this thread touches the entry
-    }
+    // (re)start at chunk ch: accumulators empty, nothing staged, so every
+    // column's scale restarts from this chunk (an outlier only coarsens the
+    // window it falls in); only this thread touches the entry
+    erun[mat][col] = chunk_exp();
     __syncthreads();
     load_chunk(min(ch + a.G, nch - 1), xn);
     limbs(pb);
@@ -458,8 +461,8 @@ __global__ void __launch_bounds__(kOzThreads)
       oz_pin<S>(acc);
       if (since + kOzRows > kOzFlushRows) break;  // uniform
       load_chunk(min(ch + a.G, nch - 1), xn);
-      const unsigned over = split_multiply(pb, pb ^ 1);
-      if (__syncthreads_or(over != 0)) {
+      const bool rescale = split_multiply(pb, pb ^ 1);
+      if (__syncthreads_or(rescale)) {
         staged = false;  // buffer pb was multiplied; ch is split again
         break;
       }

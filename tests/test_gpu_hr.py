@@ -42,11 +42,11 @@ def _emulate_g1(orc, a, b, alpha):
         for r in range(t, K, 256):
             s = orc.dd_slow_axpy2(s, a[r], b[r])
         slots.append(s)
-    part = (0.0, 0.0)
-    for s in slots:  # slot combination in LDS
-        part = orc.dd_accu(part, s)
-    tot = orc.dd_accu((0.0, 0.0), part)  # last arrival over the single partial
-    return orc.dd_scale2(orc.dd_renormalize(tot), alpha)
+    part = (0.0, 0.0, 0.0)
+    for s in slots:  # slot combination in LDS, dd plus rounding remainders
+        part = orc.dda_add(part, s)
+    tot = orc.dda_add((0.0, 0.0, 0.0), part)  # last arrival over the single partial
+    return orc.dd_scale2(orc.dd_renormalize(orc.dda_value(tot)), alpha)
 
 
 def test_kat_bitwise_against_emulation(gpu, orc):
@@ -67,6 +67,23 @@ def test_kat_bitwise_against_emulation(gpu, orc):
     assert gpu.dtsmttsm(1000, 1, 1, N, dA, 1, 1.0, dB, 1, 0.0, C64, 1)
     note(f"KAT: double {host(C64)[0]!r}  PseudoQuad {want[0] + want[1]!r}  "
          f"exact {d['exact_hi']!r}  (numeric_tests.cu prints {d['reference_printed']!r})")
+
+
+@pytest.mark.parametrize("bc", [1, 2, 16, 17, 100, 435, 1000, 4000])
+def test_kat_accurate_any_blockcount(gpu, bc):
+    """The cancellation KAT at any blockCount: partials near 2^124, 2^62 and
+    1 are combined with the rounding remainders kept, so the dd result stays
+    within a few ulp of the exact value whatever the partition (plain dd
+    partial sums lost up to 14% of it)."""
+    d, a, b = _kat()
+    N = d["N"]
+    note(f"htsmttsm KAT K={N} blockCount={bc}")
+    dA, dB = dev(a), dev(b)
+    Cd = dev(np.zeros(1))
+    assert gpu.htsmttsm(bc, 1, 1, N, dA, 1, 1.0, dB, 1, 0.0, Cd, 1)
+    got = host(Cd)[0]
+    ex = d["exact_hi"]
+    assert abs(got - ex) <= 4 * np.spacing(ex), (bc, got, ex)
 
 
 def _rel_dd(hi, lo, ehi, elo):

@@ -51,6 +51,23 @@ def test_bitwise_growth(gpu, orc):
     assert np.array_equal(host(dC), want)
 
 
+def test_bitwise_outlier(gpu, orc):
+    """An outlier chunk followed by ordinary data: the growth flush and the
+    scale restart of the next chunk, bitwise against the oracle."""
+    M, N, K = 6, 5, 4000
+    note(f"ozaki outlier M={M} N={N} K={K}")
+    A = orc.uniform(K * M, 21, 0).copy()
+    B = orc.uniform(K * N, 21, 1).copy()
+    A[100 * M] = 1e5
+    B[2500 * N + 3] = -3e7
+    for S in (3, 4):
+        want = np.zeros(M * N)
+        orc.ozaki_tsmttsm(M, N, K, A, M, 1.0, B, N, 0.0, want, M, S)
+        dC = dev(np.zeros(M * N))
+        assert gpu.dtsmttsm_ozaki(1, M, N, K, dev(A), M, 1.0, dev(B), N, 0.0, dC, M, slices=S)
+        assert np.array_equal(host(dC), want), S
+
+
 @pytest.mark.parametrize("S", [4, 5])
 def test_c3_error(gpu, orc, S):
     M = 128

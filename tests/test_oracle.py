@@ -304,6 +304,38 @@ def _kat_input():
     return d, a, b
 
 
+def test_dda_add_keeps_rounding_remainders(orc):
+    """dda_add: (s, t) is bitwise dd_accu, and s + t + u equals the exact sum
+    up to the rounding of the third word (checked with exact rationals),
+    including dd partials of very different magnitudes."""
+    rng = random.Random(5)
+    cases = [((2.0 ** 124, 2.55, 0.0), (2.0 ** 62 * 0.1, 2.56)),
+             ((2.0 ** 124, 4.6e17, 3.5), (-(2.0 ** 124), 1e-3)),
+             ((1.0, 2.0 ** -60, 0.0), (-1.0, 2.0 ** -70))]
+    for _ in range(300):
+        x = _rand_doubles(rng, 2)
+        y = _rand_doubles(rng, 2)
+        xs, xt = orc.dd_renormalize((max(x, key=abs), min(x, key=abs)))
+        ys, yt = orc.dd_renormalize((max(y, key=abs), min(y, key=abs)))
+        cases.append(((xs, xt, 0.0), (ys, yt)))
+    for x3, y in cases:
+        r = orc.dda_add(x3, y)
+        assert (r[0], r[1]) == orc.dd_accu(x3[:2], y)
+        exact = sum(Fraction(v) for v in x3) + Fraction(y[0]) + Fraction(y[1])
+        got = Fraction(r[0]) + Fraction(r[1]) + Fraction(r[2])
+        assert abs(got - exact) <= abs(Fraction(r[2])) * Fraction(1, 2 ** 52)
+    # the numeric_tests.cu situation: 2^124 and 2^62-sized partials with small
+    # terms; plain dd loses the small terms, the triple keeps them exactly
+    big, mid = 2.0 ** 124, 2.0 ** 62 * 0.1
+    parts = [(big, 2.5), (mid, 1.25), (-mid, 0.75), (-big, 0.5)]
+    acc, dd = (0.0, 0.0, 0.0), (0.0, 0.0)
+    for p in parts:
+        acc = orc.dda_add(acc, p)
+        dd = orc.dd_accu(dd, p)
+    assert sum(orc.dda_value(acc)) == 5.0
+    assert sum(dd) != 5.0
+
+
 def test_kat_numeric_tests_exact(orc):
     d, a, b = _kat_input()
     hi, lo = orc.xdot(a, b)

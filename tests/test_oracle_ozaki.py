@@ -61,3 +61,25 @@ def test_alpha_beta_and_growth(orc):
     want = C0.copy()
     orc.tsmttsm(M, N, K, A, M, 2.0, B, N, -0.5, want, M)
     assert np.linalg.norm(got - want) <= 1e-9 * np.linalg.norm(want)
+
+
+def test_outlier_coarsens_only_its_chunk(orc):
+    """A 1e5 outlier in column 0 raises that column's scale for its own
+    32-row chunk only: the next chunk is 9+ bits below the scale and restarts
+    it.  Column 0's error stays near one chunk's share (a scale kept for the
+    whole run would make it about K / 32 times larger) and the other
+    columns keep the scheme's accuracy."""
+    M = N = 6
+    K = 6000
+    A = orc.uniform(K * M, 9, 0).copy()
+    B = orc.uniform(K * N, 9, 1)
+    A[100 * M] = 1e5
+    hi, lo = orc.xtsmttsm(M, N, K, A, M, B, N)
+    keep = np.ones(K, bool)
+    keep[100] = False
+    ref = np.abs(A.reshape(K, M)[keep]).T @ np.abs(B.reshape(K, N)[keep])
+    got = np.zeros(M * N)
+    orc.ozaki_tsmttsm(M, N, K, A, M, 1.0, B, N, 0.0, got, M, 4)
+    err = np.abs((got - hi) - lo).reshape(N, M).T / ref
+    assert err[0].max() <= 1e-5, err[0].max()
+    assert err[1:].max() <= 8 * 5 * 2.0 ** -28, err[1:].max()
