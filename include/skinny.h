@@ -75,9 +75,15 @@ typedef struct {
  * a per-handle workspace on the device that is current at creation. */
 skinny_status_t skinny_create(skinny_handle_t* handle);
 skinny_status_t skinny_destroy(skinny_handle_t handle);
-/* Grow the workspace to at least `bytes` now (allocation happens here, not in
- * later calls); required before capturing calls into a hipGraph. */
+/* Each stream a handle is used on gets its own workspace, so calls on
+ * different streams may overlap; calls on one stream are ordered by it.
+ * skinny_reserve_stream grows the workspace of `stream` to at least `bytes`
+ * of scratch now (allocation happens here, not in later calls), as required
+ * before capturing calls on that stream into a hipGraph; skinny_reserve does
+ * it for the null stream. */
 skinny_status_t skinny_reserve(skinny_handle_t handle, size_t bytes);
+skinny_status_t skinny_reserve_stream(skinny_handle_t handle, size_t bytes,
+                                      skinny_stream_t stream);
 /* Workspace bytes a TSMTTSM call of this shape needs (elem = 4, 8 or 16). */
 size_t skinny_tsmttsm_workspace(skinny_handle_t handle, size_t blockCount,
                                 int64_t M, int64_t N, int64_t K,

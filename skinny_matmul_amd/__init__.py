@@ -84,6 +84,7 @@ _SIGS = {
     "skinny_create": (C.c_int, [C.POINTER(vp)]),
     "skinny_destroy": (C.c_int, [vp]),
     "skinny_reserve": (C.c_int, [vp, sz]),
+    "skinny_reserve_stream": (C.c_int, [vp, sz, vp]),
     "skinny_tsmttsm_workspace": (sz, [vp, sz, i64, i64, i64, C.c_int]),
     "skinny_default_handle": (vp, []),
     "skinny_status_string": (C.c_char_p, [C.c_int]),
@@ -199,8 +200,10 @@ class Handle:
     def value(self) -> int:
         return self._h
 
-    def reserve(self, nbytes: int) -> None:
-        _check(lib().skinny_reserve(self._h, nbytes), False)
+    def reserve(self, nbytes: int, stream=None) -> None:
+        """Size the workspace of `stream` (default: torch's current stream)
+        now, e.g. before capturing calls on it into a graph."""
+        _check(lib().skinny_reserve_stream(self._h, nbytes, _stream(stream)), False)
 
     def close(self) -> None:
         if self._h:

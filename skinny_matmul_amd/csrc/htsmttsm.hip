@@ -239,7 +239,8 @@ skinny_status_t launch_htsmttsm(skinny_handle_t h, size_t blockCount,
   const size_t PS = (size_t)a.tileM * a.tileN;
   size_t bytes = (size_t)ntiles * a.G * PS * 3 * sizeof(double);
   if (a.ngroups > 1) bytes += (size_t)ntiles * a.ngroups * PS * 3 * sizeof(double);
-  skinny_status_t st = ensure_workspace(h, bytes);
+  Workspace w;
+  skinny_status_t st = acquire_workspace(h, stream, bytes, &w);
   if (st != SKINNY_OK) return st;
   a.A = A;
   a.B = B;
@@ -253,9 +254,9 @@ skinny_status_t launch_htsmttsm(skinny_handle_t h, size_t blockCount,
   a.N = (int)N;
   a.alpha = alpha;
   a.beta = beta;
-  a.part1 = static_cast<double*>(ws_scratch(h));
+  a.part1 = static_cast<double*>(ws_scratch(w));
   a.part2 = a.part1 + (size_t)ntiles * a.G * PS * 3;
-  a.cnt = ws_counters(h);
+  a.cnt = ws_counters(w);
   const int blocks = a.G * ntiles;
   switch (TMc) {
     case 1: hipLaunchKernelGGL((htsmttsm_kernel<1, 1>), dim3(blocks), dim3(256), 0, stream, a); break;

@@ -9,6 +9,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <vector>
 
 #include "skinny.h"
 
@@ -24,15 +25,24 @@ struct TuneEntry {
   int uvar;  // unroll variant (TSMTTSM)
   int G;     // workgroups along K
 };
+// Device workspace of one stream: [counter region (zeroed at allocation,
+// reset to zero by the kernels that use it; its last kZeroBytes are never
+// written)] [scratch region].
+struct Workspace {
+  char* base = nullptr;
+  size_t bytes = 0;
+};
 }  // namespace skinny
 
 struct skinny_handle_s {
   int device = 0;
   int cu_count = 256;
-  // Workspace: [counter region (zeroed at allocation, reset to zero by the
-  // kernels that use it)] [scratch region].
-  void* ws = nullptr;
-  size_t ws_bytes = 0;
+  // One workspace per stream: calls on one stream are ordered by it, calls
+  // on different streams may overlap.  Outgrown buffers are retired, not
+  // freed, until the handle is destroyed, so a call enqueued earlier never
+  // sees its buffer disappear.
+  std::map<hipStream_t, skinny::Workspace> ws;
+  std::vector<void*> retired;
   std::mutex mu;
   // autotuned choices for blockCount == 0
   std::map<skinny::TuneKey, skinny::TuneEntry> tune;
@@ -55,19 +65,20 @@ skinny_status_t hip_fail(hipError_t e, const char* what);
 // error is included)
 skinny_status_t no_handle(const std::string& fn);
 
-// Make sure the handle's scratch region holds `scratch_bytes` (grows the
-// allocation, synchronising the device, only when it is too small).
-skinny_status_t ensure_workspace(skinny_handle_t h, size_t scratch_bytes);
+// The workspace of `stream` with at least `scratch_bytes` of scratch: made on
+// first use and grown geometrically when too small (growth allocates, zeroes
+// the new counters and synchronises the device; it never happens once the
+// stream's workspace is large enough, e.g. after skinny_reserve_stream).
+skinny_status_t acquire_workspace(skinny_handle_t h, hipStream_t stream,
+                                  size_t scratch_bytes, Workspace* out);
 
-inline unsigned int* ws_counters(skinny_handle_t h) {
-  return static_cast<unsigned int*>(h->ws);
+inline unsigned int* ws_counters(const Workspace& w) {
+  return reinterpret_cast<unsigned int*>(w.base);
 }
-inline const double* ws_zeros(skinny_handle_t h) {
-  return reinterpret_cast<const double*>(static_cast<char*>(h->ws) + kCounterBytes - kZeroBytes);
+inline const double* ws_zeros(const Workspace& w) {
+  return reinterpret_cast<const double*>(w.base + kCounterBytes - kZeroBytes);
 }
-inline void* ws_scratch(skinny_handle_t h) {
-  return static_cast<char*>(h->ws) + kCounterBytes;
-}
+inline void* ws_scratch(const Workspace& w) { return w.base + kCounterBytes; }
 
 inline skinny_handle_t resolve(skinny_handle_t h) {
   return h ? h : skinny_default_handle();

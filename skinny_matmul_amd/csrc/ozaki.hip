@@ -604,7 +604,8 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
   if (a.ngroups > 1) bytes += (size_t)ntiles * a.ngroups * PS * sizeof(double);
   const size_t part_bytes = bytes;
   bytes += (size_t)ntiles * a.G * (kOzThreads / 64) * slices * 4096 * sizeof(int);
-  skinny_status_t st = ensure_workspace(h, bytes);
+  Workspace w;
+  skinny_status_t st = acquire_workspace(h, (hipStream_t)stream, bytes, &w);
   if (st != SKINNY_OK) return st;
   a.A = A;
   a.B = B;
@@ -617,11 +618,11 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
   a.N = (int)N;
   a.alpha = alpha;
   a.beta = beta;
-  a.part1 = static_cast<double*>(ws_scratch(h));
+  a.part1 = static_cast<double*>(ws_scratch(w));
   a.part2 = a.part1 + (size_t)ntiles * a.G * PS;
-  a.stage = reinterpret_cast<int*>(static_cast<char*>(ws_scratch(h)) + part_bytes);
-  a.zeros = ws_zeros(h);
-  a.cnt = ws_counters(h);
+  a.stage = reinterpret_cast<int*>(static_cast<char*>(ws_scratch(w)) + part_bytes);
+  a.zeros = ws_zeros(w);
+  a.cnt = ws_counters(w);
   const int blocks = a.G * ntiles;
   hipStream_t s = (hipStream_t)stream;
   switch (slices) {

@@ -30,41 +30,32 @@ skinny_status_t hip_fail(hipError_t e, const char* what) {
   return e == hipErrorOutOfMemory ? SKINNY_OOM : SKINNY_HIP_ERROR;
 }
 
-static skinny_status_t grow_locked(skinny_handle_t h, size_t total) {
-  if (total <= h->ws_bytes) return SKINNY_OK;
-  hipError_t e = hipSetDevice(h->device);
-  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-  // in-flight kernels may still use the old workspace
-  if (h->ws) {
-    e = hipDeviceSynchronize();
-    if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
-    e = hipFree(h->ws);
-    if (e != hipSuccess) return hip_fail(e, "hipFree");
-    h->ws = nullptr;
-    h->ws_bytes = 0;
-  }
-  size_t bytes = total < 2 * kCounterBytes ? 2 * kCounterBytes : total;
-  e = hipMalloc(&h->ws, bytes);
-  if (e != hipSuccess) {
-    h->ws = nullptr;
-    return hip_fail(e, "hipMalloc(workspace)");
-  }
-  e = hipMemset(h->ws, 0, kCounterBytes);
-  if (e != hipSuccess) return hip_fail(e, "hipMemset(counters)");
-  e = hipDeviceSynchronize();
-  if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
-  h->ws_bytes = bytes;
-  return SKINNY_OK;
-}
-
-skinny_status_t ensure_workspace(skinny_handle_t h, size_t scratch_bytes) {
-  size_t total = kCounterBytes + scratch_bytes;
-  if (total <= h->ws_bytes) return SKINNY_OK;
+skinny_status_t acquire_workspace(skinny_handle_t h, hipStream_t stream,
+                                  size_t scratch_bytes, Workspace* out) {
+  const size_t total = kCounterBytes + scratch_bytes;
   std::lock_guard<std::mutex> lock(h->mu);
-  // grow geometrically so a sweep of shapes reallocates rarely
-  size_t want = total;
-  if (h->ws_bytes && want < 2 * h->ws_bytes) want = 2 * h->ws_bytes;
-  return grow_locked(h, want);
+  Workspace& w = h->ws[stream];
+  if (total > w.bytes) {
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    // grow geometrically so a sweep of shapes reallocates rarely
+    size_t want = total < 2 * kCounterBytes ? 2 * kCounterBytes : total;
+    if (w.bytes && want < 2 * w.bytes) want = 2 * w.bytes;
+    void* p = nullptr;
+    e = hipMalloc(&p, want);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc(workspace)");
+    e = hipMemset(p, 0, kCounterBytes);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      return hip_fail(e, "workspace counters");
+    }
+    if (w.base) h->retired.push_back(w.base);
+    w.base = static_cast<char*>(p);
+    w.bytes = want;
+  }
+  *out = w;
+  return SKINNY_OK;
 }
 
 // ---- counter RNG (same function as the host generator) -------------------
@@ -144,7 +135,8 @@ skinny_status_t skinny_create(skinny_handle_t* handle) {
     return hip_fail(e, "hipDeviceGetAttribute");
   }
   h->cu_count = cus > 0 ? cus : 256;
-  skinny_status_t s = grow_locked(h, 2 * kCounterBytes);
+  Workspace w;
+  skinny_status_t s = acquire_workspace(h, nullptr, kCounterBytes, &w);
   if (s != SKINNY_OK) {
     delete h;
     return s;
@@ -155,20 +147,25 @@ skinny_status_t skinny_create(skinny_handle_t* handle) {
 
 skinny_status_t skinny_destroy(skinny_handle_t h) {
   if (!h) return SKINNY_OK;
-  if (h->ws) {
-    (void)hipSetDevice(h->device);
-    (void)hipDeviceSynchronize();
-    (void)hipFree(h->ws);
-  }
+  (void)hipSetDevice(h->device);
+  (void)hipDeviceSynchronize();
+  for (auto& kv : h->ws)
+    if (kv.second.base) (void)hipFree(kv.second.base);
+  for (void* p : h->retired) (void)hipFree(p);
   delete h;
   return SKINNY_OK;
 }
 
 skinny_status_t skinny_reserve(skinny_handle_t h, size_t bytes) {
+  return skinny_reserve_stream(h, bytes, nullptr);
+}
+
+skinny_status_t skinny_reserve_stream(skinny_handle_t h, size_t bytes,
+                                      skinny_stream_t stream) {
   h = resolve(h);
-  if (!h) return fail(SKINNY_HIP_ERROR, "no handle");
-  std::lock_guard<std::mutex> lock(h->mu);
-  return grow_locked(h, kCounterBytes + bytes);
+  if (!h) return no_handle("reserve");
+  Workspace w;
+  return acquire_workspace(h, (hipStream_t)stream, bytes, &w);
 }
 
 skinny_handle_t skinny_default_handle(void) {

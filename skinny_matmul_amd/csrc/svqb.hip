@@ -292,7 +292,7 @@ skinny_status_t skinny_dsvqb(skinny_handle_t h, int nblocks,
   if (sweeps < 0) return fail(SKINNY_INVALID_ARG, "svqb: sweeps < 0");
   h = resolve(h);
   if (!h) return no_handle("svqb");
-  s = ensure_workspace(h, max_gram_scratch(h, blocks, nblocks));
+  s = skinny_reserve_stream(h, max_gram_scratch(h, blocks, nblocks), stream);
   if (s != SKINNY_OK) return s;
   SvqbPlanBuf b;
   s = svqb_alloc(blocks, nblocks, b);
@@ -317,7 +317,6 @@ skinny_status_t skinny_dsvqb_graph(skinny_handle_t h_unused, int nblocks,
   if (sweeps < 1) return fail(SKINNY_INVALID_ARG, "svqb graph: sweeps < 1");
   skinny_graph_t g = new skinny_graph_s();
   s = skinny_create(&g->h);
-  if (s == SKINNY_OK) s = skinny_reserve(g->h, max_gram_scratch(g->h, blocks, nblocks));
   if (s == SKINNY_OK) s = svqb_alloc(blocks, nblocks, g->buf);
   if (s != SKINNY_OK) {
     skinny_graph_destroy(g);
@@ -326,6 +325,15 @@ skinny_status_t skinny_dsvqb_graph(skinny_handle_t h_unused, int nblocks,
   hipStream_t st = (hipStream_t)stream;
   hipStream_t cap = nullptr;
   hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+  // the capture stream's workspace is sized now: nothing allocates inside
+  if (e == hipSuccess) {
+    s = skinny_reserve_stream(g->h, max_gram_scratch(g->h, blocks, nblocks), cap);
+    if (s != SKINNY_OK) {
+      (void)hipStreamDestroy(cap);
+      skinny_graph_destroy(g);
+      return s;
+    }
+  }
   if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) {
     if (cap) (void)hipStreamDestroy(cap);

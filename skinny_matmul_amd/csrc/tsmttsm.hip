@@ -325,7 +325,8 @@ skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
   const int ntiles = p.tilesM * p.tilesN;
   if ((size_t)ntiles * (p.ngroups + 1) > (size_t)kMaxCounters)
     return fail(SKINNY_UNSUPPORTED, "tsmttsm: too many partial groups");
-  skinny_status_t st = ensure_workspace(h, p.scratch_bytes);
+  Workspace w;
+  skinny_status_t st = acquire_workspace(h, stream, p.scratch_bytes, &w);
   if (st != SKINNY_OK) return st;
   TsArgs<T> a;
   a.A = A;
@@ -342,9 +343,9 @@ skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
   a.alpha_i = alpha_i;
   a.beta_i = beta_i;
   a.cplx = cplx ? 1 : 0;
-  a.part1 = static_cast<T*>(ws_scratch(h));
+  a.part1 = static_cast<T*>(ws_scratch(w));
   a.part2 = a.part1 + (size_t)ntiles * p.G * p.outM * p.outN;
-  a.cnt = ws_counters(h);
+  a.cnt = ws_counters(w);
   a.G = p.G;
   a.GS = p.GS;
   a.ngroups = p.ngroups;

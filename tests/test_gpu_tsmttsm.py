@@ -171,3 +171,31 @@ def test_many_blockcounts_reproducible(gpu, orc):
         ok, got = _run(gpu, bc, M, N, K, A, M, 1.0, B, N, 0.0, np.zeros(M * N), M)
         assert ok
         assert rel_fro(got - lo, hi) <= 8 * K * EPS64
+
+
+def test_concurrent_streams_share_a_handle(gpu, orc):
+    """Calls on two streams through one handle overlap without sharing
+    split-K partials or arrival counters (each stream has its own
+    workspace): both results stay bit-exact on integer data."""
+    import torch
+    M = N = 16
+    K = 1 << 22
+    note("tsmttsm two streams, one handle")
+    outs = []
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    data = []
+    for i in range(2):
+        A = torch.empty(K * M, dtype=torch.float64, device="cuda")
+        B = torch.empty(K * N, dtype=torch.float64, device="cuda")
+        gpu.fill_int(A, A.numel(), 40 + i, 0)
+        gpu.fill_int(B, B.numel(), 40 + i, 1)
+        data.append((A, B, torch.zeros(M * N, dtype=torch.float64, device="cuda")))
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for (A, B, C), s in zip(data, streams):
+            with torch.cuda.stream(s):
+                assert gpu.dtsmttsm(512, M, N, K, A, M, 1.0, B, N, 0.0, C, M)
+        torch.cuda.synchronize()
+        for A, B, C in data:
+            ref = A.view(K, M).t().mm(B.view(K, N))
+            assert torch.equal(C.view(N, M).t(), ref), rep
