@@ -228,7 +228,7 @@ VersionList<typename Types::dtype> tsmttsm_versions() {
   VersionList<dt> v;
   v.push_back({tsmttsm<dt, typename Types::dItype>, "SKINNY"});
   if constexpr (std::is_same<Types, TypeDR>::value) {
-    v.push_back({tsmttsm_ozaki<4>, "OZAKI4"});
+    v.push_back({tsmttsm_ozaki<3>, "OZAKI3"});
   }
   return v;
 }

@@ -763,31 +763,47 @@ void orc_gram_error(int64_t M, int64_t K, const double* X, int64_t ldx,
 /* the device path.  Algorithm: Ozaki, Ogita, Oishi, Rump, "Error-free       */
 /* transformations of matrix multiplication by using fast routines of matrix */
 /* multiplication and its applications", Numer. Algorithms 59 (2012), with   */
-/* truncation-based 7-bit limbs and a running per-column scale.              */
+/* 7-bit limbs of the rounded integer q = rint(|x| 2^(7S - E)), levels up to  */
+/* s + t = S + 2 and a running per-column scale.                             */
 /* ------------------------------------------------------------------------ */
 #define OZ_TILE 128
 #define OZ_ROWS 32
 #define OZ_FLUSH 16384
 #define OZ_EMIN (-4000)
 
-static void oz_flush_tile(int S, int64_t* acc, double* cacc, const int* ea,
+/* L = S + 1 level accumulators per entry, level g = s + t at index g - 2 */
+static void oz_flush_tile(int L, int64_t* acc, double* cacc, const int* ea,
                           const int* eb) {
   for (int n = 0; n < OZ_TILE; n++)
     for (int m = 0; m < OZ_TILE; m++) {
-      int64_t* a = acc + ((int64_t)n * OZ_TILE + m) * S;
+      int64_t* a = acc + ((int64_t)n * OZ_TILE + m) * L;
       if (ea[m] != OZ_EMIN && eb[n] != OZ_EMIN) {
-        double v = ldexp((double)a[S - 1], -7 * (S + 1));
-        for (int g = S; g >= 2; g--) v = v + ldexp((double)a[g - 2], -7 * g);
+        double v = ldexp((double)a[L - 1], -7 * (L + 1));
+        for (int g = L; g >= 2; g--) v = v + ldexp((double)a[g - 2], -7 * g);
         cacc[n * OZ_TILE + m] = cacc[n * OZ_TILE + m] + ldexp(v, ea[m] + eb[n]);
       }
-      for (int g = 0; g < S; g++) a[g] = 0;
+      for (int g = 0; g < L; g++) a[g] = 0;
     }
+}
+
+/* The limb integer of |x| against 2^E: q = min(rint(t), 2^(7 S4) - 1) with
+ * t = |x| 2^(7 S4 - E), S4 = min(S, 4) (truncation for S = 5, whose fifth
+ * limb is the rounded remainder); the device computes the same. */
+static uint64_t oz_q(double ax, int E, int S, double* t_out) {
+  int S4 = S < 4 ? S : 4;
+  double t = ldexp(ax, 7 * S4 - E);
+  double cap = ldexp(1.0, 7 * S4) - 1.0;
+  if (t_out) *t_out = t;
+  double q = S <= 4 ? fmin(nearbyint(t), cap) : fmin(trunc(t), cap);
+  return (uint64_t)q;
 }
 
 void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
                        int64_t lda, double alpha, const double* B, int64_t ldb,
                        double beta, double* C, int64_t ldc, int S) {
-  int64_t* acc = (int64_t*)malloc(sizeof(int64_t) * OZ_TILE * OZ_TILE * S);
+  const int L = S + 1;
+  const int S4 = S < 4 ? S : 4;
+  int64_t* acc = (int64_t*)malloc(sizeof(int64_t) * OZ_TILE * OZ_TILE * L);
   double* cacc = (double*)malloc(sizeof(double) * OZ_TILE * OZ_TILE);
   signed char* da = (signed char*)malloc((size_t)S * OZ_ROWS * OZ_TILE);
   signed char* db = (signed char*)malloc((size_t)S * OZ_ROWS * OZ_TILE);
@@ -795,7 +811,7 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
     for (int64_t n0 = 0; n0 < N; n0 += OZ_TILE) {
       int erun[2][OZ_TILE], ecur[2][OZ_TILE];
       for (int c = 0; c < OZ_TILE; c++) erun[0][c] = erun[1][c] = OZ_EMIN;
-      memset(acc, 0, sizeof(int64_t) * OZ_TILE * OZ_TILE * S);
+      memset(acc, 0, sizeof(int64_t) * OZ_TILE * OZ_TILE * L);
       memset(cacc, 0, sizeof(double) * OZ_TILE * OZ_TILE);
       int since = 0;
       for (int64_t r0 = 0; r0 < K; r0 += OZ_ROWS) {
@@ -815,21 +831,20 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
             if (mx > 0.0) frexp(mx, &e);
             ecur[mat][c] = e;
             if (e > erun[mat][c]) flush = 1; /* growth */
-            /* shrink: the chunk's largest limb integer q = floor(mx 2^(7 S4 -
-             * E)) is nonzero but 9 or more bits below the scale, so the
-             * scale restarts from this chunk (an outlier coarsens only its
-             * own chunk) */
-            if (erun[mat][c] != OZ_EMIN && mx > 0.0) {
-              int S4 = S < 4 ? S : 4;
-              double t = ldexp(mx, 7 * S4 - erun[mat][c]);
-              if (t >= 1.0 && t < ldexp(1.0, 7 * S4 - 9)) flush = 1;
+            /* shrink: the chunk's largest limb integer q (oz_q of its
+             * largest |x|) is nonzero but 9 or more bits below the scale, so
+             * the scale restarts from this chunk (an outlier coarsens only
+             * its own chunk) */
+            if (e <= erun[mat][c] && erun[mat][c] != OZ_EMIN && mx > 0.0) {
+              uint64_t q = oz_q(mx, erun[mat][c], S, NULL);
+              if (q != 0 && q < ((uint64_t)1 << (7 * S4 - 9))) flush = 1;
             }
           }
         if (flush) {
           /* the accumulators are empty after a flush, so every column's
            * scale restarts from this chunk: an outlier only coarsens the
            * window it falls in */
-          oz_flush_tile(S, acc, cacc, erun[0], erun[1]);
+          oz_flush_tile(L, acc, cacc, erun[0], erun[1]);
           since = 0;
           for (int mat = 0; mat < 2; mat++)
             for (int c = 0; c < OZ_TILE; c++) erun[mat][c] = ecur[mat][c];
@@ -844,18 +859,22 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
             for (int i = 0; i < OZ_ROWS; i++) {
               int64_t r = r0 + i;
               double x = (valid && r < K) ? (mat == 0 ? A[r * lda + gc] : B[r * ldb + gc]) : 0.0;
-              double y = (E == OZ_EMIN) ? 0.0 : ldexp(x, -E);
-              for (int s2 = 0; s2 < S; s2++) {
-                y = y * 128.0;
-                double dd = trunc(y);
-                y = y - dd;
-                d[((size_t)s2 * OZ_TILE + c) * OZ_ROWS + i] = (signed char)(int)dd;
+              int digit[5] = {0, 0, 0, 0, 0};
+              if (E != OZ_EMIN && x != 0.0) {
+                double t;
+                uint64_t q = oz_q(fabs(x), E, S, &t);
+                for (int s2 = 0; s2 < S4; s2++)
+                  digit[s2] = (int)((q >> (7 * (S4 - 1 - s2))) & 127);
+                if (S > 4) digit[4] = (int)fmin(nearbyint((t - (double)q) * 128.0), 127.0);
               }
+              for (int s2 = 0; s2 < S; s2++)
+                d[((size_t)s2 * OZ_TILE + c) * OZ_ROWS + i] =
+                    (signed char)(x < 0.0 ? -digit[s2] : digit[s2]);
             }
           }
         }
         /* exact integer products per level g = s + t (1-based limbs) */
-        for (int g = 2; g <= S + 1; g++)
+        for (int g = 2; g <= S + 2; g++)
           for (int s1 = 1; s1 <= S; s1++) {
             int t1 = g - s1;
             if (t1 < 1 || t1 > S) continue;
@@ -867,12 +886,12 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
                 const signed char* x = pa + (size_t)m * OZ_ROWS;
                 const signed char* y = pb + (size_t)n * OZ_ROWS;
                 for (int i = 0; i < OZ_ROWS; i++) sum += (int64_t)x[i] * y[i];
-                acc[((int64_t)n * OZ_TILE + m) * S + (g - 2)] += sum;
+                acc[((int64_t)n * OZ_TILE + m) * L + (g - 2)] += sum;
               }
           }
         since += OZ_ROWS;
       }
-      oz_flush_tile(S, acc, cacc, erun[0], erun[1]);
+      oz_flush_tile(L, acc, cacc, erun[0], erun[1]);
       for (int64_t n = 0; n < OZ_TILE && n0 + n < N; n++)
         for (int64_t m = 0; m < OZ_TILE && m0 + m < M; m++) {
           double sres = cacc[n * OZ_TILE + m];

@@ -255,7 +255,7 @@ def stsmm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
 
 
 def dtsmttsm_ozaki(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
-                   slices=4, handle=None, stream=None) -> bool:
+                   slices=3, handle=None, stream=None) -> bool:
     """FP64 TSMTTSM through INT8 limbs on the matrix cores (Ozaki split)."""
     return _check(lib().skinny_dtsmttsm_ozaki(_h(handle), blockCount, M, N, K, _addr(A), lda,
                                               float(alpha), _addr(B), ldb, float(beta),

@@ -26,8 +26,7 @@ struct TuneEntry {
   int G;     // workgroups along K
 };
 // Device workspace of one stream: [counter region (zeroed at allocation,
-// reset to zero by the kernels that use it; its last kZeroBytes are never
-// written)] [scratch region].
+// reset to zero by the kernels that use it)] [scratch region].
 struct Workspace {
   char* base = nullptr;
   size_t bytes = 0;
@@ -51,12 +50,9 @@ struct skinny_handle_s {
 
 namespace skinny {
 
-// bytes at the start of the workspace reserved for arrival counters; its
-// last kZeroBytes are never written and read as zeros (a zero row for loads
-// past the end of a matrix)
+// bytes at the start of the workspace reserved for arrival counters
 constexpr size_t kCounterBytes = 64 * 1024;
-constexpr size_t kZeroBytes = 8 * 1024;
-constexpr int kMaxCounters = (kCounterBytes - kZeroBytes) / sizeof(unsigned int);
+constexpr int kMaxCounters = kCounterBytes / sizeof(unsigned int);
 
 void set_error(const std::string& msg);
 skinny_status_t fail(skinny_status_t s, const std::string& msg);
@@ -74,9 +70,6 @@ skinny_status_t acquire_workspace(skinny_handle_t h, hipStream_t stream,
 
 inline unsigned int* ws_counters(const Workspace& w) {
   return reinterpret_cast<unsigned int*>(w.base);
-}
-inline const double* ws_zeros(const Workspace& w) {
-  return reinterpret_cast<const double*>(w.base + kCounterBytes - kZeroBytes);
 }
 inline void* ws_scratch(const Workspace& w) { return w.base + kCounterBytes; }
 

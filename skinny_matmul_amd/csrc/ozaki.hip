@@ -5,19 +5,25 @@
 // in chunks of 32 (one MFMA k-depth):
 //  * Each column c has a running scale exponent E_c: every |x| since the last
 //    flush is below 2^E_c (after a flush the scale restarts from the next
-//    chunk's largest |x|).  An element x is split into S signed 7-bit limbs,
-//      y = x 2^-E;  for s = 1..S: y = 128 y; d_s = trunc(y); y = y - d_s,
-//    so x = 2^E (sum_s d_s 2^-7s + r 2^-7S), |d_s| <= 127, |r| < 1; every
-//    step is exact in FP64 (Ozaki et al. 2012, error-free splitting).
-//  * For each limb-sum level g = s + t (2 .. S + 1; pairs with s + t > S + 1
-//    are dropped, as in the truncated Ozaki scheme) the products
-//    sum_k d_s(k, m) d'_t(k, n) accumulate exactly in INT32 through
-//    v_mfma_i32_32x32x32_i8, one accumulator per level.
+//    chunk's largest |x|).  An element x is split into S signed 7-bit limbs:
+//    with t = |x| 2^(7S - E) < 2^(7S) and q = round-to-nearest(t) (capped at
+//    2^(7S) - 1), limb s is sign(x) times bits [7(S - s), 7(S - s) + 7) of q,
+//    so x = 2^E (sum_s d_s 2^-7s + r 2^-7S), |d_s| <= 127, |r| <= 1/2 (the
+//    cap aside); every step is exact in FP64 (Ozaki et al. 2012, error-free
+//    splitting; rounding rather than truncating the last digit removes the
+//    representation's bias toward zero).  For S = 5 the first four limbs
+//    come from trunc(t') with t' = |x| 2^(28 - E) and the fifth is the
+//    rounded remainder, capped at 127.
+//  * For each limb-sum level g = s + t (2 .. S + 2; pairs with s + t > S + 2
+//    are dropped, as in the truncated Ozaki scheme, one level deeper than
+//    the limb count) the products sum_k d_s(k, m) d'_t(k, n) accumulate
+//    exactly in INT32 through v_mfma_i32_32x32x32_i8, one accumulator per
+//    level.
 //  * The INT32 accumulators are converted to FP64 ("flushed") only when a
 //    column's magnitude grows past its scale, when a chunk's largest limb
 //    integer falls 9 or more bits below it (the chunk after an outlier), or
 //    before INT32 could overflow (every 16384 rows); the flush forms
-//      V = sum_{g = S + 1 .. 2} acc_g 2^-7g   (in this order, FP64),
+//      V = sum_{g = S + 2 .. 2} acc_g 2^-7g   (in this order, FP64),
 //      C_acc(m, n) += V 2^(E_m + E_n).
 //    For data of steady magnitude the conversion cost is amortised over
 //    thousands of rows, so the INT8 units and HBM set the pace.
@@ -47,8 +53,9 @@ typedef unsigned v4u __attribute__((ext_vector_type(4)));
 // Raw buffer resources: the per-access constant parts of an address go into
 // the scalar offset, so unrolled loops over rows and accumulator entries need
 // one vector address register instead of one 64-bit pointer per access.
-// Word 3 = 0x00020000 is the gfx9 raw-buffer format (32-bit data, no swizzle);
-// loads past num_records return 0.
+// Word 3 = 0x00020000 is the gfx9 raw-buffer format (32-bit data, no swizzle).
+// Loads whose offset (vector offset plus scalar offset) reaches num_records
+// return 0 without touching memory (checked on gfx950 for the scalar offset).
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t oz_rsrc(const void* p,
                                                           int64_t bytes) {
   const int n = bytes > 0x7fffffff ? 0x7fffffff : (int)bytes;
@@ -90,8 +97,7 @@ struct OzArgs {
   double alpha, beta;
   double* part1;
   double* part2;
-  int* stage;  // per wave: S * 4096 INT32 accumulator staging
-  const double* zeros;  // >= 1024 zero doubles
+  int* stage;  // per wave: (S + 1) * 4096 INT32 accumulator staging
   unsigned* cnt;
   int G, GS, ngroups, tilesM, tilesN;
 };
@@ -100,10 +106,10 @@ struct OzArgs {
 // registers: placed at the top of the chunk loop it keeps the register
 // allocator from carrying them around the loop as VGPR copies (which the
 // flush path otherwise provokes, spilling them every iteration).
-template <int S>
-__device__ __forceinline__ void oz_pin(v16i (&acc)[S][4]) {
+template <int L>
+__device__ __forceinline__ void oz_pin(v16i (&acc)[L][4]) {
 #pragma unroll
-  for (int g = 0; g < S; g++)
+  for (int g = 0; g < L; g++)
 #pragma unroll
     for (int tt = 0; tt < 4; tt++) asm volatile("" : "+a"(acc[g][tt]));
 }
@@ -115,14 +121,14 @@ __device__ __forceinline__ void oz_pin(v16i (&acc)[S][4]) {
 // staging area and are converted from there one entry at a time, so the
 // conversion holds no copy of them.  `part` is the resource of the region's
 // first entry, `lofs` the lane's byte offset in it.
-template <int S>
-__device__ __forceinline__ void oz_flush(v16i (&acc)[S][4],
+template <int L>
+__device__ __forceinline__ void oz_flush(v16i (&acc)[L][4],
                                          __amdgpu_buffer_rsrc_t stage,
                                          __amdgpu_buffer_rsrc_t part, int lofs,
                                          const int* ea_w, const int* eb_w,
                                          int lane) {
 #pragma unroll
-  for (int g = 0; g < S; g++)
+  for (int g = 0; g < L; g++)
 #pragma unroll
     for (int tt = 0; tt < 4; tt++) {
 #pragma unroll
@@ -147,9 +153,9 @@ __device__ __forceinline__ void oz_flush(v16i (&acc)[S][4],
         return (int)__builtin_amdgcn_raw_buffer_load_b32(
             stage, lo, 1024 * ((g * 4 + tt) * 4 + (r >> 2)), 0);
       };
-      double v = ldexp((double)level(S - 1), -7 * (S + 1));
+      double v = ldexp((double)level(L - 1), -7 * (L + 1));
 #pragma unroll
-      for (int g = S; g >= 2; g--) v = v + ldexp((double)level(g - 2), -7 * g);
+      for (int g = L; g >= 2; g--) v = v + ldexp((double)level(g - 2), -7 * g);
       const int so = 8 * (32 * (tt & 1) * kOzTile + 32 * (tt >> 1) +
                           8 * (r >> 2) + (r & 3));
       oz_st(oz_ld(part, lofs, so) + ldexp(v, ea + eb), part, lofs, so);
@@ -157,47 +163,38 @@ __device__ __forceinline__ void oz_flush(v16i (&acc)[S][4],
   }
 }
 
-// Signed 7-bit limbs of x against scale 2^E: with q = floor(|x| 2^(7S - E))
-// (|x| < 2^E, so q < 2^(7S)), limb s is sign(x) * bits [7(S - s), 7(S - s) + 7)
-// of q: the same digits as the sequence y = 128 y; d = trunc(y); y -= d.
-template <int S>
-__device__ __forceinline__ void oz_limbs(double x, int E, int (&d)[S]) {
-  const double ax = fabs(x);
-  const bool neg = x < 0.0;
-  constexpr int S4 = S < 4 ? S : 4;
-  const double t = ldexp(ax, 7 * S4 - E);
-  const unsigned q = (unsigned)t;  // truncation; t < 2^28
-#pragma unroll
-  for (int s = 0; s < S4; s++) {
-    const int v = (int)((q >> (7 * (S4 - 1 - s))) & 127u);
-    d[s] = neg ? -v : v;
-  }
-  if (S > 4) {
-    const double rem = (t - (double)q) * 128.0;  // exact, in [0, 128)
-    const int v = (int)rem;
-    d[S - 1] = neg ? -v : v;
-  }
-}
-
 // Limb words of four consecutive rows of a column: w[s] holds limb s + 1 of
-// row j in byte j (two's complement), the digits of oz_limbs.  The bytes are
-// built unsigned and negated together: for a byte u <= 127,
-// ((0x80 - u) ^ 0x80) == -u mod 256 with no borrow into the next byte; the
-// sign bytes come from v_perm's sign-replicating selectors (9: bit 31 of
-// the second operand, 11: bit 31 of the first).
+// row j in byte j (two's complement).  The bytes are built unsigned and
+// negated together: for a byte u <= 127, ((0x80 - u) ^ 0x80) == -u mod 256
+// with no borrow into the next byte; the sign bytes come from v_perm's
+// sign-replicating selectors (9: bit 31 of the second operand, 11: bit 31
+// of the first).
 //
-// Returns the largest q: q >= 2^(7 min(S, 4)) iff some |x| >= 2^E (the
-// conversion saturates), i.e. a row outgrew the scale.
+// Returns the largest t (t >= 2^(7 min(S, 4)) iff some |x| >= 2^E: a row
+// outgrew the scale) and the largest limb integer q.
+struct OzSplit {
+  double tmax;
+  unsigned qmax;
+};
+
 template <int S>
-__device__ __forceinline__ unsigned oz_pack4(const double (&xv)[4], int E,
-                                             int (&w)[S]) {
+__device__ __forceinline__ OzSplit oz_pack4(const double (&xv)[4], int E,
+                                            int (&w)[S]) {
   constexpr int S4 = S < 4 ? S : 4;
+  constexpr double qcap = (double)((1u << (7 * S4)) - 1u);
   unsigned q[4], r5[4];
+  double tmax = 0.0;
 #pragma unroll
   for (int j = 0; j < 4; j++) {
     const double t = ldexp(fabs(xv[j]), 7 * S4 - E);
-    q[j] = (unsigned)t;  // truncation; t < 2^28
-    r5[j] = S > 4 ? (unsigned)((t - (double)q[j]) * 128.0) : 0u;
+    tmax = fmax(tmax, t);
+    if (S <= 4) {
+      q[j] = (unsigned)fmin(rint(t), qcap);
+      r5[j] = 0u;
+    } else {
+      q[j] = (unsigned)fmin(t, qcap);  // truncation
+      r5[j] = (unsigned)fmin(rint((t - (double)q[j]) * 128.0), 127.0);
+    }
   }
   const unsigned m =
       __builtin_amdgcn_perm((unsigned)__double2hiint(xv[1]), (unsigned)__double2hiint(xv[0]), 0x0C0C0B09u) |
@@ -213,7 +210,7 @@ __device__ __forceinline__ unsigned oz_pack4(const double (&xv)[4], int E,
     const unsigned nb = (0x80808080u - u) ^ 0x80808080u;
     w[s] = (int)((nb & m) | (u & ~m));
   }
-  return max(max(q[0], q[1]), max(q[2], q[3]));
+  return OzSplit{tmax, max(max(q[0], q[1]), max(q[2], q[3]))};
 }
 
 // Calls f(integral_constant<int, I>) for I = 0 .. N - 1 in order: unrolled
@@ -224,19 +221,27 @@ __device__ __forceinline__ void oz_static_for(F&& f, std::integer_sequence<int, 
 }
 
 // The k-th INT8 product of a chunk (tile tt, level g, limb s of A, limb
-// g - s of B), tile-major; 2 S (S + 1) products per chunk and wave.
+// g - s of B), tile-major over the pairs with 2 <= s + t <= S + 2.
 struct OzMf {
   int tt, g, s;
 };
 template <int S>
+constexpr int oz_levels() {
+  return S + 1;
+}
+template <int S>
 constexpr int oz_nmfma() {
-  return 2 * S * (S + 1);
+  int n = 0;
+  for (int g = 2; g <= S + 2; g++)
+    for (int s = 1; s <= S; s++)
+      if (g - s >= 1 && g - s <= S) n++;
+  return 4 * n;
 }
 template <int S>
 constexpr OzMf oz_mfma_of(int k) {
   int n = 0;
   for (int tt = 0; tt < 4; tt++)
-    for (int g = 2; g <= S + 1; g++)
+    for (int g = 2; g <= S + 2; g++)
       for (int s = 1; s <= S; s++) {
         const int t = g - s;
         if (t < 1 || t > S) continue;
@@ -277,9 +282,10 @@ __global__ void __launch_bounds__(kOzThreads)
   const int ld8 = (int)(8 * ld);
 
   erun[mat][col] = kOzEmin;
-  v16i acc[S][4];
+  constexpr int L = oz_levels<S>();  // accumulator levels g = 2 .. S + 2
+  v16i acc[L][4];
 #pragma unroll
-  for (int g = 0; g < S; g++)
+  for (int g = 0; g < L; g++)
 #pragma unroll
     for (int tt = 0; tt < 4; tt++)
 #pragma unroll
@@ -298,27 +304,25 @@ __global__ void __launch_bounds__(kOzThreads)
       oz_st(0.0, part, lofs,
             8 * (32 * (tt & 1) * kOzTile + 32 * (tt >> 1) + 8 * (r >> 2) + (r & 3)));
   const __amdgpu_buffer_rsrc_t stage = oz_rsrc(
-      a.stage + ((int64_t)blockIdx.x * (kOzThreads / 64) + wave) * (S * 4096),
-      (int64_t)S * 4096 * 4);
+      a.stage + ((int64_t)blockIdx.x * (kOzThreads / 64) + wave) * (L * 4096),
+      (int64_t)L * 4096 * 4);
   const int* ea_w = &erun[0][64 * wm];
   const int* eb_w = &erun[1][64 * wn];
   __syncthreads();
 
   const int64_t nch = (a.K + kOzRows - 1) / kOzRows;
   double x[kOzRows], xn[kOzRows];  // this chunk and the next one in flight
-  // rows [32 c, 32 c + 32) of this thread's column; rows at or past K read
-  // the zero area instead (the choice of resource is scalar), so every chunk
-  // issues the same 32 loads back to back.  Padding columns read column 0
-  // and are zeroed through their scale.
-  const __amdgpu_buffer_rsrc_t rz = oz_rsrc(a.zeros, 8 * 1024);
+  // rows [32 c, 32 c + 32) of this thread's column: the chunk's resource
+  // ends after its last row below K, so rows at or past K read 0 and every
+  // chunk issues the same 32 loads back to back.  Padding columns read
+  // column 0 and are zeroed through their scale.
   auto load_chunk = [&](int64_t c, double (&dst)[kOzRows]) {
     const int64_t r0 = c * kOzRows;
     const int rows = (int)min<int64_t>(kOzRows, a.K - r0);
     const __amdgpu_buffer_rsrc_t rs =
         oz_rsrc(mbase + r0 * ld, 8 * ((int64_t)(rows - 1) * ld + width));
 #pragma unroll
-    for (int i = 0; i < kOzRows; i++)
-      dst[i] = i < rows ? oz_ld(rs, voff, i * ld8) : oz_ld(rz, voff, 0);
+    for (int i = 0; i < kOzRows; i++) dst[i] = oz_ld(rs, voff, i * ld8);
   };
   // frexp exponent of the chunk's largest |x| (max |x| < 2^e), kOzEmin for
   // an all-zero chunk or a padding column
@@ -368,7 +372,7 @@ __global__ void __launch_bounds__(kOzThreads)
       const int mcol = 64 * wm + 32 * (tt >> 1) + (lane & 31);
       const int ncol = 64 * wn + 32 * (tt & 1) + (lane & 31);
 #pragma unroll
-      for (int g = 2; g <= S + 1; g++) {
+      for (int g = 2; g <= S + 2; g++) {
 #pragma unroll
         for (int s1 = 1; s1 <= S; s1++) {
           const int t1 = g - s1;
@@ -382,7 +386,7 @@ __global__ void __launch_bounds__(kOzThreads)
   };
   // Steady-state step: split the chunk in x into LDS buffer bs while the
   // chunk staged in buffer bm is multiplied.  The splits of 8 groups of 4
-  // rows and the 2 S (S + 1) products are issued interleaved, in fixed units
+  // rows and the chunk's products are issued interleaved, in fixed units
   // separated by scheduling fences, so the VALU work runs while the matrix
   // core is busy.  Returns nonzero if the column's scale must restart: a
   // value outgrew it, or the chunk's largest limb integer is nonzero but 9
@@ -398,6 +402,7 @@ __global__ void __launch_bounds__(kOzThreads)
       }
     const int Es = split_scale();
     unsigned qmax = 0;
+    double tmax = 0.0;
     int words[S][4];
     oz_static_for(
         [&](auto U_) {
@@ -405,7 +410,9 @@ __global__ void __launch_bounds__(kOzThreads)
           {
             const double xv[4] = {x[4 * U], x[4 * U + 1], x[4 * U + 2], x[4 * U + 3]};
             int w[S];
-            qmax = max(qmax, oz_pack4<S>(xv, Es, w));
+            const OzSplit sp = oz_pack4<S>(xv, Es, w);
+            qmax = max(qmax, sp.qmax);
+            tmax = fmax(tmax, sp.tmax);
 #pragma unroll
             for (int s2 = 0; s2 < S; s2++) words[s2][U % 4] = w[s2];
           }
@@ -430,7 +437,8 @@ __global__ void __launch_bounds__(kOzThreads)
         },
         std::make_integer_sequence<int, kOzRows / 4>{});
     constexpr int S4 = S < 4 ? S : 4;
-    return (qmax >> (7 * S4)) != 0u || (qmax != 0u && qmax < (1u << (7 * S4 - 9)));
+    return tmax >= (double)(1u << (7 * S4)) ||
+           (qmax != 0u && qmax < (1u << (7 * S4 - 9)));
   };
 
   // Software pipeline: while chunk c is split (VALU) into one LDS buffer,
@@ -458,7 +466,7 @@ __global__ void __launch_bounds__(kOzThreads)
     __syncthreads();
     bool staged = true;  // buffer pb holds a chunk not yet multiplied
     for (ch += a.G; ch < nch; ch += a.G) {
-      oz_pin<S>(acc);
+      oz_pin<L>(acc);
       if (since + kOzRows > kOzFlushRows) break;  // uniform
       load_chunk(min(ch + a.G, nch - 1), xn);
       const bool rescale = split_multiply(pb, pb ^ 1);
@@ -473,7 +481,7 @@ __global__ void __launch_bounds__(kOzThreads)
     }
     // drain: multiply the staged chunk, convert the accumulators
     if (staged) multiply(pb);
-    oz_flush<S>(acc, stage, part, lofs, ea_w, eb_w, lane);
+    oz_flush<L>(acc, stage, part, lofs, ea_w, eb_w, lane);
     __syncthreads();
   }
 
@@ -603,7 +611,7 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
   size_t bytes = (size_t)ntiles * a.G * PS * sizeof(double);
   if (a.ngroups > 1) bytes += (size_t)ntiles * a.ngroups * PS * sizeof(double);
   const size_t part_bytes = bytes;
-  bytes += (size_t)ntiles * a.G * (kOzThreads / 64) * slices * 4096 * sizeof(int);
+  bytes += (size_t)ntiles * a.G * (kOzThreads / 64) * (slices + 1) * 4096 * sizeof(int);
   Workspace w;
   skinny_status_t st = acquire_workspace(h, (hipStream_t)stream, bytes, &w);
   if (st != SKINNY_OK) return st;
@@ -621,7 +629,6 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
   a.part1 = static_cast<double*>(ws_scratch(w));
   a.part2 = a.part1 + (size_t)ntiles * a.G * PS;
   a.stage = reinterpret_cast<int*>(static_cast<char*>(ws_scratch(w)) + part_bytes);
-  a.zeros = ws_zeros(w);
   a.cnt = ws_counters(w);
   const int blocks = a.G * ntiles;
   hipStream_t s = (hipStream_t)stream;

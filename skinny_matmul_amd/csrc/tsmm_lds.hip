@@ -191,6 +191,97 @@ static hipError_t launch_lds(const TlArgs<T>& a, int S, int blocks,
 
 // M, N, lda, ldc: real-view values; ldb in B elements (complex units when
 // cplx).  Requires M, N <= 128.
+// Complex TSMM for narrow blocks (at most 16 complex columns of A): one
+// thread owns whole rows, reads its row of A (M complex values) into
+// registers, forms the N complex dot products with B and writes the row of C
+// (C may alias A: every A value of a row is loaded before it is written).
+// B's address is the same in every lane, so its values come through scalar
+// loads.  Per entry: re = sum a.x b.x - a.y b.y, im = sum a.x b.y + a.y b.x
+// (fused multiply-adds in m order), then C = alpha s + beta C in complex
+// arithmetic (cu_complex.h axpby order: beta C + alpha s).
+template <typename T>
+struct Cx {
+  T x, y;
+};
+
+__device__ __forceinline__ double cfma(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+__device__ __forceinline__ float cfma(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+
+template <typename T, int MC, bool BETA0>
+__global__ void __launch_bounds__(256) ztsmm_rows_kernel(TlArgs<T> a) {
+  constexpr int R = MC <= 4 ? 4 : (MC <= 8 ? 2 : 1);  // rows in flight
+  const int M = a.M / 2, N = a.N / 2;                   // complex widths
+  const int64_t K = a.K;
+  const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const Cx<T>* __restrict__ Bc = reinterpret_cast<const Cx<T>*>(a.B);
+  const int64_t lda = a.lda / 2, ldc = a.ldc / 2;  // complex units
+  const Cx<T>* Ac = reinterpret_cast<const Cx<T>*>(a.A);
+  Cx<T>* Cc = reinterpret_cast<Cx<T>*>(a.C);
+
+  for (int64_t base = tid; base < K; base += nthreads * R) {
+    Cx<T> av[R][MC];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int64_t row = base + (int64_t)r * nthreads;
+      const bool ok = row < K;
+#pragma unroll
+      for (int m = 0; m < MC; m++)
+        av[r][m] = (m < M && ok) ? Ac[row * lda + m] : Cx<T>{T(0), T(0)};
+    }
+    for (int n = 0; n < N; n++) {
+      const Cx<T>* bn = Bc + (int64_t)n * a.ldb;
+      T sx[R], sy[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) sx[r] = sy[r] = T(0);
+#pragma unroll
+      for (int m = 0; m < MC; m++) {
+        if (m < M) {
+          const Cx<T> b = bn[m];
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            sx[r] = cfma(av[r][m].x, b.x, sx[r]);
+            sx[r] = cfma(-av[r][m].y, b.y, sx[r]);
+            sy[r] = cfma(av[r][m].x, b.y, sy[r]);
+            sy[r] = cfma(av[r][m].y, b.x, sy[r]);
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int64_t row = base + (int64_t)r * nthreads;
+        if (row < K) {
+#pragma clang fp contract(off)
+          Cx<T>* cr = Cc + row * ldc + n;
+          Cx<T> o;
+          o.x = a.alpha * sx[r] - a.alpha_i * sy[r];
+          o.y = a.alpha * sy[r] + a.alpha_i * sx[r];
+          if (!BETA0) {
+            const Cx<T> c = *cr;
+            o.x = (a.beta * c.x - a.beta_i * c.y) + o.x;
+            o.y = (a.beta * c.y + a.beta_i * c.x) + o.y;
+          }
+          *cr = o;
+        }
+      }
+    }
+  }
+}
+
+template <typename T, int MC>
+static hipError_t launch_rows(const TlArgs<T>& a, int blocks, bool beta0,
+                              hipStream_t st) {
+  if (beta0)
+    hipLaunchKernelGGL((ztsmm_rows_kernel<T, MC, true>), dim3(blocks), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL((ztsmm_rows_kernel<T, MC, false>), dim3(blocks), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
 template <typename T>
 skinny_status_t launch_tsmm_lds(skinny_handle_t h, size_t blockCount,
                                 int64_t M, int64_t N, int64_t K, const T* A,
@@ -230,6 +321,16 @@ skinny_status_t launch_tsmm_lds(skinny_handle_t h, size_t blockCount,
   int blocks = blockCount > 0 ? (int)blockCount
                               : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T));
   const bool beta0 = cplx ? (beta == T(0) && beta_i == T(0)) : (beta == T(0));
+  if (cplx && M <= 32) {  // at most 16 complex columns: whole rows per thread
+    const int Mc = (int)(M / 2);
+    hipError_t e = Mc <= 1 ? launch_rows<T, 1>(a, blocks, beta0, stream)
+                 : Mc <= 2 ? launch_rows<T, 2>(a, blocks, beta0, stream)
+                 : Mc <= 4 ? launch_rows<T, 4>(a, blocks, beta0, stream)
+                 : Mc <= 8 ? launch_rows<T, 8>(a, blocks, beta0, stream)
+                           : launch_rows<T, 16>(a, blocks, beta0, stream);
+    if (e != hipSuccess) return hip_fail(e, "tsmm (complex rows) launch");
+    return SKINNY_OK;
+  }
   hipError_t e = cplx ? launch_lds<T, true>(a, S, blocks, lds, beta0, stream)
                       : launch_lds<T, false>(a, S, blocks, lds, beta0, stream);
   if (e != hipSuccess) return hip_fail(e, "tsmm (LDS) launch");

@@ -6,7 +6,7 @@
 // W (K x M, row-major) holds 1 + pert * U[0,1) with two large entries
 // (+-1e5 at flat indices 234 and 235), so its columns become nearly
 // collinear as pert shrinks.  For each Gram variant (FP32, FP64,
-// double-double, Ozaki with 4 limbs) one SVQB step runs: S = W^T W;
+// double-double, Ozaki with 3 limbs) one SVQB step runs: S = W^T W;
 // D = diag(S)^-1/2; the eigen-decomposition D S D = U L U^T (cyclic Jacobi on
 // the host, M is small); T = D U L^-1/2; Q = W T (TSMM).  The table reports
 // max |Q^T Q - I| with Q^T Q formed in double-double, averaged over --reps
@@ -111,8 +111,8 @@ int main(int argc, char** argv) {
   skinny::set_abort_on_error(true);
 
   // Gram of W in the given precision, to the host (column-major)
-  enum { FP32, FP64, DD, OZ4, DEVICE, NV };
-  const char* names[NV] = {"FP32 Gram", "FP64 Gram", "dd Gram", "Ozaki-4 Gram",
+  enum { FP32, FP64, DD, OZ3, DEVICE, NV };
+  const char* names[NV] = {"FP32 Gram", "FP64 Gram", "dd Gram", "Ozaki-3 Gram",
                            "device SVQB"};
   auto gram = [&](int v) {
     if (v == FP32) {
@@ -130,7 +130,7 @@ int main(int argc, char** argv) {
                                    nullptr));
     else
       SKINNY_CHECK(skinny_dtsmttsm_ozaki(nullptr, 0, M, M, K, W.p, M, 1.0, W.p, M, 0.0,
-                                         S.p, M, 4, nullptr));
+                                         S.p, M, 3, nullptr));
     HIP_CHECK(hipMemcpy(hS.data(), S.p, sizeof(double) * M * M, hipMemcpyDeviceToHost));
   };
   // max |Q^T Q - I| with the Gram in double-double

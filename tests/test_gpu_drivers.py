@@ -53,7 +53,7 @@ def test_perf_into_benchdb(gpu, tmp_path):
     names = {(m, n, name, inpl) for m, n, name, inpl, zb, bw, t in rows}
     # both variants (library FP64 and Ozaki), A*B and Gram, both widths
     for M in (4, 5):
-        for name in ("SKINNY", "OZAKI4"):
+        for name in ("SKINNY", "OZAKI3"):
             for inpl in (0, 1):
                 assert (M, M, name, inpl) in names
     assert all(bw > 0 and t > 0 for *_, bw, t in rows)
@@ -66,8 +66,8 @@ def test_svqb_precision_table(gpu):
     assert r.returncode == 0, r.stdout + r.stderr
     rows = [l for l in r.stdout.splitlines() if l.startswith("| ") and "pert" not in l]
     assert len(rows) >= 8
-    pert, fp32, fp64, dd, oz4, dev = [float(x) for x in rows[0].strip("| ").split(" | ")]
+    pert, fp32, fp64, dd, oz3, dev = [float(x) for x in rows[0].strip("| ").split(" | ")]
     # well-conditioned columns: FP64-or-better Gram matrices orthogonalise to
-    # near double precision; the 4-limb Ozaki Gram (coarse only in the two
+    # near double precision; the 3-limb Ozaki Gram (coarse only in the two
     # outlier chunks) to a few 1e-6
-    assert max(fp64, dd, dev) < 1e-8 and oz4 < 1e-4, rows[0]
+    assert max(fp64, dd, dev) < 1e-8 and oz3 < 1e-4, rows[0]

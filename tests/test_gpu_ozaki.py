@@ -68,7 +68,7 @@ def test_bitwise_outlier(gpu, orc):
         assert np.array_equal(host(dC), want), S
 
 
-@pytest.mark.parametrize("S", [4, 5])
+@pytest.mark.parametrize("S", [3, 4, 5])
 def test_c3_error(gpu, orc, S):
     M = 128
     K = 1 << 17
