@@ -55,6 +55,171 @@ def traffic_from_profiles(kernel_key: str):
         return None
 
 
+def _time_ms(torch, fn, reps=3):
+    """Median milliseconds of fn() over reps calls (one untimed call first),
+    HIP events on the current stream."""
+    fn()
+    ts = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
+def extra_configs(sk, torch, logK: int) -> dict:
+    """The other benchmark configurations (not part of `value`):
+    C2 complex-double TSMM and in-place FP64 TSMM over the same widths at
+    K = 2^logK; C3 Ozaki-INT8 against FP64 TSMTTSM at M = N = 128, K = 2^24,
+    errors against the double-double (HR) result; C4 one persistent batched
+    launch over 512 uneven products; C5 two SVQB sweeps over 64 blocks
+    (K = 2^22, M cycling 8..64) replayed from a hipGraph."""
+    import random
+
+    dev = torch.device("cuda:0")
+    out = {}
+    K = 1 << logK
+
+    # ---- C2: DC TSMM and in-place DR TSMM, GB/s per width ----------------
+    dc, ip = {}, {}
+    for M in WIDTHS:
+        log(f"C2 width {M}")
+        A = torch.empty(2 * K * M, dtype=torch.float64, device=dev)
+        Cc = torch.empty(2 * K * M, dtype=torch.float64, device=dev)
+        Bc = torch.empty(2 * M * M, dtype=torch.float64, device=dev)
+        sk.fill_uniform(A, A.numel(), 3, 0)
+        sk.fill_uniform(Bc, Bc.numel(), 3, 1)
+        ms = _time_ms(torch, lambda: sk.ztsmm(0, M, M, K, A, M, 1.0, Bc, M, 0.0, Cc, M))
+        dc[str(M)] = round(op_bytes("tsmm", M, M, K) * 2 / (ms * 1e-3) / 1e9, 1)
+        del Cc
+        X = A[: K * M]
+        Br = Bc[: M * M]
+        ms = _time_ms(torch, lambda: sk.dtsmm(0, M, M, K, X, M, 1.0, Br, M, 0.0, X, M))
+        ip[str(M)] = round(op_bytes("tsmm", M, M, K) / (ms * 1e-3) / 1e9, 1)
+        del A, Bc, X, Br
+        torch.cuda.empty_cache()
+    out["c2_dc_tsmm_GBs"] = dc
+    out["c2_inplace_tsmm_GBs"] = ip
+
+    # ---- C3: Ozaki vs FP64 vs dd, M = N = 128, K = 2^24 ------------------
+    log("C3 ozaki")
+    M = 128
+    K3 = 1 << 24
+    # C3 data: column j = (v + 1e-8 u_j) 2^e_j with v, u_j ~ U[-1,1) and
+    # integer e_j ~ U[-20, 20] (exact scaling); B = A (the Gram)
+    v = torch.empty(K3, dtype=torch.float64, device=dev)
+    sk.fill_uniform(v, K3, 5, 0)
+    A = torch.empty(K3 * M, dtype=torch.float64, device=dev)
+    sk.fill_uniform(A, A.numel(), 5, 1)
+    erng = random.Random(5)
+    scale = torch.tensor([2.0 ** erng.randint(-20, 20) for _ in range(M)],
+                         dtype=torch.float64, device=dev)
+    A.view(K3, M).mul_(1e-8).add_(v[:, None]).mul_(scale)
+    del v
+    B = A
+    Cd = torch.zeros(M * M, dtype=torch.float64, device=dev)
+    Cq = torch.zeros(2 * M * M, dtype=torch.float64, device=dev)
+    assert sk.htsmttsm_dd(0, M, M, K3, A, M, 1.0, B, M, 0.0, Cq, M)
+    torch.cuda.synchronize()
+    q = Cq.view(M * M, 2)
+    ref_hi, ref_lo = q[:, 0].clone(), q[:, 1].clone()
+
+    def rel_err(C):
+        return float(((C - ref_hi) - ref_lo).norm() / ref_hi.norm())
+
+    c3 = {"workload": "TSMTTSM Gram A^T A, M=N=128, K=2^24, columns (v + 1e-8 u_j) 2^e_j",
+          "error": "relative Frobenius against the double-double (HR) result"}
+    ms64 = _time_ms(torch, lambda: sk.dtsmttsm(0, M, M, K3, A, M, 1.0, B, M, 0.0, Cd, M), 5)
+    c3["fp64_ms"] = round(ms64, 3)
+    c3["fp64_rel_err"] = rel_err(Cd)
+    for S in (3, 4, 5):
+        ms = _time_ms(torch, lambda: sk.dtsmttsm_ozaki(0, M, M, K3, A, M, 1.0, B, M, 0.0, Cd,
+                                                      M, slices=S), 5)
+        c3[f"ozaki{S}_ms"] = round(ms, 3)
+        c3[f"ozaki{S}_speedup"] = round(ms64 / ms, 3)
+        c3[f"ozaki{S}_rel_err"] = rel_err(Cd)
+    c3["tolerance_8Keps"] = 8 * K3 * 2.0 ** -53
+    out["c3"] = c3
+    del A, B, Cd, Cq, q, ref_hi, ref_lo, scale
+    torch.cuda.empty_cache()
+
+    # ---- C4: 512 uneven products in one persistent launch ----------------
+    log("C4 batch")
+    rng = random.Random(4)
+    choices = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64)
+    shapes = []
+    total = 0
+    for i in range(512):
+        Kb = int(2 ** rng.uniform(12, 22))
+        Mb, Nb = rng.choice(choices), rng.choice(choices)
+        op = "tsmttsm" if i % 2 == 0 else "tsmm"
+        sizes = (Kb * Mb, Kb * Nb, Mb * Nb) if op == "tsmttsm" else (Kb * Mb, Mb * Nb, Kb * Nb)
+        shapes.append((op, Mb, Nb, Kb, sizes))
+        total += sum((n + 31) // 32 * 32 for n in sizes)
+    pool = torch.empty(total, dtype=torch.float64, device=dev)
+    sk.fill_uniform(pool, pool.numel(), 7, 0)
+    base, off, ops, nbytes = pool.data_ptr(), 0, [], 0
+    for op, Mb, Nb, Kb, sizes in shapes:
+        ptrs = []
+        for n in sizes:
+            ptrs.append(base + 8 * off)
+            off += (n + 31) // 32 * 32
+        a, b, c = ptrs
+        if op == "tsmttsm":
+            ops.append((op, Mb, Nb, Kb, a, Mb, 1.0, b, Nb, 0.0, c, Mb))
+        else:
+            ops.append((op, Mb, Nb, Kb, a, Mb, 1.0, b, Mb, 0.0, c, Nb))
+        nbytes += op_bytes(op, Mb, Nb, Kb) if op == "tsmm" else (Mb + Nb) * Kb * 8
+    batch = sk.Batch(ops)
+    ms = _time_ms(torch, batch.run, 5)
+    out["c4"] = {"ops": len(ops), "items": batch.items, "ms": round(ms, 3),
+                 "GBs": round(nbytes / (ms * 1e-3) / 1e9, 1), "bytes": nbytes}
+    batch.close()
+    del pool
+    torch.cuda.empty_cache()
+
+    # ---- C5: SVQB, 64 blocks, K = 2^22, two sweeps from a hipGraph -------
+    log("C5 svqb")
+    K5 = 1 << 22
+    blocks, bufs = [], []
+    for i in range(64):
+        M5 = (8, 16, 32, 64)[i % 4]
+        X = torch.empty(K5 * M5, dtype=torch.float64, device=dev)
+        sk.fill_uniform(X, X.numel(), 9, i)
+        sigma = torch.logspace(0, -6, M5, dtype=torch.float64, device=dev)
+        X.view(K5, M5).mul_(sigma)
+        bufs.append((X, X.clone()))
+        blocks.append((M5, K5, X, M5))
+    graph = sk.Graph(blocks, sweeps=2)
+    torch.cuda.synchronize()
+
+    def replay():
+        for X, X0 in bufs:
+            X.copy_(X0)
+        graph.launch()
+
+    # restore + replay; the restore copies are timed first and removed, and
+    # the errors are taken after the last replay
+    ms_copy = _time_ms(torch, lambda: [X.copy_(X0) for X, X0 in bufs], 3)
+    ms_all = _time_ms(torch, replay, 3)
+    errs = [sk.orth_error(M5, K5, X, M5) for (M5, K5, X, _) in blocks]
+    svqb_bytes = sum(2 * 3 * M5 * K5 * 8 for (M5, K5, _, _) in blocks)
+    ms_graph = max(ms_all - ms_copy, 1e-6)
+    out["c5"] = {"blocks": 64, "K": K5, "sweeps": 2, "ms": round(ms_graph, 3),
+                 "ms_per_block": round(ms_graph / 64, 4),
+                 "GBs": round(svqb_bytes / (ms_graph * 1e-3) / 1e9, 1),
+                 "max_orth_err_fro": max(errs)}
+    graph.close()
+    del bufs, blocks
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(K_cpu: int, min_seconds: float, threads: int):
     import numpy as np
 
@@ -100,6 +265,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the untimed autotuning of each shape")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the C2-C5 measurements reported beside the headline")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -185,6 +352,11 @@ def main() -> int:
     traffic = traffic_from_profiles(kernel_key)
     value = step_bytes * args.steps / wall / 1e9
     log(f"per-width GB/s: {per_width}")
+    del A, B, Cs, Bsm
+    torch.cuda.empty_cache()
+    extras = None if args.headline_only else extra_configs(sk, torch, args.logK)
+    if extras:
+        log(f"extras: {json.dumps(extras)}")
 
     cpu = None
     if not args.no_cpu:
@@ -224,6 +396,7 @@ def main() -> int:
             "traffic": traffic,
         },
         "cpu_baseline": cpu,
+        "configs": extras,
     }
     print(json.dumps(out))
     sys.stdout.flush()
