@@ -76,7 +76,8 @@ def extra_configs(sk, torch, logK: int) -> dict:
     """The other benchmark configurations (not part of `value`):
     C2 complex-double TSMM and in-place FP64 TSMM over the same widths at
     K = 2^logK; C3 Ozaki-INT8 against FP64 TSMTTSM at M = N = 128, K = 2^24,
-    errors against the double-double (HR) result; C4 one persistent batched
+    for the Gram product and a general one, errors against the
+    double-double (HR) result; C4 one persistent batched
     launch over 512 uneven products; C5 two SVQB sweeps over 64 blocks
     (K = 2^22, M cycling 8..64) replayed from a hipGraph."""
     import random
@@ -107,45 +108,55 @@ def extra_configs(sk, torch, logK: int) -> dict:
     out["c2_inplace_tsmm_GBs"] = ip
 
     # ---- C3: Ozaki vs FP64 vs dd, M = N = 128, K = 2^24 ------------------
-    log("C3 ozaki")
+    # C3 data: column j = (v + 1e-8 u_j) 2^e_j with v, u_j ~ U[-1,1) and
+    # integer e_j ~ U[-20, 20] (exact scaling).  Two cases: the Gram product
+    # A^T A (B is A: read once, tiles below the diagonal mirrored) and a
+    # general A^T B with B built the same way from other noise.
     M = 128
     K3 = 1 << 24
-    # C3 data: column j = (v + 1e-8 u_j) 2^e_j with v, u_j ~ U[-1,1) and
-    # integer e_j ~ U[-20, 20] (exact scaling); B = A (the Gram)
+    erng = random.Random(5)
     v = torch.empty(K3, dtype=torch.float64, device=dev)
     sk.fill_uniform(v, K3, 5, 0)
-    A = torch.empty(K3 * M, dtype=torch.float64, device=dev)
-    sk.fill_uniform(A, A.numel(), 5, 1)
-    erng = random.Random(5)
-    scale = torch.tensor([2.0 ** erng.randint(-20, 20) for _ in range(M)],
-                         dtype=torch.float64, device=dev)
-    A.view(K3, M).mul_(1e-8).add_(v[:, None]).mul_(scale)
-    del v
-    B = A
+
+    def c3_matrix(stream_id):
+        X = torch.empty(K3 * M, dtype=torch.float64, device=dev)
+        sk.fill_uniform(X, X.numel(), 5, stream_id)
+        scale = torch.tensor([2.0 ** erng.randint(-20, 20) for _ in range(M)],
+                             dtype=torch.float64, device=dev)
+        X.view(K3, M).mul_(1e-8).add_(v[:, None]).mul_(scale)
+        return X
+
+    A = c3_matrix(1)
+    c3 = {"workload": "TSMTTSM A^T B, M=N=128, K=2^24, columns (v + 1e-8 u_j) 2^e_j",
+          "error": "relative Frobenius against the double-double (HR) result",
+          "tolerance_8Keps": 8 * K3 * 2.0 ** -53}
     Cd = torch.zeros(M * M, dtype=torch.float64, device=dev)
     Cq = torch.zeros(2 * M * M, dtype=torch.float64, device=dev)
-    assert sk.htsmttsm_dd(0, M, M, K3, A, M, 1.0, B, M, 0.0, Cq, M)
-    torch.cuda.synchronize()
-    q = Cq.view(M * M, 2)
-    ref_hi, ref_lo = q[:, 0].clone(), q[:, 1].clone()
+    for case in ("gram", "general"):
+        log(f"C3 ozaki {case}")
+        B = A if case == "gram" else c3_matrix(2)
+        assert sk.htsmttsm_dd(0, M, M, K3, A, M, 1.0, B, M, 0.0, Cq, M)
+        torch.cuda.synchronize()
+        q = Cq.view(M * M, 2)
+        ref_hi, ref_lo = q[:, 0].clone(), q[:, 1].clone()
 
-    def rel_err(C):
-        return float(((C - ref_hi) - ref_lo).norm() / ref_hi.norm())
+        def rel_err(C):
+            return float(((C - ref_hi) - ref_lo).norm() / ref_hi.norm())
 
-    c3 = {"workload": "TSMTTSM Gram A^T A, M=N=128, K=2^24, columns (v + 1e-8 u_j) 2^e_j",
-          "error": "relative Frobenius against the double-double (HR) result"}
-    ms64 = _time_ms(torch, lambda: sk.dtsmttsm(0, M, M, K3, A, M, 1.0, B, M, 0.0, Cd, M), 5)
-    c3["fp64_ms"] = round(ms64, 3)
-    c3["fp64_rel_err"] = rel_err(Cd)
-    for S in (3, 4, 5):
-        ms = _time_ms(torch, lambda: sk.dtsmttsm_ozaki(0, M, M, K3, A, M, 1.0, B, M, 0.0, Cd,
-                                                      M, slices=S), 5)
-        c3[f"ozaki{S}_ms"] = round(ms, 3)
-        c3[f"ozaki{S}_speedup"] = round(ms64 / ms, 3)
-        c3[f"ozaki{S}_rel_err"] = rel_err(Cd)
-    c3["tolerance_8Keps"] = 8 * K3 * 2.0 ** -53
+        r = {}
+        ms64 = _time_ms(torch, lambda: sk.dtsmttsm(0, M, M, K3, A, M, 1.0, B, M, 0.0, Cd, M), 5)
+        r["fp64_ms"] = round(ms64, 3)
+        r["fp64_rel_err"] = rel_err(Cd)
+        for S in (3, 4, 5):
+            ms = _time_ms(torch, lambda: sk.dtsmttsm_ozaki(0, M, M, K3, A, M, 1.0, B, M, 0.0,
+                                                          Cd, M, slices=S), 5)
+            r[f"ozaki{S}_ms"] = round(ms, 3)
+            r[f"ozaki{S}_speedup"] = round(ms64 / ms, 3)
+            r[f"ozaki{S}_rel_err"] = rel_err(Cd)
+        c3[case] = r
+        del B, q, ref_hi, ref_lo
     out["c3"] = c3
-    del A, B, Cd, Cq, q, ref_hi, ref_lo, scale
+    del A, v, Cd, Cq
     torch.cuda.empty_cache()
 
     # ---- C4: 512 uneven products in one persistent launch ----------------

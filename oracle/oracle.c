@@ -759,15 +759,18 @@ void orc_gram_error(int64_t M, int64_t K, const double* X, int64_t ldx,
 
 /* ------------------------------------------------------------------------ */
 /* Ozaki split TSMTTSM, restated for blockCount = 1 (one workgroup per       */
-/* 128 x 128 tile walking all 32-row chunks in order): the bit-exact twin of */
-/* the device path.  Algorithm: Ozaki, Ogita, Oishi, Rump, "Error-free       */
+/* 128 x 128 tile walking all row chunks in order): the bit-exact twin of    */
+/* the device path.  Chunks are 32 rows, or 64 rows for the diagonal tiles  */
+/* of a Gram product (A == B, lda == ldb, M == N, lda <= 2^21), where both  */
+/* operands share one scale per column.  Algorithm: Ozaki, Ogita, Oishi, Rump, "Error-free       */
 /* transformations of matrix multiplication by using fast routines of matrix */
 /* multiplication and its applications", Numer. Algorithms 59 (2012), with   */
 /* 7-bit limbs of the rounded integer q = rint(|x| 2^(7S - E)), levels up to  */
 /* s + t = S + 2 and a running per-column scale.                             */
 /* ------------------------------------------------------------------------ */
 #define OZ_TILE 128
-#define OZ_ROWS 32
+#define OZ_ROWS 32      /* rows per chunk */
+#define OZ_ROWS_SELF 64 /* rows per chunk, diagonal tiles of a Gram product */
 #define OZ_FLUSH 16384
 #define OZ_EMIN (-4000)
 
@@ -805,23 +808,25 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
   const int S4 = S < 4 ? S : 4;
   int64_t* acc = (int64_t*)malloc(sizeof(int64_t) * OZ_TILE * OZ_TILE * L);
   double* cacc = (double*)malloc(sizeof(double) * OZ_TILE * OZ_TILE);
-  signed char* da = (signed char*)malloc((size_t)S * OZ_ROWS * OZ_TILE);
-  signed char* db = (signed char*)malloc((size_t)S * OZ_ROWS * OZ_TILE);
+  signed char* da = (signed char*)malloc((size_t)S * OZ_ROWS_SELF * OZ_TILE);
+  signed char* db = (signed char*)malloc((size_t)S * OZ_ROWS_SELF * OZ_TILE);
+  const int gram = A == B && lda == ldb && M == N && lda <= ((int64_t)1 << 21);
   for (int64_t m0 = 0; m0 < M; m0 += OZ_TILE)
     for (int64_t n0 = 0; n0 < N; n0 += OZ_TILE) {
+      const int R = (gram && m0 == n0) ? OZ_ROWS_SELF : OZ_ROWS;
       int erun[2][OZ_TILE], ecur[2][OZ_TILE];
       for (int c = 0; c < OZ_TILE; c++) erun[0][c] = erun[1][c] = OZ_EMIN;
       memset(acc, 0, sizeof(int64_t) * OZ_TILE * OZ_TILE * L);
       memset(cacc, 0, sizeof(double) * OZ_TILE * OZ_TILE);
       int since = 0;
-      for (int64_t r0 = 0; r0 < K; r0 += OZ_ROWS) {
-        int flush = (since + OZ_ROWS > OZ_FLUSH);
+      for (int64_t r0 = 0; r0 < K; r0 += R) {
+        int flush = (since + R > OZ_FLUSH);
         for (int mat = 0; mat < 2; mat++)
           for (int c = 0; c < OZ_TILE; c++) {
             int64_t gc = (mat == 0 ? m0 : n0) + c;
             int valid = gc < (mat == 0 ? M : N);
             double mx = 0.0;
-            for (int i = 0; i < OZ_ROWS && valid; i++) {
+            for (int i = 0; i < R && valid; i++) {
               int64_t r = r0 + i;
               if (r >= K) break;
               double x = mat == 0 ? A[r * lda + gc] : B[r * ldb + gc];
@@ -856,7 +861,7 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
             int64_t gc = (mat == 0 ? m0 : n0) + c;
             int valid = gc < (mat == 0 ? M : N);
             int E = erun[mat][c];
-            for (int i = 0; i < OZ_ROWS; i++) {
+            for (int i = 0; i < R; i++) {
               int64_t r = r0 + i;
               double x = (valid && r < K) ? (mat == 0 ? A[r * lda + gc] : B[r * ldb + gc]) : 0.0;
               int digit[5] = {0, 0, 0, 0, 0};
@@ -868,7 +873,7 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
                 if (S > 4) digit[4] = (int)fmin(nearbyint((t - (double)q) * 128.0), 127.0);
               }
               for (int s2 = 0; s2 < S; s2++)
-                d[((size_t)s2 * OZ_TILE + c) * OZ_ROWS + i] =
+                d[((size_t)s2 * OZ_TILE + c) * R + i] =
                     (signed char)(x < 0.0 ? -digit[s2] : digit[s2]);
             }
           }
@@ -878,18 +883,18 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
           for (int s1 = 1; s1 <= S; s1++) {
             int t1 = g - s1;
             if (t1 < 1 || t1 > S) continue;
-            const signed char* pa = da + (size_t)(s1 - 1) * OZ_TILE * OZ_ROWS;
-            const signed char* pb = db + (size_t)(t1 - 1) * OZ_TILE * OZ_ROWS;
+            const signed char* pa = da + (size_t)(s1 - 1) * OZ_TILE * R;
+            const signed char* pb = db + (size_t)(t1 - 1) * OZ_TILE * R;
             for (int n = 0; n < OZ_TILE; n++)
               for (int m = 0; m < OZ_TILE; m++) {
                 int64_t sum = 0;
-                const signed char* x = pa + (size_t)m * OZ_ROWS;
-                const signed char* y = pb + (size_t)n * OZ_ROWS;
-                for (int i = 0; i < OZ_ROWS; i++) sum += (int64_t)x[i] * y[i];
+                const signed char* x = pa + (size_t)m * R;
+                const signed char* y = pb + (size_t)n * R;
+                for (int i = 0; i < R; i++) sum += (int64_t)x[i] * y[i];
                 acc[((int64_t)n * OZ_TILE + m) * L + (g - 2)] += sum;
               }
           }
-        since += OZ_ROWS;
+        since += R;
       }
       oz_flush_tile(L, acc, cacc, erun[0], erun[1]);
       for (int64_t n = 0; n < OZ_TILE && n0 + n < N; n++)

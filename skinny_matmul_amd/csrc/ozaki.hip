@@ -37,6 +37,13 @@
 //    flight while this chunk is split (VALU, packed 4 rows to a word) into
 //    one LDS buffer and the previous chunk's limbs in the other buffer are
 //    multiplied, the two interleaved instruction by instruction.
+//  * Gram products (A == B, lda == ldb, M == N) read A once: the diagonal
+//    tiles run with 64-row chunks whose two 32-row halves take the places of
+//    A and B in the staging (one scale per column, shared by both operands),
+//    and of the off-diagonal tiles only those above the diagonal are
+//    computed, each written to C twice (C(m, n) and C(n, m)).  The loads in
+//    flight per workgroup stay the same, so a Gram product streams half the
+//    bytes of a general one in about half the time.
 #include <cmath>
 #include <utility>
 
@@ -83,7 +90,7 @@ __device__ __forceinline__ int i8_drow(int lane, int r) {
 }
 
 constexpr int kOzTile = 128;
-constexpr int kOzRows = 32;
+constexpr int kOzRows = 32;  // rows per thread and chunk (rows per chunk: 32, Gram diagonal 64)
 constexpr int kOzThreads = 256;
 constexpr int kOzFlushRows = 16384;
 constexpr int kOzEmin = -4000;
@@ -100,7 +107,31 @@ struct OzArgs {
   int* stage;  // per wave: (S + 1) * 4096 INT32 accumulator staging
   unsigned* cnt;
   int G, GS, ngroups, tilesM, tilesN;
+  int ntiles;  // tiles of this launch
+  int mode;    // tile set: OZ_ALL, OZ_DIAG or OZ_UPPER
 };
+
+// The tile sets a launch covers: all tilesM x tilesN tiles (column-major
+// numbering), the diagonal tiles of a Gram product, or its tiles above the
+// diagonal (column by column), each also written transposed.
+enum { OZ_ALL = 0, OZ_DIAG = 1, OZ_UPPER = 2 };
+
+__device__ __forceinline__ void oz_tile_of(const OzArgs& a, int tile, int& tm,
+                                           int& tn) {
+  if (a.mode == OZ_ALL) {
+    tm = tile % a.tilesM;
+    tn = tile / a.tilesM;
+  } else if (a.mode == OZ_DIAG) {
+    tm = tn = tile;
+  } else {  // column tn holds tiles tm = 0 .. tn - 1
+    tn = 1;
+    while (tile >= tn) {
+      tile -= tn;
+      tn++;
+    }
+    tm = tile;
+  }
+}
 
 // Empty asm that requires the accumulators in the matrix-core (AGPR)
 // registers: placed at the top of the chunk loop it keeps the register
@@ -251,10 +282,12 @@ constexpr OzMf oz_mfma_of(int k) {
   return OzMf{0, 0, 0};
 }
 
-template <int S>
+// SELF: a diagonal tile of a Gram product (64-row chunks, the second index of
+// the limb staging is the chunk's 32-row half instead of A/B).
+template <int S, bool SELF>
 __global__ void __launch_bounds__(kOzThreads)
     __attribute__((amdgpu_waves_per_eu(1, 1))) ozaki_kernel(OzArgs a) {
-  // limbs of two chunks (double buffer): [buffer][A/B][limb][column][row]
+  // limbs of two chunks (double buffer): [buffer][A/B or half][limb][column][row]
   __shared__ __attribute__((aligned(16))) signed char sl[2][2][S][kOzTile][kOzRows];
   __shared__ int erun[2][kOzTile];
   __shared__ int s_last;
@@ -264,22 +297,30 @@ __global__ void __launch_bounds__(kOzThreads)
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave >> 1, wn = wave & 1;  // this wave's 64 x 64 region
-  const int ntiles = a.tilesM * a.tilesN;
+  const int ntiles = a.ntiles;
   const int tile = blockIdx.x % ntiles;
   const int kp = blockIdx.x / ntiles;
-  const int m0 = (tile % a.tilesM) * kOzTile;
-  const int n0 = (tile / a.tilesM) * kOzTile;
+  int tm, tn;
+  oz_tile_of(a, tile, tm, tn);
+  const int m0 = tm * kOzTile;
+  const int n0 = tn * kOzTile;
 
-  // waves 0-1 stream the 128 columns of A's tile, waves 2-3 those of B;
-  // thread `col` holds its column's 32 rows of the current chunk
-  const int mat = wave >> 1, col = tid & 127;
-  const int width = mat == 0 ? a.M : a.N;
-  const int gcol = (mat == 0 ? m0 : n0) + col;
+  // General tiles: waves 0-1 stream the 128 columns of A's tile, waves 2-3
+  // those of B; thread `col` holds its column's 32 rows of the current chunk.
+  // SELF: wave w streams columns 32 w .. 32 w + 31, lanes 0-31 the chunk's
+  // first 32 rows and lanes 32-63 its last 32 (mat = the half), so the two
+  // halves of a column meet in one wave.
+  constexpr int CR = SELF ? 2 * kOzRows : kOzRows;  // rows per chunk
+  const int mat = SELF ? lane >> 5 : wave >> 1;
+  const int col = SELF ? 32 * wave + (lane & 31) : tid & 127;
+  const bool isA = SELF || mat == 0;
+  const int width = isA ? a.M : a.N;
+  const int gcol = (isA ? m0 : n0) + col;
   const bool cvalid = gcol < width;
-  const int voff = 8 * (cvalid ? gcol : 0);
-  const double* mbase = mat == 0 ? a.A : a.B;
-  const int64_t ld = mat == 0 ? a.lda : a.ldb;
+  const double* mbase = isA ? a.A : a.B;
+  const int64_t ld = isA ? a.lda : a.ldb;
   const int ld8 = (int)(8 * ld);
+  const int voff = 8 * (cvalid ? gcol : 0) + (SELF ? kOzRows * mat * ld8 : 0);
 
   erun[mat][col] = kOzEmin;
   constexpr int L = oz_levels<S>();  // accumulator levels g = 2 .. S + 2
@@ -310,15 +351,15 @@ __global__ void __launch_bounds__(kOzThreads)
   const int* eb_w = &erun[1][64 * wn];
   __syncthreads();
 
-  const int64_t nch = (a.K + kOzRows - 1) / kOzRows;
+  const int64_t nch = (a.K + CR - 1) / CR;
   double x[kOzRows], xn[kOzRows];  // this chunk and the next one in flight
-  // rows [32 c, 32 c + 32) of this thread's column: the chunk's resource
+  // this thread's 32 rows of chunk c in its column: the chunk's resource
   // ends after its last row below K, so rows at or past K read 0 and every
   // chunk issues the same 32 loads back to back.  Padding columns read
   // column 0 and are zeroed through their scale.
   auto load_chunk = [&](int64_t c, double (&dst)[kOzRows]) {
-    const int64_t r0 = c * kOzRows;
-    const int rows = (int)min<int64_t>(kOzRows, a.K - r0);
+    const int64_t r0 = c * CR;
+    const int rows = (int)min<int64_t>(CR, a.K - r0);
     const __amdgpu_buffer_rsrc_t rs =
         oz_rsrc(mbase + r0 * ld, 8 * ((int64_t)(rows - 1) * ld + width));
 #pragma unroll
@@ -332,6 +373,7 @@ __global__ void __launch_bounds__(kOzThreads)
     for (int i = 0; i < kOzRows; i++) mx = fmax(mx, fabs(x[i]));
     int e = kOzEmin;
     if (cvalid && mx > 0.0) (void)frexp(mx, &e);
+    if (SELF) e = max(e, __shfl_xor(e, 32));  // both halves of the column
     return e;
   };
   // the scale the limbs are taken against: 2^E for a live column; an
@@ -368,21 +410,23 @@ __global__ void __launch_bounds__(kOzThreads)
   const int kh = 16 * (lane >> 5);
   auto multiply = [&](int b) {
 #pragma unroll
-    for (int tt = 0; tt < 4; tt++) {
-      const int mcol = 64 * wm + 32 * (tt >> 1) + (lane & 31);
-      const int ncol = 64 * wn + 32 * (tt & 1) + (lane & 31);
+    for (int h = 0; h < (SELF ? 2 : 1); h++)  // SELF: both halves of the chunk
 #pragma unroll
-      for (int g = 2; g <= S + 2; g++) {
+      for (int tt = 0; tt < 4; tt++) {
+        const int mcol = 64 * wm + 32 * (tt >> 1) + (lane & 31);
+        const int ncol = 64 * wn + 32 * (tt & 1) + (lane & 31);
 #pragma unroll
-        for (int s1 = 1; s1 <= S; s1++) {
-          const int t1 = g - s1;
-          if (t1 < 1 || t1 > S) continue;
-          const v4i av = *reinterpret_cast<const v4i*>(&sl[b][0][s1 - 1][mcol][kh]);
-          const v4i bv = *reinterpret_cast<const v4i*>(&sl[b][1][t1 - 1][ncol][kh]);
-          acc[g - 2][tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc[g - 2][tt], 0, 0, 0);
+        for (int g = 2; g <= S + 2; g++) {
+#pragma unroll
+          for (int s1 = 1; s1 <= S; s1++) {
+            const int t1 = g - s1;
+            if (t1 < 1 || t1 > S) continue;
+            const v4i av = *reinterpret_cast<const v4i*>(&sl[b][SELF ? h : 0][s1 - 1][mcol][kh]);
+            const v4i bv = *reinterpret_cast<const v4i*>(&sl[b][SELF ? h : 1][t1 - 1][ncol][kh]);
+            acc[g - 2][tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc[g - 2][tt], 0, 0, 0);
+          }
         }
       }
-    }
   };
   // Steady-state step: split the chunk in x into LDS buffer bs while the
   // chunk staged in buffer bm is multiplied.  The splits of 8 groups of 4
@@ -391,15 +435,21 @@ __global__ void __launch_bounds__(kOzThreads)
   // core is busy.  Returns nonzero if the column's scale must restart: a
   // value outgrew it, or the chunk's largest limb integer is nonzero but 9
   // or more bits below it (the chunk after an outlier).
+  // SELF: the products of the chunk's first half run in units 0-3, its
+  // second half's operands are read at the end of unit 3 and multiplied in
+  // units 4-7.
   auto split_multiply = [&](int bm, int bs) {
     v4i av[2][S], bv[2][S];
+    auto operands = [&](int h2) {
 #pragma unroll
-    for (int h = 0; h < 2; h++)
+      for (int h = 0; h < 2; h++)
 #pragma unroll
-      for (int s2 = 0; s2 < S; s2++) {
-        av[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][0][s2][64 * wm + 32 * h + (lane & 31)][kh]);
-        bv[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][1][s2][64 * wn + 32 * h + (lane & 31)][kh]);
-      }
+        for (int s2 = 0; s2 < S; s2++) {
+          av[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][SELF ? h2 : 0][s2][64 * wm + 32 * h + (lane & 31)][kh]);
+          bv[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][SELF ? h2 : 1][s2][64 * wn + 32 * h + (lane & 31)][kh]);
+        }
+    };
+    operands(0);
     const int Es = split_scale();
     unsigned qmax = 0;
     double tmax = 0.0;
@@ -417,15 +467,17 @@ __global__ void __launch_bounds__(kOzThreads)
             for (int s2 = 0; s2 < S; s2++) words[s2][U % 4] = w[s2];
           }
           constexpr int NM = oz_nmfma<S>();
-          constexpr int k0 = U * NM / 8, k1 = (U + 1) * NM / 8;
+          constexpr int NT = SELF ? 2 * NM : NM;  // products per chunk
+          constexpr int k0 = U * NT / 8, k1 = (U + 1) * NT / 8;
           oz_static_for(
               [&](auto K_) {
-                constexpr OzMf m = oz_mfma_of<S>(k0 + decltype(K_)::value);
+                constexpr OzMf m = oz_mfma_of<S>((k0 + decltype(K_)::value) % NM);
                 acc[m.g - 2][m.tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
                     av[m.tt >> 1][m.s - 1], bv[m.tt & 1][m.g - m.s - 1],
                     acc[m.g - 2][m.tt], 0, 0, 0);
               },
               std::make_integer_sequence<int, k1 - k0>{});
+          if constexpr (SELF && U == 3) operands(1);
           if constexpr (U % 4 == 3) {
 #pragma unroll
             for (int s2 = 0; s2 < S; s2++) {
@@ -437,6 +489,7 @@ __global__ void __launch_bounds__(kOzThreads)
         },
         std::make_integer_sequence<int, kOzRows / 4>{});
     constexpr int S4 = S < 4 ? S : 4;
+    if (SELF) qmax = max(qmax, (unsigned)__shfl_xor((int)qmax, 32));
     return tmax >= (double)(1u << (7 * S4)) ||
            (qmax != 0u && qmax < (1u << (7 * S4 - 9)));
   };
@@ -460,21 +513,21 @@ __global__ void __launch_bounds__(kOzThreads)
     __syncthreads();
     load_chunk(min(ch + a.G, nch - 1), xn);
     limbs(pb);
-    int since = kOzRows;
+    int since = CR;
 #pragma unroll
     for (int i = 0; i < kOzRows; i++) x[i] = xn[i];
     __syncthreads();
     bool staged = true;  // buffer pb holds a chunk not yet multiplied
     for (ch += a.G; ch < nch; ch += a.G) {
       oz_pin<L>(acc);
-      if (since + kOzRows > kOzFlushRows) break;  // uniform
+      if (since + CR > kOzFlushRows) break;  // uniform
       load_chunk(min(ch + a.G, nch - 1), xn);
       const bool rescale = split_multiply(pb, pb ^ 1);
       if (__syncthreads_or(rescale)) {
         staged = false;  // buffer pb was multiplied; ch is split again
         break;
       }
-      since += kOzRows;
+      since += CR;
 #pragma unroll
       for (int i = 0; i < kOzRows; i++) x[i] = xn[i];
       pb ^= 1;
@@ -500,6 +553,7 @@ __global__ void __launch_bounds__(kOzThreads)
   if (!s_last) return;
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   const int Mloc = min(a.M - m0, kOzTile), Nloc = min(a.N - n0, kOzTile);
+  const bool mirror = a.mode == OZ_UPPER;
   auto apply = [&](int e, double s) {
     const int ml = e % kOzTile, nl = e / kOzTile;
     if (ml < Mloc && nl < Nloc) {
@@ -508,6 +562,12 @@ __global__ void __launch_bounds__(kOzThreads)
       double r = a.alpha * s;
       if (a.beta != 0.0) r = a.beta * *c + r;
       *c = r;
+      if (mirror) {  // C(n, m) of a Gram product
+        double* c2 = a.C + (int64_t)(m0 + ml) * a.ldc + (n0 + nl);
+        double r2 = a.alpha * s;
+        if (a.beta != 0.0) r2 = a.beta * *c2 + r2;
+        *c2 = r2;
+      }
     }
   };
   const double* gsrc = a.part1 + ((int64_t)tile * a.G + (int64_t)grp * a.GS) * PS;
@@ -587,58 +647,96 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
     return fail(SKINNY_UNSUPPORTED, "ozaki: lda, ldb <= 2^22");
   h = resolve(h);
   if (!h) return no_handle("ozaki");
-  OzArgs a;
-  a.tilesM = (int)((M + kOzTile - 1) / kOzTile);
-  a.tilesN = (int)((N + kOzTile - 1) / kOzTile);
-  const int ntiles = a.tilesM * a.tilesN;
-  int64_t G = (int64_t)blockCount;
-  if (G <= 0) {
-    G = std::max<int64_t>(1, h->cu_count / ntiles);
-    G = std::min<int64_t>(G, std::max<int64_t>(1, K / (kOzRows * 64)));
-  }
-  if (G > 65536) G = 65536;
-  a.G = (int)G;
-  if (a.G <= 16) {
-    a.GS = a.G;
-    a.ngroups = 1;
+  // Gram products read A once (64-row chunks on the diagonal tiles, whose
+  // offsets need lda <= 2^21) and skip the tiles below the diagonal
+  const bool gram = A == B && lda == ldb && M == N && lda <= (1 << 21);
+  const int tilesM = (int)((M + kOzTile - 1) / kOzTile);
+  const int tilesN = (int)((N + kOzTile - 1) / kOzTile);
+  struct Set {
+    int mode, ntiles;
+    bool self;
+    OzArgs a;
+  };
+  Set sets[2];
+  int nsets = 0;
+  if (gram) {
+    sets[nsets++] = Set{OZ_DIAG, tilesM, true, {}};
+    if (tilesM > 1) sets[nsets++] = Set{OZ_UPPER, tilesM * (tilesM - 1) / 2, false, {}};
   } else {
-    a.GS = isqrt_u(a.G);
-    a.ngroups = (a.G + a.GS - 1) / a.GS;
+    sets[nsets++] = Set{OZ_ALL, tilesM * tilesN, false, {}};
   }
-  if ((size_t)ntiles * (a.ngroups + 1) > (size_t)kMaxCounters)
-    return fail(SKINNY_UNSUPPORTED, "ozaki: too many partial groups");
   const size_t PS = (size_t)kOzTile * kOzTile;
-  size_t bytes = (size_t)ntiles * a.G * PS * sizeof(double);
-  if (a.ngroups > 1) bytes += (size_t)ntiles * a.ngroups * PS * sizeof(double);
-  const size_t part_bytes = bytes;
-  bytes += (size_t)ntiles * a.G * (kOzThreads / 64) * (slices + 1) * 4096 * sizeof(int);
+  size_t bytes = 0;  // the launches run one after the other: the larger one
+  for (int i = 0; i < nsets; i++) {
+    OzArgs& a = sets[i].a;
+    const int ntiles = sets[i].ntiles;
+    const int64_t rows = sets[i].self ? 2 * kOzRows : kOzRows;
+    int64_t G = (int64_t)blockCount;
+    if (G <= 0) {
+      G = std::max<int64_t>(1, h->cu_count / ntiles);
+      G = std::min<int64_t>(G, std::max<int64_t>(1, K / (rows * 64)));
+    }
+    if (G > 65536) G = 65536;
+    a.G = (int)G;
+    if (a.G <= 16) {
+      a.GS = a.G;
+      a.ngroups = 1;
+    } else {
+      a.GS = isqrt_u(a.G);
+      a.ngroups = (a.G + a.GS - 1) / a.GS;
+    }
+    if ((size_t)ntiles * (a.ngroups + 1) > (size_t)kMaxCounters)
+      return fail(SKINNY_UNSUPPORTED, "ozaki: too many partial groups");
+    size_t b = (size_t)ntiles * a.G * PS * sizeof(double);
+    if (a.ngroups > 1) b += (size_t)ntiles * a.ngroups * PS * sizeof(double);
+    b += (size_t)ntiles * a.G * (kOzThreads / 64) * (slices + 1) * 4096 * sizeof(int);
+    bytes = std::max(bytes, b);
+  }
   Workspace w;
   skinny_status_t st = acquire_workspace(h, (hipStream_t)stream, bytes, &w);
   if (st != SKINNY_OK) return st;
-  a.A = A;
-  a.B = B;
-  a.C = C;
-  a.K = K;
-  a.lda = lda;
-  a.ldb = ldb;
-  a.ldc = ldc;
-  a.M = (int)M;
-  a.N = (int)N;
-  a.alpha = alpha;
-  a.beta = beta;
-  a.part1 = static_cast<double*>(ws_scratch(w));
-  a.part2 = a.part1 + (size_t)ntiles * a.G * PS;
-  a.stage = reinterpret_cast<int*>(static_cast<char*>(ws_scratch(w)) + part_bytes);
-  a.cnt = ws_counters(w);
-  const int blocks = a.G * ntiles;
   hipStream_t s = (hipStream_t)stream;
-  switch (slices) {
-    case 3: hipLaunchKernelGGL(ozaki_kernel<3>, dim3(blocks), dim3(kOzThreads), 0, s, a); break;
-    case 4: hipLaunchKernelGGL(ozaki_kernel<4>, dim3(blocks), dim3(kOzThreads), 0, s, a); break;
-    default: hipLaunchKernelGGL(ozaki_kernel<5>, dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+  for (int i = 0; i < nsets; i++) {
+    OzArgs& a = sets[i].a;
+    const int ntiles = sets[i].ntiles;
+    a.A = A;
+    a.B = B;
+    a.C = C;
+    a.K = K;
+    a.lda = lda;
+    a.ldb = ldb;
+    a.ldc = ldc;
+    a.M = (int)M;
+    a.N = (int)N;
+    a.alpha = alpha;
+    a.beta = beta;
+    a.tilesM = tilesM;
+    a.tilesN = tilesN;
+    a.ntiles = ntiles;
+    a.mode = sets[i].mode;
+    a.part1 = static_cast<double*>(ws_scratch(w));
+    a.part2 = a.part1 + (size_t)ntiles * a.G * PS;
+    size_t part_bytes = (size_t)ntiles * a.G * PS * sizeof(double);
+    if (a.ngroups > 1) part_bytes += (size_t)ntiles * a.ngroups * PS * sizeof(double);
+    a.stage = reinterpret_cast<int*>(static_cast<char*>(ws_scratch(w)) + part_bytes);
+    a.cnt = ws_counters(w);
+    const int blocks = a.G * ntiles;
+    if (sets[i].self) {
+      switch (slices) {
+        case 3: hipLaunchKernelGGL((ozaki_kernel<3, true>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((ozaki_kernel<4, true>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+        default: hipLaunchKernelGGL((ozaki_kernel<5, true>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+      }
+    } else {
+      switch (slices) {
+        case 3: hipLaunchKernelGGL((ozaki_kernel<3, false>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((ozaki_kernel<4, false>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+        default: hipLaunchKernelGGL((ozaki_kernel<5, false>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+      }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "ozaki launch");
   }
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return hip_fail(e, "ozaki launch");
   return SKINNY_OK;
 }
 

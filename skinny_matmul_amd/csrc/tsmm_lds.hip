@@ -213,7 +213,7 @@ __device__ __forceinline__ float cfma(float a, float b, float c) {
 
 template <typename T, int MC, bool BETA0>
 __global__ void __launch_bounds__(256) ztsmm_rows_kernel(TlArgs<T> a) {
-  constexpr int R = MC <= 4 ? 4 : (MC <= 8 ? 2 : 1);  // rows in flight
+  constexpr int R = 4;  // rows in flight
   const int M = a.M / 2, N = a.N / 2;                   // complex widths
   const int64_t K = a.K;
   const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
@@ -321,13 +321,14 @@ skinny_status_t launch_tsmm_lds(skinny_handle_t h, size_t blockCount,
   int blocks = blockCount > 0 ? (int)blockCount
                               : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T));
   const bool beta0 = cplx ? (beta == T(0) && beta_i == T(0)) : (beta == T(0));
-  if (cplx && M <= 32) {  // at most 16 complex columns: whole rows per thread
+  // At most 4 complex columns: whole rows per thread.  Wider rows make each
+  // lane's loads span several cache lines and the LDS kernel is faster
+  // (measured for 8 and 16 complex columns at K = 2^26).
+  if (cplx && M <= 8) {
     const int Mc = (int)(M / 2);
     hipError_t e = Mc <= 1 ? launch_rows<T, 1>(a, blocks, beta0, stream)
                  : Mc <= 2 ? launch_rows<T, 2>(a, blocks, beta0, stream)
-                 : Mc <= 4 ? launch_rows<T, 4>(a, blocks, beta0, stream)
-                 : Mc <= 8 ? launch_rows<T, 8>(a, blocks, beta0, stream)
-                           : launch_rows<T, 16>(a, blocks, beta0, stream);
+                           : launch_rows<T, 4>(a, blocks, beta0, stream);
     if (e != hipSuccess) return hip_fail(e, "tsmm (complex rows) launch");
     return SKINNY_OK;
   }

@@ -5,8 +5,11 @@
 * blockCount = 1: bitwise equal to the oracle's restatement of the same
   sequence (limbs, INT32 levels, flush points, FP64 reconstruction order),
   including ragged K, padded shapes and magnitudes that grow along K;
+* Gram products (B is A): the diagonal tiles' 64-row chunks and the
+  mirrored tiles above the diagonal, bitwise against the oracle at
+  blockCount = 1 (ragged K, beta != 0, several tiles, an outlier);
 * default blockCount: error against the exact product within the scheme's
-  bound and within 8 K eps at K = 2^24 (3e-8) from 4 limbs on.
+  bound and within 8 K eps at K = 2^24 (1.5e-8) from 3 limbs on.
 """
 import numpy as np
 import pytest
@@ -65,6 +68,38 @@ def test_bitwise_outlier(gpu, orc):
         orc.ozaki_tsmttsm(M, N, K, A, M, 1.0, B, N, 0.0, want, M, S)
         dC = dev(np.zeros(M * N))
         assert gpu.dtsmttsm_ozaki(1, M, N, K, dev(A), M, 1.0, dev(B), N, 0.0, dC, M, slices=S)
+        assert np.array_equal(host(dC), want), S
+
+
+@pytest.mark.parametrize("M,K,S", [(5, 999, 3), (128, 3001, 3), (128, 40, 4), (128, 20, 3),
+                                   (200, 2000, 4), (300, 130, 5), (130, 7000, 3)])
+def test_bitwise_gram(gpu, orc, M, K, S):
+    note(f"ozaki gram bitwise M={M} K={K} S={S}")
+    A = orc.uniform(K * M, 2 * M, 0)
+    C0 = orc.uniform(M * M, 2 * M, 2)
+    want = C0.copy()
+    orc.ozaki_tsmttsm(M, M, K, A, M, 1.5, A, M, -1.0, want, M, S)
+    dA = dev(A)
+    dC = dev(C0.copy())
+    assert gpu.dtsmttsm_ozaki(1, M, M, K, dA, M, 1.5, dA, M, -1.0, dC, M, slices=S)
+    assert np.array_equal(host(dC), want)
+
+
+def test_bitwise_gram_outlier(gpu, orc):
+    """Outliers in one 32-row half of a 64-row chunk: the column's shared
+    scale, the growth flush and the restart, bitwise against the oracle."""
+    M, K = 40, 6000
+    note(f"ozaki gram outlier M={M} K={K}")
+    A = orc.uniform(K * M, 23, 0).copy()
+    A[100 * M + 3] = 1e5      # row 100: second half of chunk 1
+    A[2000 * M + 7] = -3e7    # row 2000: first half of chunk 31
+    A[4033 * M + 7] = 1e-6    # tiny rows do not trigger anything
+    for S in (3, 4):
+        want = np.zeros(M * M)
+        orc.ozaki_tsmttsm(M, M, K, A, M, 1.0, A, M, 0.0, want, M, S)
+        dA = dev(A)
+        dC = dev(np.zeros(M * M))
+        assert gpu.dtsmttsm_ozaki(1, M, M, K, dA, M, 1.0, dA, M, 0.0, dC, M, slices=S)
         assert np.array_equal(host(dC), want), S
 
 

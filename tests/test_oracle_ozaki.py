@@ -87,3 +87,30 @@ def test_outlier_coarsens_only_its_chunk(orc):
     err = np.abs((got - hi) - lo).reshape(N, M).T / ref
     assert err[0].max() <= 1e-5, err[0].max()
     assert err[1:].max() <= 8 * 5 * 2.0 ** -28, err[1:].max()
+
+
+@pytest.mark.parametrize("M,K", [(5, 999), (128, 3001), (200, 2000), (130, 40)])
+def test_gram_path(orc, M, K):
+    """A Gram product (B is A) takes 64-row chunks on the diagonal tiles, one
+    scale per column for both operands: exact on integers, symmetric bit for
+    bit (the off-diagonal tiles are transposes of each other), and within the
+    scheme's bound of the exact result like the general path on a copy."""
+    A = orc.integers(K * M, 3, 0, -100, 100)
+    want = np.zeros(M * M)
+    orc.tsmttsm(M, M, K, A, M, 1.0, A, M, 0.0, want, M)
+    got = np.zeros(M * M)
+    orc.ozaki_tsmttsm(M, M, K, A, M, 1.0, A, M, 0.0, got, M, 3)
+    assert np.array_equal(got, want)
+
+    A = orc.c3_columns(M, K, 7)
+    hi, lo = orc.xtsmttsm(M, M, K, A, M, A, M)
+    for S in (3, 4):
+        gram = np.zeros(M * M)
+        orc.ozaki_tsmttsm(M, M, K, A, M, 1.0, A, M, 0.0, gram, M, S)
+        g2 = gram.reshape(M, M)
+        assert np.array_equal(g2, g2.T), S
+        general = np.zeros(M * M)
+        orc.ozaki_tsmttsm(M, M, K, A, M, 1.0, A.copy(), M, 0.0, general, M, S)
+        for res in (gram, general):
+            err = np.linalg.norm((res - hi) - lo) / np.linalg.norm(hi)
+            assert err <= 8 * (S + 1) * 2.0 ** (-7 * S - 7), (S, err)
