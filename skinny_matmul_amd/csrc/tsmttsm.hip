@@ -17,11 +17,15 @@
 //    larger C is split into 64 x 64 tiles along the grid.
 //  * The four waves of a workgroup stream disjoint rows and are summed
 //    through LDS in a fixed pairwise order; each workgroup writes one partial.
-//    The last workgroup to arrive in a group of GS partials sums the group,
-//    and the last group to arrive sums the group partials and applies the
-//    alpha/beta epilogue (genv3.cuh:25-26).  Summation order is fixed, so the
-//    result is bitwise reproducible for a fixed blockCount.  Arrival counters
-//    are reset by their last user, so no memset is needed between calls.
+//  * With G > 1 workgroups per tile a second kernel sums the G partials of
+//    every entry and applies the alpha/beta epilogue (the two stages of
+//    genv3.cuh:9-27 and 31-80).  Handing the partials over at the kernel
+//    boundary costs one cache write-back for the whole grid; an in-kernel
+//    hand-over (release at device scope, then an arrival counter) writes
+//    back the XCD's L2 once per workgroup, which measured 40-60 us per call
+//    at 256-512 workgroups on MI355X.  Summation order is fixed, so the
+//    result is bitwise reproducible for a fixed blockCount.
+//  * With G == 1 the one workgroup of a tile applies the epilogue itself.
 #include "internal.hpp"
 #include "mfma.hpp"
 #include "tsmttsm_core.hpp"
@@ -43,6 +47,7 @@ struct TsArgs {
   T* part2;      // [tile][ngroups][PS]
   unsigned* cnt; // [tile][ngroups] then [tile]
   int G, GS, ngroups, tilesM, tilesN, outM, outN;
+  int two_pass;  // 1: partials only, summed by tsmttsm_reduce_kernel
 };
 
 constexpr int kWaves = 4;
@@ -137,6 +142,7 @@ __global__ void __launch_bounds__(256)
   const int PS = a.outM * a.outN;
   T* mine = a.part1 + ((int64_t)tile * a.G + kp) * PS;
   Core::wg_partial(acc, red, mine, a.M, a.N, a.outM, a.outN);
+  if (a.two_pass) return;  // the partials are summed by the next kernel
 
   // ---- grid reduction: last arrival per group, then last group ----------
   // Arrival publishes this workgroup's partial with a release fence at agent
@@ -184,6 +190,81 @@ __global__ void __launch_bounds__(256)
               red, LDSN, true);
   final_epilogue<T>(a, red + LDSN / 2, m0, n0, Mloc, Nloc);
   if (threadIdx.x == 0) *cnt2 = 0u;
+}
+
+// Second pass: sums the G partials of each tile entry in a fixed order and
+// applies the epilogue.  An item is one output: one entry of the real view,
+// or the four entries (2m, 2n), (2m + 1, 2n), (2m, 2n + 1), (2m + 1, 2n + 1)
+// of a complex output.  Workgroup b handles E consecutive items of tile
+// b / nsl; thread t takes item t % E and the partials j, j + TPE, ... with
+// j = t / E and TPE = 256 / E, summed in increasing order, then the TPE
+// sums of an item are added pairwise through LDS.
+template <typename T>
+__global__ void __launch_bounds__(256) tsmttsm_reduce_kernel(TsArgs<T> a, int E) {
+  __shared__ T red[4 * 256];
+  const int PS = a.outM * a.outN;
+  const int oMi = a.cplx ? a.outM / 2 : a.outM;
+  const int nitems = a.cplx ? PS / 4 : PS;
+  const int nsl = (nitems + E - 1) / E;
+  const int tile = blockIdx.x / nsl;
+  const int sl = blockIdx.x % nsl;
+  const int TPE = 256 / E;
+  const int t = threadIdx.x;
+  const int il = t % E, j = t / E;
+  const int item = sl * E + il;
+  const bool live = item < nitems;
+  const int mi = item % oMi, ni = item / oMi;
+  const int nent = a.cplx ? 4 : 1;
+  int ent[4];
+  if (a.cplx) {
+    const int e00 = (2 * ni) * a.outM + 2 * mi;
+    ent[0] = e00;              // R(2m, 2n)
+    ent[1] = e00 + a.outM + 1; // R(2m + 1, 2n + 1)
+    ent[2] = e00 + a.outM;     // R(2m, 2n + 1)
+    ent[3] = e00 + 1;          // R(2m + 1, 2n)
+  } else {
+    ent[0] = item;
+    ent[1] = ent[2] = ent[3] = item;
+  }
+  const T* src = a.part1 + (int64_t)tile * a.G * PS;
+  for (int k = 0; k < nent; k++) {
+    T sum = T(0);
+    if (live) {
+      const T* p = src + ent[k];
+      int q = j;
+      for (; q + 3 * TPE < a.G; q += 4 * TPE) {
+        const T v0 = p[(int64_t)q * PS];
+        const T v1 = p[(int64_t)(q + TPE) * PS];
+        const T v2 = p[(int64_t)(q + 2 * TPE) * PS];
+        const T v3 = p[(int64_t)(q + 3 * TPE) * PS];
+        sum += v0;
+        sum += v1;
+        sum += v2;
+        sum += v3;
+      }
+      for (; q < a.G; q += TPE) sum += p[(int64_t)q * PS];
+    }
+    red[k * 256 + t] = sum;
+  }
+  __syncthreads();
+  for (int h = TPE / 2; h >= 1; h >>= 1) {
+    if (j < h)
+      for (int k = 0; k < nent; k++) red[k * 256 + t] += red[k * 256 + t + h * E];
+    __syncthreads();
+  }
+  if (j != 0 || !live) return;
+  const int tm = tile % a.tilesM, tn = tile / a.tilesM;
+  const int m0 = tm * a.outM, n0 = tn * a.outN;
+  const int Mloc = min(a.M - m0, a.outM), Nloc = min(a.N - n0, a.outN);
+  if (a.cplx) {
+    if (2 * mi >= Mloc || 2 * ni >= Nloc) return;
+    T* c = a.C + 2 * ((int64_t)(n0 / 2 + ni) * a.ldc + (m0 / 2 + mi));
+    complex_epilogue(a, red[il], red[256 + il], red[512 + il], red[768 + il], c);
+  } else {
+    if (mi >= Mloc || ni >= Nloc) return;
+    T* c = a.C + (int64_t)(n0 + ni) * a.ldc + (m0 + mi);
+    *c = epilogue(red[il], a.alpha, a.beta, c);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -357,11 +438,26 @@ skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
   // on the diagonal; off-diagonal tiles of a large Gram read two column blocks
   const bool self = (A == B) && (lda == ldb) && (M == N) && ntiles == 1;
   const int blocks = p.G * ntiles;
+  a.two_pass = p.G > 1 ? 1 : 0;
   bool ok = self ? dispatch<T, true>(p, a, blocks, stream)
                  : dispatch<T, false>(p, a, blocks, stream);
   if (!ok) return fail(SKINNY_UNSUPPORTED, "tsmttsm: no kernel for this shape");
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "tsmttsm launch");
+  if (a.two_pass) {
+    // items per workgroup: at most about 8 partials per thread, E a power of
+    // two <= 64 and no larger than the item count needs
+    const int PS = p.outM * p.outN;
+    const int nitems = cplx ? PS / 4 : PS;
+    int E = 64;
+    while (E > 1 && (int64_t)p.G * E > 256 * 8) E >>= 1;
+    while (E > 1 && E / 2 >= nitems) E >>= 1;
+    const int nsl = (nitems + E - 1) / E;
+    hipLaunchKernelGGL(tsmttsm_reduce_kernel<T>, dim3(ntiles * nsl), dim3(256), 0,
+                       stream, a, E);
+    e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "tsmttsm reduce launch");
+  }
   return SKINNY_OK;
 }
 
