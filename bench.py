@@ -375,7 +375,7 @@ def main() -> int:
         cpu = cpu_baseline(1 << 22, args.cpu_seconds, min(16, os.cpu_count() or 1))
 
     out = {
-        "metric": "TSMTTSM/TSMM GB/s (% HBM roofline) fp64 M=N 1-64",
+        "metric": "TSMTTSM/TSMM GB/s (% HBM roofline) fp64 M=N 1\u201364; Ozaki vs fp64 at M=N=128",
         "value": round(value, 1),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -409,6 +409,15 @@ def main() -> int:
         "cpu_baseline": cpu,
         "configs": extras,
     }
+    if extras and "c3" in extras:
+        # the metric's second half: Ozaki (3 limbs) against the FP64 kernel
+        # at M = N = 128, K = 2^24 on the cancelling C3 columns
+        c3 = extras["c3"]
+        out["ozaki_vs_fp64"] = {
+            case: {"speedup": c3[case]["ozaki3_speedup"], "ozaki3_ms": c3[case]["ozaki3_ms"],
+                   "fp64_ms": c3[case]["fp64_ms"], "rel_err": c3[case]["ozaki3_rel_err"],
+                   "tolerance_8Keps": c3["tolerance_8Keps"]}
+            for case in ("gram", "general")}
     print(json.dumps(out))
     sys.stdout.flush()
     return 0

@@ -300,7 +300,9 @@ skinny_status_t skinny_dorth_error(skinny_handle_t h, int64_t M, int64_t K,
 /* ---- microbenchmarks: this chip's own ceilings ------------------------ */
 /* HBM stream over arrays of `bytes_per_array` (allocated internally, far
  * above the Infinity Cache when >= 1 GiB); kind 0 read, 1 copy, 2 triad,
- * 3 write, 4 update (a += s b); best of `reps`, in GB/s of array traffic.
+ * 3 write, 4 update (a += s b); best of `reps` and of eight launch shapes
+ * (1, 2, 4 or 8 workgroups per CU, 1 or 4 accesses in flight per thread),
+ * in GB/s of array traffic.
  * Replaces microbenches/membw.cu:19-164 (copy/scale/update/sum/triad). */
 skinny_status_t skinny_mb_stream(int kind, size_t bytes_per_array, int reps,
                                  double* gbs, skinny_stream_t stream);

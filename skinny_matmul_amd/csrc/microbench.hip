@@ -6,6 +6,7 @@
 // rewritten for gfx950: 16-byte accesses, working sets far above the 256 MB
 // Infinity Cache, launch shapes from the CU count, and timing with HIP events
 // on the caller's stream.
+#include <algorithm>
 #include <vector>
 
 #include "internal.hpp"
@@ -16,27 +17,43 @@ namespace {
 typedef double d2 __attribute__((ext_vector_type(2)));
 
 // kind 0: read (sum), 1: copy, 2: triad a = b + s c, 3: write, 4: update a += s b
-template <int KIND>
+// U accesses of 16 bytes per array and thread in flight (grid-stride).
+template <int KIND, int U>
 __global__ void __launch_bounds__(256)
     stream_kernel(d2* __restrict__ a, const d2* __restrict__ b,
                   const d2* __restrict__ c, int64_t n2, double s,
                   double* __restrict__ sink) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   d2 acc = {0.0, 0.0};
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2;
-       i += stride) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  auto one = [&](int64_t j, d2 vb, d2 vc, d2 va) {
     if (KIND == 0) {
-      acc += b[i];
+      acc += vb;
     } else if (KIND == 1) {
-      a[i] = b[i];
+      a[j] = vb;
     } else if (KIND == 2) {
-      a[i] = b[i] + s * c[i];
+      a[j] = vb + s * vc;
     } else if (KIND == 3) {
-      a[i] = d2{s, s};
+      a[j] = d2{s, s};
     } else {
-      a[i] = a[i] + s * b[i];
+      a[j] = va + s * vb;
     }
+  };
+  for (; i + (U - 1) * stride < n2; i += U * stride) {
+    d2 vb[U], vc[U], va[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t j = i + u * stride;
+      vb[u] = (KIND != 3) ? b[j] : d2{0, 0};
+      vc[u] = (KIND == 2) ? c[j] : d2{0, 0};
+      va[u] = (KIND == 4) ? a[j] : d2{0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) one(i + u * stride, vb[u], vc[u], va[u]);
   }
+  for (; i < n2; i += stride)
+    one(i, (KIND != 3) ? b[i] : d2{0, 0}, (KIND == 2) ? c[i] : d2{0, 0},
+        (KIND == 4) ? a[i] : d2{0, 0});
   // keep the read-only loop alive without a store per element
   if (KIND == 0 && acc.x + acc.y == 12345.678) sink[threadIdx.x] = acc.x;
 }
@@ -154,20 +171,37 @@ skinny_status_t skinny_mb_stream(int kind, size_t bytes_per_array, int reps,
     (void)hipFree(sink);
     return hip_fail(e, "skinny_mb_stream alloc");
   }
-  const int blocks = 8 * h->cu_count;
-  auto launch = [&]() {
-    d2* A = (d2*)a;
-    const d2* B = (const d2*)b;
-    const d2* Cc = (const d2*)c;
-    switch (kind) {
-      case 0: hipLaunchKernelGGL(stream_kernel<0>, dim3(blocks), dim3(256), 0, st, A, B, Cc, n2, 1.5, (double*)sink); break;
-      case 1: hipLaunchKernelGGL(stream_kernel<1>, dim3(blocks), dim3(256), 0, st, A, B, Cc, n2, 1.5, (double*)sink); break;
-      case 2: hipLaunchKernelGGL(stream_kernel<2>, dim3(blocks), dim3(256), 0, st, A, B, Cc, n2, 1.5, (double*)sink); break;
-      case 3: hipLaunchKernelGGL(stream_kernel<3>, dim3(blocks), dim3(256), 0, st, A, B, Cc, n2, 1.5, (double*)sink); break;
-      default: hipLaunchKernelGGL(stream_kernel<4>, dim3(blocks), dim3(256), 0, st, A, B, Cc, n2, 1.5, (double*)sink); break;
+  // best over launch shapes: 1, 2, 4 or 8 workgroups per CU and 1 or 4
+  // accesses in flight per thread (the best shape differs by kind: writes
+  // and copies peak with few workgroups, reads with many)
+  float ms = 1e30f;
+  for (int per : {1, 2, 4, 8}) {
+    for (int unroll : {1, 4}) {
+      const int blocks = per * h->cu_count;
+      auto launch = [&]() {
+        d2* A = (d2*)a;
+        const d2* B = (const d2*)b;
+        const d2* Cc = (const d2*)c;
+        double* sk = (double*)sink;
+#define SK_MB(k)                                                                              \
+  if (unroll == 1)                                                                            \
+    hipLaunchKernelGGL((stream_kernel<k, 1>), dim3(blocks), dim3(256), 0, st, A, B, Cc, n2,   \
+                       1.5, sk);                                                              \
+  else                                                                                        \
+    hipLaunchKernelGGL((stream_kernel<k, 4>), dim3(blocks), dim3(256), 0, st, A, B, Cc, n2,   \
+                       1.5, sk);
+        switch (kind) {
+          case 0: SK_MB(0) break;
+          case 1: SK_MB(1) break;
+          case 2: SK_MB(2) break;
+          case 3: SK_MB(3) break;
+          default: SK_MB(4) break;
+        }
+#undef SK_MB
+      };
+      ms = std::min(ms, time_ms(launch, st, reps));
     }
-  };
-  float ms = time_ms(launch, st, reps);
+  }
   e = hipGetLastError();
   const double arrays[5] = {1, 2, 3, 1, 3};  // bytes moved per array size
   *gbs = arrays[kind] * (double)nb / (ms * 1e-3) / 1e9;

@@ -445,12 +445,12 @@ skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "tsmttsm launch");
   if (a.two_pass) {
-    // items per workgroup: at most about 8 partials per thread, E a power of
-    // two <= 64 and no larger than the item count needs
+    // items per workgroup: 16 (a 128-byte line of one partial per 16 lanes,
+    // and TPE = 16 threads per item) or 64 when G <= 32 (at most 8 partials
+    // per thread); no more than the item count needs
     const int PS = p.outM * p.outN;
     const int nitems = cplx ? PS / 4 : PS;
-    int E = 64;
-    while (E > 1 && (int64_t)p.G * E > 256 * 8) E >>= 1;
+    int E = p.G <= 32 ? 64 : 16;
     while (E > 1 && E / 2 >= nitems) E >>= 1;
     const int nsl = (nitems + E - 1) / E;
     hipLaunchKernelGGL(tsmttsm_reduce_kernel<T>, dim3(ntiles * nsl), dim3(256), 0,

@@ -2,7 +2,7 @@
 
     python -m skinny_matmul_amd.profile_run --op tsmm --M 64 --logK 26 --reps 3
     python -m skinny_matmul_amd.profile_run --op copy --logK 26 --reps 3
-    python -m skinny_matmul_amd.profile_run --op ozaki --M 128 --logK 24 --S 4
+    python -m skinny_matmul_amd.profile_run --op ozaki --M 128 --logK 24 --S 3 [--gram]
     python -m skinny_matmul_amd.profile_run --op hr --M 64 --logK 22
 
 ``--op copy`` runs the HBM copy microbenchmark over arrays of the same byte
@@ -22,6 +22,7 @@ def main() -> int:
     ap.add_argument("--M", type=int, default=64)
     ap.add_argument("--logK", type=int, default=26)
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--gram", action="store_true", help="TSMTTSM/Ozaki/HR: B is A")
     a = ap.parse_args()
 
     import torch
@@ -41,10 +42,13 @@ def main() -> int:
     Bs = torch.empty(M * M, dtype=torch.float64, device="cuda")
     sk.fill_uniform(Bs, M * M, 1, 2)
     Cs = torch.zeros(M * M, dtype=torch.float64, device="cuda")
+    Cw = B  # TSMM output (K x M)
     torch.cuda.synchronize()
+    if a.gram:
+        B = A
     for _ in range(a.reps):
         if a.op == "tsmm":
-            ok = sk.dtsmm(0, M, M, K, A, M, 1.0, Bs, M, 0.0, B, M)
+            ok = sk.dtsmm(0, M, M, K, A, M, 1.0, Bs, M, 0.0, Cw, M)
         elif a.op == "ozaki":
             ok = sk.dtsmttsm_ozaki(0, M, M, K, A, M, 1.0, B, M, 0.0, Cs, M, slices=a.S)
         elif a.op == "hr":
