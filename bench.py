@@ -74,8 +74,8 @@ def _time_ms(torch, fn, reps=3):
 
 def extra_configs(sk, torch, logK: int) -> dict:
     """The other benchmark configurations (not part of `value`):
-    C2 complex-double TSMM and in-place FP64 TSMM over the same widths at
-    K = 2^logK; C3 Ozaki-INT8 against FP64 TSMTTSM at M = N = 128, K = 2^24,
+    C2 complex-double TSMM (out of place and in place) and in-place FP64
+    TSMM over the same widths at K = 2^logK; C3 Ozaki-INT8 against FP64 TSMTTSM at M = N = 128, K = 2^24,
     for the Gram product and a general one, errors against the
     double-double (HR) result; C4 one persistent batched
     launch over 512 uneven products; C5 two SVQB sweeps over 64 blocks
@@ -86,8 +86,8 @@ def extra_configs(sk, torch, logK: int) -> dict:
     out = {}
     K = 1 << logK
 
-    # ---- C2: DC TSMM and in-place DR TSMM, GB/s per width ----------------
-    dc, ip = {}, {}
+    # ---- C2: DC TSMM (out of place and in place) and in-place DR TSMM ----
+    dc, dci, ip = {}, {}, {}
     for M in WIDTHS:
         log(f"C2 width {M}")
         A = torch.empty(2 * K * M, dtype=torch.float64, device=dev)
@@ -95,8 +95,11 @@ def extra_configs(sk, torch, logK: int) -> dict:
         Bc = torch.empty(2 * M * M, dtype=torch.float64, device=dev)
         sk.fill_uniform(A, A.numel(), 3, 0)
         sk.fill_uniform(Bc, Bc.numel(), 3, 1)
+        nb = op_bytes("tsmm", M, M, K) * 2
         ms = _time_ms(torch, lambda: sk.ztsmm(0, M, M, K, A, M, 1.0, Bc, M, 0.0, Cc, M))
-        dc[str(M)] = round(op_bytes("tsmm", M, M, K) * 2 / (ms * 1e-3) / 1e9, 1)
+        dc[str(M)] = round(nb / (ms * 1e-3) / 1e9, 1)
+        ms = _time_ms(torch, lambda: sk.ztsmm(0, M, M, K, A, M, 1.0, Bc, M, 0.0, A, M))
+        dci[str(M)] = round(nb / (ms * 1e-3) / 1e9, 1)
         del Cc
         X = A[: K * M]
         Br = Bc[: M * M]
@@ -105,6 +108,7 @@ def extra_configs(sk, torch, logK: int) -> dict:
         del A, Bc, X, Br
         torch.cuda.empty_cache()
     out["c2_dc_tsmm_GBs"] = dc
+    out["c2_dc_inplace_tsmm_GBs"] = dci
     out["c2_inplace_tsmm_GBs"] = ip
 
     # ---- C3: Ozaki vs FP64 vs dd, M = N = 128, K = 2^24 ------------------

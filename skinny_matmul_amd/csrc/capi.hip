@@ -121,6 +121,12 @@ skinny_status_t tsmm_cplx(const char* fn, skinny_handle_t h, size_t blockCount,
   }
   h = resolve(h);
   if (!h) return no_handle(fn);
+  // real beta, 2..32 complex columns: the real matrix-core kernel on the
+  // interleaved view (alpha folded into the operand)
+  if (beta.y == 0 && M >= 2 && M <= 32 && N <= 32)
+    return launch_tsmm_cview<T>(h, blockCount, M, N, K, (const T*)A, lda, alpha.x,
+                                alpha.y, (const T*)B, ldb, beta.x, (T*)C, ldc,
+                                (hipStream_t)stream);
   return launch_tsmm_lds<T>(h, blockCount, 2 * M, 2 * N, K, (const T*)A,
                             2 * lda, alpha.x, alpha.y, (const T*)B, ldb,
                             beta.x, beta.y, (T*)C, 2 * ldc, true,

@@ -124,6 +124,15 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
                                  int64_t lda, T alpha, const T* B, int64_t ldb,
                                  T beta, T* C, int64_t ldc, hipStream_t stream);
 
+// Complex TSMM with a real beta on the real matrix-core kernel (Mc, Nc
+// complex widths, 2 <= Mc, Nc <= 32; lda, ldb, ldc in complex units).
+template <typename T>
+skinny_status_t launch_tsmm_cview(skinny_handle_t h, size_t blockCount,
+                                  int64_t Mc, int64_t Nc, int64_t K, const T* A,
+                                  int64_t lda, T alpha_r, T alpha_i, const T* B,
+                                  int64_t ldb, T beta, T* C, int64_t ldc,
+                                  hipStream_t stream);
+
 // HR: double in, PseudoQuad accumulation; dd_out writes (s, t) pairs to C
 skinny_status_t launch_htsmttsm(skinny_handle_t h, size_t blockCount,
                                 int64_t M, int64_t N, int64_t K,

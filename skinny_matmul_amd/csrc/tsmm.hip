@@ -33,6 +33,13 @@
 //  * A wave owns whole 16-row blocks: every A load of a block completes before
 //    the MFMAs that consume it, and C of the block is stored after them, so C
 //    may alias A.
+//
+// Complex products with a real beta run on the same matrix-core kernel
+// through the interleaved real view: A (K x M complex) is K x 2M real, C is
+// K x 2N real, and C = A_r Bh + beta C with the 2M x 2N real matrix
+//   Bh(2m, 2n) = Re b',  Bh(2m + 1, 2n) = -Im b',
+//   Bh(2m, 2n + 1) = Im b',  Bh(2m + 1, 2n + 1) = Re b',   b' = alpha B(m, n),
+// formed from the complex B while the operands are loaded into registers.
 #include "internal.hpp"
 #include "mfma.hpp"
 
@@ -46,7 +53,27 @@ struct TmArgs {
   int64_t K, lda, ldb, ldc;
   int M, N;
   T alpha, beta;
+  // complex view (tsmm_mfma_kernel only): B is complex M/2 x N/2 with ldb in
+  // complex units, alpha = 1 and (cre, cim) the complex alpha folded into Bh
+  int cplxB;
+  T cre, cim;
 };
+
+// Element (m, n) of the real B operand: B itself, or Bh of the complex view.
+template <typename T>
+__device__ __forceinline__ T tm_bval(const TmArgs<T>& a, int m, int n) {
+  if (!a.cplxB) return a.B[(int64_t)n * a.ldb + m];
+  const T* b = a.B + 2 * ((int64_t)(n >> 1) * a.ldb + (m >> 1));
+  const T br = b[0], bi = b[1];
+  T pr, pi;
+  {
+#pragma clang fp contract(off)
+    pr = a.cre * br - a.cim * bi;
+    pi = a.cre * bi + a.cim * br;
+  }
+  if ((m & 1) == 0) return (n & 1) == 0 ? pr : pi;
+  return (n & 1) == 0 ? -pi : pr;
+}
 
 template <typename T>
 __device__ __forceinline__ T fma_t(T a, T b, T c) {
@@ -205,7 +232,7 @@ __global__ void __launch_bounds__(256) tsmm_mfma_kernel(TmArgs<T> a) {
 #pragma unroll
     for (int u = 0; u < NT; u++) {
       const int m = lg * S + s, n = 16 * u + li;
-      bop[s][u] = (m < M && n < N) ? a.B[(int64_t)n * a.ldb + m] : T(0);
+      bop[s][u] = (m < M && n < N) ? tm_bval(a, m, n) : T(0);
     }
   const int mbeg = lg * S;
 
@@ -349,6 +376,9 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
   a.N = (int)N;
   a.alpha = alpha;
   a.beta = beta;
+  a.cplxB = 0;
+  a.cre = T(1);
+  a.cim = T(0);
   int blocks = blockCount > 0 ? (int)blockCount
                               : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T));
   const bool beta0 = (beta == T(0));
@@ -388,6 +418,57 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
   if (e != hipSuccess) return hip_fail(e, "tsmm launch");
   return SKINNY_OK;
 }
+
+template <typename T>
+skinny_status_t launch_tsmm_cview(skinny_handle_t h, size_t blockCount,
+                                  int64_t Mc, int64_t Nc, int64_t K, const T* A,
+                                  int64_t lda, T alpha_r, T alpha_i, const T* B,
+                                  int64_t ldb, T beta, T* C, int64_t ldc,
+                                  hipStream_t stream) {
+  const int64_t M = 2 * Mc, N = 2 * Nc;
+  if (Mc < 2 || M > 64 || N > 64)
+    return fail(SKINNY_UNSUPPORTED, "tsmm complex view: 2 <= M <= 32, N <= 32");
+  TmArgs<T> a;
+  a.A = A;
+  a.B = B;
+  a.C = C;
+  a.K = K;
+  a.lda = 2 * lda;
+  a.ldb = ldb;
+  a.ldc = 2 * ldc;
+  a.M = (int)M;
+  a.N = (int)N;
+  a.alpha = T(1);
+  a.beta = beta;
+  a.cplxB = 1;
+  a.cre = alpha_r;
+  a.cim = alpha_i;
+  int blocks = blockCount > 0 ? (int)blockCount
+                              : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T));
+  const bool beta0 = (beta == T(0));
+  const int S = (int)((M + 3) / 4);
+  const int Sc = S <= 1 ? 1 : S <= 2 ? 2 : S <= 4 ? 4 : S <= 8 ? 8 : 16;
+  const int nt = (int)((N + 15) / 16);
+  const bool vecm = sizeof(T) == 8 && (M % 4 == 0) && (S % 2 == 0) &&
+                    (Sc == S) && (a.lda % 2 == 0) && ((uintptr_t)A % 16 == 0);
+  switch (Sc) {
+    case 1: launch_mfma_s<T, 1>(a, blocks, nt, beta0, false, stream); break;
+    case 2: launch_mfma_s<T, 2>(a, blocks, nt, beta0, vecm, stream); break;
+    case 4: launch_mfma_s<T, 4>(a, blocks, nt, beta0, vecm, stream); break;
+    case 8: launch_mfma_s<T, 8>(a, blocks, nt, beta0, vecm, stream); break;
+    default: launch_mfma_s<T, 16>(a, blocks, nt, beta0, vecm, stream); break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "tsmm (complex view) launch");
+  return SKINNY_OK;
+}
+
+template skinny_status_t launch_tsmm_cview<double>(
+    skinny_handle_t, size_t, int64_t, int64_t, int64_t, const double*, int64_t,
+    double, double, const double*, int64_t, double, double*, int64_t, hipStream_t);
+template skinny_status_t launch_tsmm_cview<float>(
+    skinny_handle_t, size_t, int64_t, int64_t, int64_t, const float*, int64_t,
+    float, float, const float*, int64_t, float, float*, int64_t, hipStream_t);
 
 template skinny_status_t launch_tsmm_real<double>(
     skinny_handle_t, size_t, int64_t, int64_t, int64_t, const double*, int64_t,
