@@ -123,8 +123,10 @@ __device__ void run_ts_item(const BatchOpDev& o, const BatchItemDev& it,
 }
 
 // ---- TSMM: rows [r0, r1) of one op by the four waves --------------------
-// classes: S in {1, 2, 4, 8, 16} x NT in {1, 2, 4}  ->  3 * iS + iN
+// classes: S in {1, 2, 4, 8, 16} x NT in {1, 2, 4}  ->  3 * iS + iN;
+// 15, 16: M, N <= 2 (M = 1, 2), one thread per row
 static int tm_class(int64_t M, int64_t N) {
+  if (M <= 2 && N <= 2) return M <= 1 ? 15 : 16;
   const int64_t S = (M + 3) / 4;
   const int iS = S <= 1 ? 0 : S <= 2 ? 1 : S <= 4 ? 2 : S <= 8 ? 3 : 4;
   const int64_t nt = (N + 15) / 16;
@@ -149,33 +151,86 @@ __device__ void tm_item(const BatchOpDev& o, const BatchItemDev& it) {
     }
   const int mbeg = lg * S;
   const bool beta0 = o.beta == 0.0;
-  // rows [r0, r1) in 16-row blocks; r0 is a multiple of 16
-  const int64_t b_end = (it.r1 - it.r0 + 15) >> 4;
-  for (int64_t b = wave; b < b_end; b += 4) {
-    double av[S];
-    const int64_t row = it.r0 + b * 16 + li;
-    const bool ok = row < it.r1;
-    const double* ar = o.A + row * o.lda + mbeg;
+  // rows [r0, r1) in 16-row blocks (r0 is a multiple of 16); a wave takes RB
+  // consecutive blocks per step, all their loads in flight before the first
+  // MFMA (about 8 loads per lane); wave w owns blocks w RB .. w RB + RB - 1
+  // of every group of 4 RB, so in place needs no barrier
+  constexpr int RB = S >= 16 ? 1 : (S >= 8 ? 2 : (S >= 4 ? 4 : 8));
+  const int64_t nblk = (it.r1 - it.r0 + 15) >> 4;
+  for (int64_t b0 = (int64_t)wave * RB; b0 < nblk; b0 += 4 * RB) {
+    double av[RB][S];
 #pragma unroll
-    for (int s = 0; s < S; s++) av[s] = (ok && mbeg + s < M) ? ar[s] : 0.0;
-    V4 acc[NT];
+    for (int r = 0; r < RB; r++) {
+      const int64_t row = it.r0 + (b0 + r) * 16 + li;
+      const bool ok = row < it.r1;
+      const double* ar = o.A + row * o.lda + mbeg;
 #pragma unroll
-    for (int u = 0; u < NT; u++) acc[u] = V4{0, 0, 0, 0};
+      for (int s = 0; s < S; s++) av[r][s] = (ok && mbeg + s < M) ? ar[s] : 0.0;
+    }
 #pragma unroll
-    for (int s = 0; s < S; s++)
+    for (int r = 0; r < RB; r++) {
+      V4 acc[NT];
 #pragma unroll
-      for (int u = 0; u < NT; u++) acc[u] = Mfma<double>::mac(av[s], bop[s][u], acc[u]);
+      for (int u = 0; u < NT; u++) acc[u] = V4{0, 0, 0, 0};
 #pragma unroll
-    for (int u = 0; u < NT; u++) {
-      const int col = 16 * u + li;
+      for (int s = 0; s < S; s++)
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int64_t r = it.r0 + b * 16 + Mfma<double>::drow(lg, v);
-        if (r < it.r1 && col < N) {
+        for (int u = 0; u < NT; u++) acc[u] = Mfma<double>::mac(av[r][s], bop[s][u], acc[u]);
+#pragma unroll
+      for (int u = 0; u < NT; u++) {
+        const int col = 16 * u + li;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int64_t rr = it.r0 + (b0 + r) * 16 + Mfma<double>::drow(lg, v);
+          if (rr < it.r1 && col < N) {
 #pragma clang fp contract(off)
-          double* c = o.C + r * o.ldc + col;
-          const double s = acc[u][v];
-          *c = beta0 ? o.alpha * s : o.beta * *c + o.alpha * s;
+            double* c = o.C + rr * o.ldc + col;
+            const double sres = acc[u][v];
+            *c = beta0 ? o.alpha * sres : o.beta * *c + o.alpha * sres;
+          }
+        }
+      }
+    }
+  }
+}
+
+// M, N <= 2: a thread owns whole rows (R rows in flight), B in registers;
+// every load of a row precedes its stores, so C may alias A
+template <int MC>
+__device__ void tm_rows_item(const BatchOpDev& o, const BatchItemDev& it) {
+  constexpr int R = 4;
+  const int M = o.M, N = o.N;
+  const bool beta0 = o.beta == 0.0;
+  double b[2][2];
+#pragma unroll
+  for (int m = 0; m < 2; m++)
+#pragma unroll
+    for (int n = 0; n < 2; n++)
+      b[m][n] = (m < M && n < N) ? o.B[(int64_t)n * o.ldb + m] : 0.0;
+  const int64_t nrows = it.r1 - it.r0;
+  for (int64_t base = threadIdx.x; base < nrows; base += 256 * R) {
+    double av[R][MC];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int64_t q = base + 256 * r;
+      const double* ar = o.A + (it.r0 + q) * o.lda;
+#pragma unroll
+      for (int m = 0; m < MC; m++) av[r][m] = (q < nrows && m < M) ? ar[m] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int64_t q = base + 256 * r;
+      if (q >= nrows) continue;
+      double* cr = o.C + (it.r0 + q) * o.ldc;
+#pragma unroll
+      for (int n = 0; n < 2; n++) {
+        if (n >= N) continue;
+        double sres = 0.0;
+#pragma unroll
+        for (int m = 0; m < MC; m++) sres = __builtin_fma(av[r][m], b[m][n], sres);
+        {
+#pragma clang fp contract(off)
+          cr[n] = beta0 ? o.alpha * sres : o.beta * cr[n] + o.alpha * sres;
         }
       }
     }
@@ -184,6 +239,8 @@ __device__ void tm_item(const BatchOpDev& o, const BatchItemDev& it) {
 
 __device__ void run_tm_item(const BatchOpDev& o, const BatchItemDev& it) {
   switch (o.cls) {
+    case 15: tm_rows_item<1>(o, it); break;
+    case 16: tm_rows_item<2>(o, it); break;
     case 0: tm_item<1, 1>(o, it); break;
     case 1: tm_item<1, 2>(o, it); break;
     case 2: tm_item<1, 4>(o, it); break;

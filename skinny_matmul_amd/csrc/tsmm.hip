@@ -20,7 +20,8 @@
 //    {1, 2, 4, 8, 16, 32, 64} and masked with uniform branches; N is a runtime
 //    loop unrolled by two, so one instantiation serves a range of widths.
 //
-// For 3 <= M <= 64 and N <= 64 the matrix cores take over (tsmm_mfma_kernel):
+// For 3 <= M <= 64 and N <= 64, and for M <= 2 with 4 <= N <= 64, the matrix
+// cores take over (tsmm_mfma_kernel):
 //  * One v_mfma_f64_16x16x4f64 produces a 16-row x 16-column tile of C from
 //    four columns of A: lane (i, g) = (lane % 16, lane / 16) supplies
 //    A(row0 + i, m_g) and B(m_g, n0 + i).
@@ -325,10 +326,15 @@ int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
                       int elem_bytes) {
   TuneEntry te;
   if (elem_bytes == 8 && tune_lookup(h, 1, 0, M, N, K, &te)) return te.G;
-  // measured on MI355X at K = 2^26 (fp64): the one-row-per-thread kernel for
-  // M = 1 wants four workgroups per CU, everything else two
+  // measured on MI355X at K = 2^26 (fp64): the one-row-per-thread kernel
+  // (M, N <= 2) wants four workgroups per CU for M = 1 and two for M = 2; the
+  // matrix-core kernel wants eight when A is at most 4 columns wide (few
+  // loads per lane) and two otherwise
   int64_t want = (K + 255) / 256;
-  int64_t cap = (M <= 1 ? 4LL : 2LL) * h->cu_count;
+  int64_t per_cu;
+  if (M <= 2 && N <= 2) per_cu = M <= 1 ? 4 : 2;
+  else per_cu = M <= 4 ? 8 : 2;
+  int64_t cap = per_cu * h->cu_count;
   if (want > cap) want = cap;
   if (want < 1) want = 1;
   return (int)want;
@@ -382,7 +388,9 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
   int blocks = blockCount > 0 ? (int)blockCount
                               : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T));
   const bool beta0 = (beta == T(0));
-  if (M >= 3 && N <= 64) {
+  // the matrix-core kernel from 3 columns of A, and for M <= 2 once C is 4
+  // or more columns wide (its 16-column stores then beat whole-row threads)
+  if ((M >= 3 || N >= 4) && N <= 64) {
     const int S = (int)((M + 3) / 4);
     const int Sc = S <= 1 ? 1 : S <= 2 ? 2 : S <= 4 ? 4 : S <= 8 ? 8 : 16;
     const int nt = (int)((N + 15) / 16);
