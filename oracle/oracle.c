@@ -761,8 +761,8 @@ void orc_gram_error(int64_t M, int64_t K, const double* X, int64_t ldx,
 /* Ozaki split TSMTTSM, restated for blockCount = 1 (one workgroup per       */
 /* 128 x 128 tile walking all row chunks in order): the bit-exact twin of    */
 /* the device path.  Chunks are 32 rows, or 64 rows for the diagonal tiles  */
-/* of a Gram product (A == B, lda == ldb, M == N, lda <= 2^21), where both  */
-/* operands share one scale per column.  Algorithm: Ozaki, Ogita, Oishi, Rump, "Error-free       */
+/* of a Gram product with 3 limbs (A == B, lda == ldb, M == N,              */
+/* lda <= 2^21), where both operands share one scale per column.  Algorithm: Ozaki, Ogita, Oishi, Rump, "Error-free       */
 /* transformations of matrix multiplication by using fast routines of matrix */
 /* multiplication and its applications", Numer. Algorithms 59 (2012), with   */
 /* 7-bit limbs of the rounded integer q = rint(|x| 2^(7S - E)), levels up to  */
@@ -813,7 +813,7 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
   const int gram = A == B && lda == ldb && M == N && lda <= ((int64_t)1 << 21);
   for (int64_t m0 = 0; m0 < M; m0 += OZ_TILE)
     for (int64_t n0 = 0; n0 < N; n0 += OZ_TILE) {
-      const int R = (gram && m0 == n0) ? OZ_ROWS_SELF : OZ_ROWS;
+      const int R = (gram && m0 == n0 && S == 3) ? OZ_ROWS_SELF : OZ_ROWS;
       int erun[2][OZ_TILE], ecur[2][OZ_TILE];
       for (int c = 0; c < OZ_TILE; c++) erun[0][c] = erun[1][c] = OZ_EMIN;
       memset(acc, 0, sizeof(int64_t) * OZ_TILE * OZ_TILE * L);

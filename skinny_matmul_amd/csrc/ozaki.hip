@@ -33,17 +33,18 @@
 //    same two-level last-arrival reduction as the FP64 kernel.  With
 //    blockCount == 1 the whole computation is one fixed sequence, restated
 //    bit for bit on the host by the oracle.
-//  * Per chunk the loop is software-pipelined: the next chunk's loads are in
+//  * Per chunk the loop is software-pipelined: the next chunks' loads are in
 //    flight while this chunk is split (VALU, packed 4 rows to a word) into
 //    one LDS buffer and the previous chunk's limbs in the other buffer are
-//    multiplied, the two interleaved instruction by instruction.
-//  * Gram products (A == B, lda == ldb, M == N) read A once: the diagonal
-//    tiles run with 64-row chunks whose two 32-row halves take the places of
-//    A and B in the staging (one scale per column, shared by both operands),
-//    and of the off-diagonal tiles only those above the diagonal are
-//    computed, each written to C twice (C(m, n) and C(n, m)).  The loads in
-//    flight per workgroup stay the same, so a Gram product streams half the
-//    bytes of a general one in about half the time.
+//    multiplied, the two interleaved instruction by instruction.  With 3
+//    limbs two chunks are in flight (prefetch distance 2), with 4 or 5 one.
+//  * Gram products (A == B, lda == ldb, M == N) compute only the tiles on
+//    and above the diagonal, writing each off-diagonal tile to C twice
+//    (C(m, n) and C(n, m)).  With 3 limbs they also read A once: the
+//    diagonal tiles run with 64-row chunks whose two 32-row halves take the
+//    places of A and B in the staging (one scale per column, shared by both
+//    operands).  The loads in flight per workgroup stay the same, so such a
+//    tile streams half the bytes of a general one in about half the time.
 #include <cmath>
 #include <utility>
 
@@ -352,22 +353,29 @@ __global__ void __launch_bounds__(kOzThreads)
   __syncthreads();
 
   const int64_t nch = (a.K + CR - 1) / CR;
-  double x[kOzRows], xn[kOzRows];  // this chunk and the next one in flight
+  // D chunks of this thread's rows in flight while one is split: a chunk's
+  // registers are refilled row group by row group, as it is split, with the
+  // chunk D steps ahead.  3 limbs keep two chunks (prefetch distance 2);
+  // 4 and 5 limbs need the registers for their accumulators and keep one.
+  constexpr int D = S == 3 ? 2 : 1;
+  double xa[kOzRows], xb[kOzRows];
   // this thread's 32 rows of chunk c in its column: the chunk's resource
   // ends after its last row below K, so rows at or past K read 0 and every
   // chunk issues the same 32 loads back to back.  Padding columns read
   // column 0 and are zeroed through their scale.
-  auto load_chunk = [&](int64_t c, double (&dst)[kOzRows]) {
+  auto chunk_rsrc = [&](int64_t c) {
     const int64_t r0 = c * CR;
     const int rows = (int)min<int64_t>(CR, a.K - r0);
-    const __amdgpu_buffer_rsrc_t rs =
-        oz_rsrc(mbase + r0 * ld, 8 * ((int64_t)(rows - 1) * ld + width));
+    return oz_rsrc(mbase + r0 * ld, 8 * ((int64_t)(rows - 1) * ld + width));
+  };
+  auto load_chunk = [&](int64_t c, double (&dst)[kOzRows]) {
+    const __amdgpu_buffer_rsrc_t rs = chunk_rsrc(c);
 #pragma unroll
     for (int i = 0; i < kOzRows; i++) dst[i] = oz_ld(rs, voff, i * ld8);
   };
   // frexp exponent of the chunk's largest |x| (max |x| < 2^e), kOzEmin for
   // an all-zero chunk or a padding column
-  auto chunk_exp = [&]() {
+  auto chunk_exp = [&](const double (&x)[kOzRows]) {
     double mx = 0.0;
 #pragma unroll
     for (int i = 0; i < kOzRows; i++) mx = fmax(mx, fabs(x[i]));
@@ -383,8 +391,8 @@ __global__ void __launch_bounds__(kOzThreads)
   auto split_scale = [&]() {
     return cvalid ? max(erun[mat][col], -1074) : 2000;
   };
-  // split chunk c (in x) into S limbs and stage them in LDS buffer b
-  auto limbs = [&](int b) {
+  // split the chunk in x into S limbs and stage them in LDS buffer b
+  auto limbs = [&](int b, const double (&x)[kOzRows]) {
     const int Es = split_scale();
 #pragma unroll
     for (int half = 0; half < 2; half++) {
@@ -429,16 +437,17 @@ __global__ void __launch_bounds__(kOzThreads)
       }
   };
   // Steady-state step: split the chunk in x into LDS buffer bs while the
-  // chunk staged in buffer bm is multiplied.  The splits of 8 groups of 4
-  // rows and the chunk's products are issued interleaved, in fixed units
-  // separated by scheduling fences, so the VALU work runs while the matrix
-  // core is busy.  Returns nonzero if the column's scale must restart: a
-  // value outgrew it, or the chunk's largest limb integer is nonzero but 9
-  // or more bits below it (the chunk after an outlier).
+  // chunk staged in buffer bm is multiplied, and refill x with chunk cnext.
+  // The splits of 8 groups of 4 rows, their refill loads and the chunk's
+  // products are issued interleaved, in fixed units separated by scheduling
+  // fences, so the VALU work runs while the matrix core is busy.  Returns
+  // nonzero if the column's scale must restart: a value outgrew it, or the
+  // chunk's largest limb integer is nonzero but 9 or more bits below it (the
+  // chunk after an outlier).
   // SELF: the products of the chunk's first half run in units 0-3, its
   // second half's operands are read at the end of unit 3 and multiplied in
   // units 4-7.
-  auto split_multiply = [&](int bm, int bs) {
+  auto split_multiply = [&](int bm, int bs, double (&x)[kOzRows], int64_t cnext) {
     v4i av[2][S], bv[2][S];
     auto operands = [&](int h2) {
 #pragma unroll
@@ -450,6 +459,7 @@ __global__ void __launch_bounds__(kOzThreads)
         }
     };
     operands(0);
+    const __amdgpu_buffer_rsrc_t rn = chunk_rsrc(cnext);
     const int Es = split_scale();
     unsigned qmax = 0;
     double tmax = 0.0;
@@ -466,6 +476,9 @@ __global__ void __launch_bounds__(kOzThreads)
 #pragma unroll
             for (int s2 = 0; s2 < S; s2++) words[s2][U % 4] = w[s2];
           }
+          // these 4 rows are split: their registers take chunk cnext
+#pragma unroll
+          for (int j = 0; j < 4; j++) x[4 * U + j] = oz_ld(rn, voff, (4 * U + j) * ld8);
           constexpr int NM = oz_nmfma<S>();
           constexpr int NT = SELF ? 2 * NM : NM;  // products per chunk
           constexpr int k0 = U * NT / 8, k1 = (U + 1) * NT / 8;
@@ -496,41 +509,55 @@ __global__ void __launch_bounds__(kOzThreads)
 
   // Software pipeline: while chunk c is split (VALU) into one LDS buffer,
   // the previous chunk's limbs in the other buffer are multiplied (matrix
-  // cores) and chunk c + G is being loaded.  A flush happens before chunk c
-  // when c raises a column's scale or when its 32 rows would take the
-  // accumulators past kOzFlushRows rows; `since` counts the rows staged
+  // cores) and chunks c + G and c + 2 G are being loaded.  A flush happens
+  // before chunk c when c raises a column's scale or when its rows would take
+  // the accumulators past kOzFlushRows rows; `since` counts the rows staged
   // since the last flush.  Growth shows up while c is being split: the
-  // previous chunk has been multiplied by then, and c is split again after
-  // the flush.
+  // previous chunk has been multiplied by then, and c is loaded and split
+  // again after the flush.
   int64_t ch = kp;
-  load_chunk(min(ch, nch - 1), x);
   int pb = 0;
   while (ch < nch) {
     // (re)start at chunk ch: accumulators empty, nothing staged, so every
     // column's scale restarts from this chunk (an outlier only coarsens the
     // window it falls in); only this thread touches the entry
-    erun[mat][col] = chunk_exp();
+    load_chunk(ch, xa);
+    erun[mat][col] = chunk_exp(xa);
     __syncthreads();
-    load_chunk(min(ch + a.G, nch - 1), xn);
-    limbs(pb);
+    if (D == 2) load_chunk(min(ch + a.G, nch - 1), xb);
+    limbs(pb, xa);
+    load_chunk(min(ch + D * a.G, nch - 1), xa);
     int since = CR;
-#pragma unroll
-    for (int i = 0; i < kOzRows; i++) x[i] = xn[i];
     __syncthreads();
     bool staged = true;  // buffer pb holds a chunk not yet multiplied
-    for (ch += a.G; ch < nch; ch += a.G) {
+    ch += a.G;
+    // one chunk: 0 go on, 1 flush (buffer pb staged), 2 rescale (buffer pb
+    // multiplied, chunk ch split again after the flush)
+    auto step = [&](double (&x)[kOzRows]) -> int {
+      if (ch >= nch) return 1;
       oz_pin<L>(acc);
-      if (since + CR > kOzFlushRows) break;  // uniform
-      load_chunk(min(ch + a.G, nch - 1), xn);
-      const bool rescale = split_multiply(pb, pb ^ 1);
-      if (__syncthreads_or(rescale)) {
-        staged = false;  // buffer pb was multiplied; ch is split again
+      if (since + CR > kOzFlushRows) return 1;  // uniform
+      const bool rescale = split_multiply(pb, pb ^ 1, x, min(ch + D * a.G, nch - 1));
+      if (__syncthreads_or(rescale)) return 2;
+      since += CR;
+      pb ^= 1;
+      ch += a.G;
+      return 0;
+    };
+    // D == 2: xb holds chunk ch and xa chunk ch + G, the two alternate;
+    // D == 1: xa holds chunk ch
+    for (;;) {
+      int r;
+      if constexpr (D == 2) {
+        r = step(xb);
+        if (r == 0) r = step(xa);
+      } else {
+        r = step(xa);
+      }
+      if (r != 0) {
+        staged = r == 1;
         break;
       }
-      since += CR;
-#pragma unroll
-      for (int i = 0; i < kOzRows; i++) x[i] = xn[i];
-      pb ^= 1;
     }
     // drain: multiply the staged chunk, convert the accumulators
     if (staged) multiply(pb);
@@ -647,8 +674,9 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
     return fail(SKINNY_UNSUPPORTED, "ozaki: lda, ldb <= 2^22");
   h = resolve(h);
   if (!h) return no_handle("ozaki");
-  // Gram products read A once (64-row chunks on the diagonal tiles, whose
-  // offsets need lda <= 2^21) and skip the tiles below the diagonal
+  // Gram products skip the tiles below the diagonal; with 3 limbs they also
+  // read A once (64-row chunks on the diagonal tiles, whose offsets need
+  // lda <= 2^21)
   const bool gram = A == B && lda == ldb && M == N && lda <= (1 << 21);
   const int tilesM = (int)((M + kOzTile - 1) / kOzTile);
   const int tilesN = (int)((N + kOzTile - 1) / kOzTile);
@@ -660,7 +688,9 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
   Set sets[2];
   int nsets = 0;
   if (gram) {
-    sets[nsets++] = Set{OZ_DIAG, tilesM, true, {}};
+    // 64-row chunks for 3 limbs; 4 and 5 limbs keep 32-row chunks (their
+    // accumulators leave no registers for the second half's operands)
+    sets[nsets++] = Set{OZ_DIAG, tilesM, slices == 3, {}};
     if (tilesM > 1) sets[nsets++] = Set{OZ_UPPER, tilesM * (tilesM - 1) / 2, false, {}};
   } else {
     sets[nsets++] = Set{OZ_ALL, tilesM * tilesN, false, {}};
