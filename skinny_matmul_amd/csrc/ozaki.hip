@@ -77,6 +77,14 @@ __device__ __forceinline__ double oz_ld(__amdgpu_buffer_rsrc_t r, int voff,
                             __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
+// Input stream: the same load with the nontemporal hint (cache-policy bit 1,
+// `nt` on gfx950), since every row of A and B is read once.
+__device__ __forceinline__ double oz_ld_stream(__amdgpu_buffer_rsrc_t r,
+                                               int voff, int soff) {
+  return __builtin_bit_cast(double,
+                            __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
+}
+
 __device__ __forceinline__ void oz_st(double v, __amdgpu_buffer_rsrc_t r,
                                       int voff, int soff) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, voff,
@@ -371,7 +379,7 @@ __global__ void __launch_bounds__(kOzThreads)
   auto load_chunk = [&](int64_t c, double (&dst)[kOzRows]) {
     const __amdgpu_buffer_rsrc_t rs = chunk_rsrc(c);
 #pragma unroll
-    for (int i = 0; i < kOzRows; i++) dst[i] = oz_ld(rs, voff, i * ld8);
+    for (int i = 0; i < kOzRows; i++) dst[i] = oz_ld_stream(rs, voff, i * ld8);
   };
   // frexp exponent of the chunk's largest |x| (max |x| < 2^e), kOzEmin for
   // an all-zero chunk or a padding column
@@ -478,7 +486,7 @@ __global__ void __launch_bounds__(kOzThreads)
           }
           // these 4 rows are split: their registers take chunk cnext
 #pragma unroll
-          for (int j = 0; j < 4; j++) x[4 * U + j] = oz_ld(rn, voff, (4 * U + j) * ld8);
+          for (int j = 0; j < 4; j++) x[4 * U + j] = oz_ld_stream(rn, voff, (4 * U + j) * ld8);
           constexpr int NM = oz_nmfma<S>();
           constexpr int NT = SELF ? 2 * NM : NM;  // products per chunk
           constexpr int k0 = U * NT / 8, k1 = (U + 1) * NT / 8;

@@ -135,8 +135,21 @@ __global__ void __launch_bounds__(256)
 
   V4 acc[MT][NT];
   Core::zero(acc);
-  Core::accumulate(a.A, a.lda, a.B, a.ldb, a.M, a.N, m0, n0, Mloc, Nloc, a.K,
-                   (int64_t)kp * kWaves + wave, (int64_t)a.G * kWaves, acc);
+  // rows read by one tile only stream past the caches (nontemporal loads);
+  // with several column tiles (only possible with 4 subtiles on a side)
+  // neighbouring workgroups read the same rows and share them through L2
+  if (MT < 4 && NT < 4)
+    Core::template accumulate<true>(a.A, a.lda, a.B, a.ldb, a.M, a.N, m0, n0,
+                                    Mloc, Nloc, a.K, (int64_t)kp * kWaves + wave,
+                                    (int64_t)a.G * kWaves, acc);
+  else if (ntiles == 1)
+    Core::template accumulate<true>(a.A, a.lda, a.B, a.ldb, a.M, a.N, m0, n0,
+                                    Mloc, Nloc, a.K, (int64_t)kp * kWaves + wave,
+                                    (int64_t)a.G * kWaves, acc);
+  else
+    Core::template accumulate<false>(a.A, a.lda, a.B, a.ldb, a.M, a.N, m0, n0,
+                                     Mloc, Nloc, a.K, (int64_t)kp * kWaves + wave,
+                                     (int64_t)a.G * kWaves, acc);
 
   __shared__ T red[Core::RED_ELEMS];
   const int PS = a.outM * a.outN;

@@ -34,7 +34,18 @@ struct TsCore {
   }
 
   // M, N: widths of the whole product (packing uses them); m0, n0: first
-  // column of this tile; Mloc, Nloc: tile columns in range.
+  // column of this tile; Mloc, Nloc: tile columns in range.  STREAM loads the
+  // rows with the nontemporal hint: right when every row is read by one
+  // workgroup only, wrong when several column tiles of a wide product read
+  // the same rows at about the same time and would otherwise share them
+  // through the caches.
+  template <bool STREAM>
+  static __device__ __forceinline__ T ld(const T* p) {
+    if (STREAM) return ld_stream(p);
+    return *p;
+  }
+
+  template <bool STREAM = true>
   static __device__ __forceinline__ void accumulate(
       const T* A, int64_t lda, const T* B, int64_t ldb, int M, int N, int m0,
       int n0, int Mloc, int Nloc, int64_t K, int64_t w0, int64_t wstride,
@@ -74,11 +85,11 @@ struct TsCore {
       for (int s = 0; s < U; s++) {
 #pragma unroll
         for (int t = 0; t < MT; t++)
-          av[s][t] = va[t] ? pa[s * stepA + 16 * t] : T(0);
+          av[s][t] = va[t] ? ld<STREAM>(&pa[s * stepA + 16 * t]) : T(0);
         if (!SELF) {
 #pragma unroll
           for (int u = 0; u < NT; u++)
-            bv[s][u] = vb[u] ? pb[s * stepB + 16 * u] : T(0);
+            bv[s][u] = vb[u] ? ld<STREAM>(&pb[s * stepB + 16 * u]) : T(0);
         }
       }
 #pragma unroll
@@ -101,13 +112,13 @@ struct TsCore {
         T av[MT], bv[NT];
 #pragma unroll
         for (int t = 0; t < MT; t++)
-          av[t] = (va[t] && okA) ? Ab[(r0 + s * RS) * lda + 16 * t] : T(0);
+          av[t] = (va[t] && okA) ? ld<STREAM>(&Ab[(r0 + s * RS) * lda + 16 * t]) : T(0);
 #pragma unroll
         for (int u = 0; u < NT; u++) {
           if (SELF)
             bv[u] = av[u];
           else
-            bv[u] = (vb[u] && okB) ? Bb[(r0 + s * RS) * ldb + 16 * u] : T(0);
+            bv[u] = (vb[u] && okB) ? ld<STREAM>(&Bb[(r0 + s * RS) * ldb + 16 * u]) : T(0);
         }
 #pragma unroll
         for (int t = 0; t < MT; t++)
