@@ -187,7 +187,11 @@ skinny_status_t skinny_stsmm(skinny_handle_t h, size_t blockCount, int64_t M,
                              float beta, float* C, int64_t ldc,
                              skinny_stream_t stream);
 
-/* complex (DC, FC): M, N <= 64; in place as for the real types */
+/* complex (DC, FC): M, N <= 512; in place as for the real types.
+ * Widths: real M, N <= 1024 (complex 512); above 128 real-view columns the
+ * row band of A is staged in LDS and B is packed into the stream's
+ * workspace (tsmm/var1.cuh takes any width out of place, varip1..3 in place
+ * up to 256 columns). */
 skinny_status_t skinny_ztsmm(skinny_handle_t h, size_t blockCount, int64_t M,
                              int64_t N, int64_t K,
                              const skinny_double_complex* A, int64_t lda,
@@ -204,6 +208,41 @@ skinny_status_t skinny_ctsmm(skinny_handle_t h, size_t blockCount, int64_t M,
                              skinny_float_complex beta,
                              skinny_float_complex* C, int64_t ldc,
                              skinny_stream_t stream);
+
+/* ---- TSMM in place, GHOST argument order ------------------------------ */
+/* replaces the kernels fix_ip_ghost_kernel / var_ip_ghost_kernel
+ * (tsmm/fix_ip_ghost.cuh:4-21, tsmm/var_ip_ghost.cuh:7-27):
+ *   x(r, n) <- beta x(r, n) + alpha sum_m x(r, m) w[n stridew + m]
+ * for rows r < nrows, n < ncolsout, m < ncolsin; x row-major with row stride
+ * stridex >= max(ncolsin, ncolsout).  Same kernels, limits and status codes
+ * as skinny_[dszc]tsmm with A = C = x, blockCount 0. */
+skinny_status_t skinny_dtsmm_inplace(skinny_handle_t h, double* x,
+                                     const double* w, double alpha,
+                                     double beta, int64_t nrows,
+                                     int64_t ncolsin, int64_t ncolsout,
+                                     int64_t stridex, int64_t stridew,
+                                     skinny_stream_t stream);
+skinny_status_t skinny_stsmm_inplace(skinny_handle_t h, float* x,
+                                     const float* w, float alpha, float beta,
+                                     int64_t nrows, int64_t ncolsin,
+                                     int64_t ncolsout, int64_t stridex,
+                                     int64_t stridew, skinny_stream_t stream);
+skinny_status_t skinny_ztsmm_inplace(skinny_handle_t h,
+                                     skinny_double_complex* x,
+                                     const skinny_double_complex* w,
+                                     skinny_double_complex alpha,
+                                     skinny_double_complex beta, int64_t nrows,
+                                     int64_t ncolsin, int64_t ncolsout,
+                                     int64_t stridex, int64_t stridew,
+                                     skinny_stream_t stream);
+skinny_status_t skinny_ctsmm_inplace(skinny_handle_t h,
+                                     skinny_float_complex* x,
+                                     const skinny_float_complex* w,
+                                     skinny_float_complex alpha,
+                                     skinny_float_complex beta, int64_t nrows,
+                                     int64_t ncolsin, int64_t ncolsout,
+                                     int64_t stridex, int64_t stridew,
+                                     skinny_stream_t stream);
 
 /* ---- device data generation (counter RNG, identical to the host's) ----- */
 /* out[i] = U[-1,1) on a 2^-52 grid from (seed, stream_id, offset + i); the

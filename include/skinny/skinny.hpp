@@ -194,6 +194,15 @@ inline bool tsmm<skinny_float_complex>(
                                      ldb, beta, C, ldc, nullptr));
 }
 
+// In-place variant (the reference's tsmm_var_ip_ghost, var_ip_ghost.cuh:29-50,
+// registered as VARIPG): false unless C aliases A.
+template <typename T>
+bool tsmm_inplace(size_t bc, int M, int N, int K, const T* A, int lda, T alpha,
+                  const T* B, int ldb, T beta, T* C, int ldc) {
+  if ((const void*)A != (const void*)C || lda != ldc) return false;
+  return tsmm<T>(bc, M, N, K, A, lda, alpha, B, ldb, beta, C, ldc);
+}
+
 // ---- compile-time-width shims (the reference's template shape) ------------
 namespace fixed {
 template <typename T, typename iT, int M, int N>
@@ -237,6 +246,7 @@ VersionList<typename Types::dtype> tsmm_versions() {
   typedef typename Types::dtype dt;
   VersionList<dt> v;
   v.push_back({tsmm<dt>, "SKINNY"});
+  v.push_back({tsmm_inplace<dt>, "SKINNY_IP"});
   return v;
 }
 

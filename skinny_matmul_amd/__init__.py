@@ -100,6 +100,10 @@ _SIGS = {
     "skinny_ctsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, _FComplex, vp, i64, _FComplex, vp, i64, vp]),
     "skinny_ztsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, _DComplex, vp, i64, _DComplex, vp, i64, vp]),
     "skinny_ctsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, _FComplex, vp, i64, _FComplex, vp, i64, vp]),
+    "skinny_dtsmm_inplace": (C.c_int, [vp, vp, vp, dbl, dbl, i64, i64, i64, i64, i64, vp]),
+    "skinny_stsmm_inplace": (C.c_int, [vp, vp, vp, flt, flt, i64, i64, i64, i64, i64, vp]),
+    "skinny_ztsmm_inplace": (C.c_int, [vp, vp, vp, _DComplex, _DComplex, i64, i64, i64, i64, i64, vp]),
+    "skinny_ctsmm_inplace": (C.c_int, [vp, vp, vp, _FComplex, _FComplex, i64, i64, i64, i64, i64, vp]),
     "skinny_dtsmttsm_ozaki": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, C.c_int, vp]),
     "skinny_probe_i8_layout": (C.c_int, [vp]),
     "skinny_htsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, vp]),
@@ -322,6 +326,25 @@ def ctsmm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
     return _check(lib().skinny_ctsmm(_h(handle), blockCount, M, N, K, _addr(A), lda,
                                      _c(alpha), _addr(B), ldb, _c(beta),
                                      _addr(Cm), ldc, _stream(stream)))
+
+
+def tsmm_inplace(x, w, alpha, beta, nrows, ncolsin, ncolsout, stridex, stridew,
+                 handle=None, stream=None) -> bool:
+    """x <- alpha x w + beta x in place, in the GHOST kernels' argument order
+    (tsmm/fix_ip_ghost.cuh:4-7): x is nrows x max(ncolsin, ncolsout) row-major
+    with row stride stridex, w is ncolsin x ncolsout with column stride
+    stridew; the routine is picked from x's dtype."""
+    key = _dtype_key(x)
+    L = lib()
+    args = (_h(handle), _addr(x), _addr(w))
+    rest = (nrows, ncolsin, ncolsout, stridex, stridew, _stream(stream))
+    if key == "DR":
+        return _check(L.skinny_dtsmm_inplace(*args, float(alpha), float(beta), *rest))
+    if key == "FR":
+        return _check(L.skinny_stsmm_inplace(*args, float(alpha), float(beta), *rest))
+    if key == "DC":
+        return _check(L.skinny_ztsmm_inplace(*args, _z(alpha), _z(beta), *rest))
+    return _check(L.skinny_ctsmm_inplace(*args, _c(alpha), _c(beta), *rest))
 
 
 def _dtype_key(t) -> str:

@@ -106,8 +106,8 @@ skinny_status_t tsmm_cplx(const char* fn, skinny_handle_t h, size_t blockCount,
       ldc < std::max<int64_t>(1, N))
     return fail(SKINNY_INVALID_ARG, std::string(fn) + ": leading dimension too small");
   if (K == 0 || N == 0) return SKINNY_OK;
-  if (M > 64 || N > 64)
-    return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": M, N <= 64 only");
+  if (M > kTsmmWideMax / 2 || N > kTsmmWideMax / 2)
+    return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": M, N <= 512 only");
   const bool inplace = (const void*)A == (const void*)C;
   if (inplace && lda != ldc)
     return fail(SKINNY_UNSUPPORTED, std::string(fn) + ": in place needs lda == ldc");
@@ -263,6 +263,58 @@ skinny_status_t skinny_ctsmm(skinny_handle_t h, size_t blockCount, int64_t M,
                              skinny_stream_t stream) {
   return tsmm_cplx<float>("skinny_ctsmm", h, blockCount, M, N, K, A, lda,
                           alpha, B, ldb, beta, C, ldc, stream);
+}
+
+// In-place block-vector update in the argument order of the GHOST kernels
+// (tsmm/fix_ip_ghost.cuh:4-21, var_ip_ghost.cuh:7-27): x is nrows x ncols
+// row-major with row stride stridex (ncols = max(ncolsin, ncolsout)), w is
+// ncolsin x ncolsout with column stride stridew; x <- alpha x w + beta x.
+// The same call as the TSMM entry points with A = C = x.
+skinny_status_t skinny_dtsmm_inplace(skinny_handle_t h, double* x,
+                                     const double* w, double alpha,
+                                     double beta, int64_t nrows,
+                                     int64_t ncolsin, int64_t ncolsout,
+                                     int64_t stridex, int64_t stridew,
+                                     skinny_stream_t stream) {
+  return tsmm_real<double>("skinny_dtsmm_inplace", h, 0, ncolsin, ncolsout,
+                           nrows, x, stridex, alpha, w, stridew, beta, x,
+                           stridex, stream);
+}
+
+skinny_status_t skinny_stsmm_inplace(skinny_handle_t h, float* x,
+                                     const float* w, float alpha, float beta,
+                                     int64_t nrows, int64_t ncolsin,
+                                     int64_t ncolsout, int64_t stridex,
+                                     int64_t stridew, skinny_stream_t stream) {
+  return tsmm_real<float>("skinny_stsmm_inplace", h, 0, ncolsin, ncolsout,
+                          nrows, x, stridex, alpha, w, stridew, beta, x,
+                          stridex, stream);
+}
+
+skinny_status_t skinny_ztsmm_inplace(skinny_handle_t h,
+                                     skinny_double_complex* x,
+                                     const skinny_double_complex* w,
+                                     skinny_double_complex alpha,
+                                     skinny_double_complex beta, int64_t nrows,
+                                     int64_t ncolsin, int64_t ncolsout,
+                                     int64_t stridex, int64_t stridew,
+                                     skinny_stream_t stream) {
+  return tsmm_cplx<double>("skinny_ztsmm_inplace", h, 0, ncolsin, ncolsout,
+                           nrows, x, stridex, alpha, w, stridew, beta, x,
+                           stridex, stream);
+}
+
+skinny_status_t skinny_ctsmm_inplace(skinny_handle_t h,
+                                     skinny_float_complex* x,
+                                     const skinny_float_complex* w,
+                                     skinny_float_complex alpha,
+                                     skinny_float_complex beta, int64_t nrows,
+                                     int64_t ncolsin, int64_t ncolsout,
+                                     int64_t stridex, int64_t stridew,
+                                     skinny_stream_t stream) {
+  return tsmm_cplx<float>("skinny_ctsmm_inplace", h, 0, ncolsin, ncolsout,
+                          nrows, x, stridex, alpha, w, stridew, beta, x,
+                          stridex, stream);
 }
 
 skinny_status_t skinny_htsmttsm(skinny_handle_t h, size_t blockCount,

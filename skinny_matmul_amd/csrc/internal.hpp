@@ -150,4 +150,15 @@ skinny_status_t launch_tsmm_lds(skinny_handle_t h, size_t blockCount,
                                 int64_t ldb, T beta, T beta_i, T* C,
                                 int64_t ldc, bool cplx, hipStream_t stream);
 
+// Wide blocks (real-view M or N above 128, up to kTsmmWideMax): the row band
+// of A in LDS, B packed into the stream's workspace; same arguments as
+// launch_tsmm_lds.
+constexpr int kTsmmWideMax = 1024;
+template <typename T>
+skinny_status_t launch_tsmm_wide(skinny_handle_t h, size_t blockCount,
+                                 int64_t M, int64_t N, int64_t K, const T* A,
+                                 int64_t lda, T alpha, T alpha_i, const T* B,
+                                 int64_t ldb, T beta, T beta_i, T* C,
+                                 int64_t ldc, bool cplx, hipStream_t stream);
+
 }  // namespace skinny

@@ -190,7 +190,7 @@ static hipError_t launch_lds(const TlArgs<T>& a, int S, int blocks,
 }
 
 // M, N, lda, ldc: real-view values; ldb in B elements (complex units when
-// cplx).  Requires M, N <= 128.
+// cplx).  Widths above 128 go to the wide kernel (tsmm_wide.hip).
 // Complex TSMM for narrow blocks (at most 16 complex columns of A): one
 // thread owns whole rows, reads its row of A (M complex values) into
 // registers, forms the N complex dot products with B and writes the row of C
@@ -289,7 +289,8 @@ skinny_status_t launch_tsmm_lds(skinny_handle_t h, size_t blockCount,
                                 int64_t ldb, T beta, T beta_i, T* C,
                                 int64_t ldc, bool cplx, hipStream_t stream) {
   if (M > 128 || N > 128)
-    return fail(SKINNY_UNSUPPORTED, "tsmm: widths above 128 (real view)");
+    return launch_tsmm_wide<T>(h, blockCount, M, N, K, A, lda, alpha, alpha_i,
+                               B, ldb, beta, beta_i, C, ldc, cplx, stream);
   TlArgs<T> a;
   a.A = A;
   a.B = B;
