@@ -225,10 +225,20 @@ def extra_configs(sk, torch, logK: int) -> dict:
     errs = [sk.orth_error(M5, K5, X, M5) for (M5, K5, X, _) in blocks]
     svqb_bytes = sum(2 * 3 * M5 * K5 * 8 for (M5, K5, _, _) in blocks)
     ms_graph = max(ms_all - ms_copy, 1e-6)
+    # the same two sweeps as one cooperative launch with grid barriers
+    # (synchronous call: its per-call buffers are included in the time)
+    def fused():
+        for X, X0 in bufs:
+            X.copy_(X0)
+        sk.dsvqb_fused(blocks, sweeps=2)
+    ms_fused = max(_time_ms(torch, fused, 3) - ms_copy, 1e-6)
+    errs_fused = [sk.orth_error(M5, K5, X, M5) for (M5, K5, X, _) in blocks]
     out["c5"] = {"blocks": 64, "K": K5, "sweeps": 2, "ms": round(ms_graph, 3),
                  "ms_per_block": round(ms_graph / 64, 4),
                  "GBs": round(svqb_bytes / (ms_graph * 1e-3) / 1e9, 1),
-                 "max_orth_err_fro": max(errs)}
+                 "max_orth_err_fro": max(errs),
+                 "fused_grid_barrier_ms": round(ms_fused, 3),
+                 "fused_max_orth_err_fro": max(errs_fused)}
     graph.close()
     del bufs, blocks
     torch.cuda.empty_cache()

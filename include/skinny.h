@@ -293,7 +293,10 @@ typedef struct {
   int64_t ldc;
 } skinny_batch_op;
 typedef struct skinny_batch_s* skinny_batch_t;
-/* Plan (allocates and uploads the queue; item_bytes <= 0 picks 8 MB items).
+/* Plan (allocates and uploads the queue).  Each op is split into items of
+ * about item_bytes of row data; item_bytes <= 0 picks about 8 items per
+ * workgroup of the persistent grid, between 1 and 8 MiB each.  TSMM items
+ * interleave the op's 64-row chunks, so workgroups stream neighbouring rows.
  * TSMTTSM results are reproducible: each op's partials are summed in item
  * order whatever the schedule. */
 skinny_status_t skinny_dbatch_create(skinny_handle_t h, int nops,
@@ -327,6 +330,15 @@ skinny_status_t skinny_dsvqb_graph(skinny_handle_t h, int nblocks,
                                    const skinny_svqb_block* blocks, int sweeps,
                                    skinny_stream_t stream,
                                    skinny_graph_t* graph);
+/* The same sweeps as one cooperative launch of a persistent kernel whose
+ * phases (Gram partials of all blocks; per-block sum and eigensolve; X <- X T
+ * of all blocks) are separated by grid barriers (the barrier of
+ * global_sync/sync.cu:38-53).  Synchronises `stream` before returning; a
+ * barrier that waits longer than about 2 s fails the call instead of
+ * hanging. */
+skinny_status_t skinny_dsvqb_fused(skinny_handle_t h, int nblocks,
+                                   const skinny_svqb_block* blocks, int sweeps,
+                                   skinny_stream_t stream);
 skinny_status_t skinny_graph_launch(skinny_graph_t graph,
                                     skinny_stream_t stream);
 skinny_status_t skinny_graph_destroy(skinny_graph_t graph);
@@ -349,6 +361,17 @@ skinny_status_t skinny_mb_stream(int kind, size_t bytes_per_array, int reps,
  * TFLOP/s (TOP/s).  Replaces microbenches/roofline.cu:19-66. */
 skinny_status_t skinny_mb_peak(int kind, int reps, double* tflops,
                                skinny_stream_t stream);
+/* Memory hierarchy by working-set size (replaces the LDS-throughput and
+ * L1/L2 latency sweeps of microbenches/instruction_rates.cu:30-148):
+ * kind 0: ns per dependent load of one lane chasing a chain over the 128-byte
+ *         lines of a `bytes` buffer (rounded down to a power of two);
+ * kind 1: GB/s of the whole GPU re-reading a `bytes` buffer (each workgroup
+ *         the same slice on every pass; from 4 MiB on all workgroups read);
+ * kind 2: LDS read GB/s of all CUs (bytes ignored);
+ * kind 3: ns per dependent LDS read (bytes ignored).
+ * 4 KiB <= bytes <= 16 GiB for kinds 0 and 1. */
+skinny_status_t skinny_mb_cache(int kind, size_t bytes, int reps, double* out,
+                                skinny_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -121,11 +121,13 @@ _SIGS = {
     "skinny_dbatch_items": (i64, [vp]),
     "skinny_dsvqb": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
     "skinny_dsvqb_graph": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, C.POINTER(vp)]),
+    "skinny_dsvqb_fused": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
     "skinny_graph_launch": (C.c_int, [vp, vp]),
     "skinny_graph_destroy": (C.c_int, [vp]),
     "skinny_dorth_error": (C.c_int, [vp, i64, i64, vp, i64, C.POINTER(C.c_double), vp]),
     "skinny_mb_stream": (C.c_int, [C.c_int, sz, C.c_int, C.POINTER(C.c_double), vp]),
     "skinny_mb_peak": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double), vp]),
+    "skinny_mb_cache": (C.c_int, [C.c_int, sz, C.c_int, C.POINTER(C.c_double), vp]),
 }
 
 # every symbol include/skinny.h declares (checked by the CPU test suite)
@@ -480,6 +482,14 @@ def dsvqb(blocks, sweeps=2, handle=None, stream=None) -> None:
                               _stream(stream)), False)
 
 
+def dsvqb_fused(blocks, sweeps=2, handle=None, stream=None) -> None:
+    """The same SVQB sweeps as one cooperative launch with grid barriers
+    between the Gram, eigensolve and update phases."""
+    arr = _blocks(blocks)
+    _check(lib().skinny_dsvqb_fused(_h(handle), len(blocks), C.cast(arr, vp), sweeps,
+                                    _stream(stream)), False)
+
+
 class Graph:
     """An executable hipGraph of SVQB sweeps (skinny_dsvqb_graph)."""
 
@@ -529,6 +539,20 @@ def mb_peak(kind: str, reps: int = 3, stream=None) -> float:
     """Measured compute peak in TFLOP/s (TOP/s for int8)."""
     out = C.c_double()
     _check(lib().skinny_mb_peak(PEAK_KINDS[kind], reps, C.byref(out), _stream(stream)), False)
+    return out.value
+
+
+CACHE_KINDS = {"latency": 0, "bandwidth": 1, "lds_bandwidth": 2, "lds_latency": 3}
+
+
+def mb_cache(kind: str, nbytes: int = 1 << 20, reps: int = 3, stream=None) -> float:
+    """Memory hierarchy microbenchmark (microbenches/instruction_rates.cu):
+    ns per dependent load over an nbytes chain ("latency"), GB/s re-reading
+    an nbytes buffer ("bandwidth"), LDS GB/s ("lds_bandwidth") or ns per
+    dependent LDS read ("lds_latency")."""
+    out = C.c_double()
+    _check(lib().skinny_mb_cache(CACHE_KINDS[kind], nbytes, reps, C.byref(out),
+                                 _stream(stream)), False)
     return out.value
 
 

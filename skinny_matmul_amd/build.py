@@ -42,6 +42,7 @@ SOURCES = [
     "capi.hip",
     "microbench.hip",
     "svqb.hip",
+    "svqb_fused.hip",
     "batch.hip",
     "tune.hip",
     "ozaki.hip",

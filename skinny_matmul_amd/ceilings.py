@@ -1,6 +1,7 @@
 """Measure this MI355X's own ceilings: HBM streams and compute peaks.
 
     python -m skinny_matmul_amd.ceilings [--gib 4] [--reps 5] [--json out.json]
+                                         [--hierarchy]
 
 The analogue of the reference's membw and roofline microbenchmarks
 (microbenches/membw.cu:19-329, roofline.cu:19-66), run through the library's
@@ -11,6 +12,10 @@ The analogue of the reference's membw and roofline microbenchmarks
   traffic, best of --reps;
 * FP64 vector FMA, FP64 matrix-core (16x16x4) and INT8 matrix-core
   (32x32x32) peaks in TFLOP/s (TOP/s).
+
+* with --hierarchy: ns per dependent load over working sets from 4 KiB to
+  1 GiB, GB/s re-reading buffers from 4 MiB to 8 GiB, LDS GB/s and latency
+  (the role of microbenches/instruction_rates.cu).
 
 These are the measured denominators of the roofline fractions; the datasheet
 figures (8 TB/s, 78.6 TF FP64, 5 POP/s INT8) are upper bounds only.
@@ -40,13 +45,42 @@ def measure(gib: float = 4.0, reps: int = 5) -> dict:
     return out
 
 
+LATENCY_SIZES = tuple(1 << k for k in range(12, 31))          # 4 KiB .. 1 GiB
+BANDWIDTH_SIZES = tuple(1 << k for k in range(22, 34))        # 4 MiB .. 8 GiB
+
+
+def hierarchy(reps: int = 3) -> dict:
+    """Latency and re-read bandwidth by working-set size, and the LDS
+    figures: where L1 (32 KiB per CU), L2 (4 MiB per XCD), the 256 MiB
+    Infinity Cache and HBM take over."""
+    from . import mb_cache
+    out = {"latency_ns": {}, "bandwidth_GBs": {}}
+    for b in LATENCY_SIZES:
+        v = round(mb_cache("latency", b, reps), 1)
+        out["latency_ns"][str(b)] = v
+        print(f"latency   {b / 1024:12.0f} KiB {v:9.1f} ns", flush=True)
+    for b in BANDWIDTH_SIZES:
+        v = round(mb_cache("bandwidth", b, reps), 1)
+        out["bandwidth_GBs"][str(b)] = v
+        print(f"bandwidth {b / 2**20:12.0f} MiB {v:9.1f} GB/s", flush=True)
+    out["lds_GBs"] = round(mb_cache("lds_bandwidth", 0, reps), 1)
+    out["lds_latency_ns"] = round(mb_cache("lds_latency", 0, reps), 2)
+    print(f"LDS {out['lds_GBs']:.1f} GB/s, {out['lds_latency_ns']:.2f} ns per dependent read",
+          flush=True)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gib", type=float, default=4.0)
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--json", default=None)
+    ap.add_argument("--hierarchy", action="store_true",
+                    help="also sweep latency and bandwidth over working-set sizes")
     args = ap.parse_args(argv)
     res = measure(args.gib, args.reps)
+    if args.hierarchy:
+        res["hierarchy"] = hierarchy(max(1, args.reps // 2))
     if args.json:
         with open(args.json, "w") as f:
             json.dump(res, f, indent=1)

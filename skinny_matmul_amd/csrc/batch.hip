@@ -20,29 +20,9 @@
 #include <algorithm>
 #include <vector>
 
-#include "internal.hpp"
-#include "mfma.hpp"
-#include "tsmttsm_core.hpp"
+#include "batch_items.hpp"
 
 namespace skinny {
-
-struct BatchOpDev {
-  int op;  // 0 TSMTTSM, 1 TSMM
-  int M, N, cls;
-  int64_t K, lda, ldb, ldc;
-  const double* A;
-  const double* B;
-  double* C;
-  double alpha, beta;
-  int nparts, outM, outN;
-  int64_t part_off;  // doubles into the partial buffer
-  int cnt;           // completion counter index
-};
-
-struct BatchItemDev {
-  int op, part;
-  int64_t r0, r1;
-};
 
 struct BatchArgs {
   const BatchOpDev* ops;
@@ -52,212 +32,6 @@ struct BatchArgs {
   unsigned* cnt;    // per-op completion counters
   unsigned* queue;  // [0] next item, [1] workgroups exited
 };
-
-// ---- TSMTTSM tile classes ------------------------------------------------
-// 0..5: packed P = 16, 8, 5, 4, 3, 2;  6..14: (MT, NT) in {1, 2, 4}^2
-static int ts_class(int64_t M, int64_t N) {
-  const int64_t W = std::max(M, N);
-  if (W <= 8) {
-    switch (16 / W) {
-      case 16: return 0;
-      case 8: return 1;
-      case 5: return 2;
-      case 4: return 3;
-      case 3: return 4;
-      default: return 5;
-    }
-  }
-  auto c3 = [](int64_t x) { return x <= 16 ? 0 : (x <= 32 ? 1 : 2); };
-  return 6 + 3 * c3(M) + c3(N);
-}
-static void ts_class_shape(int cls, int64_t M, int64_t N, int* outM, int* outN,
-                           int* rows_min) {
-  if (cls < 6) {
-    static const int P[6] = {16, 8, 5, 4, 3, 2};
-    *outM = (int)M;
-    *outN = (int)N;
-    *rows_min = 4 * P[cls] * 8;
-    return;
-  }
-  static const int sz[3] = {1, 2, 4};
-  const int mt = sz[(cls - 6) / 3], nt = sz[(cls - 6) % 3];
-  *outM = 16 * mt;
-  *outN = 16 * nt;
-  *rows_min = 4 * 8;
-}
-
-template <int P, int MT, int NT>
-__device__ __forceinline__ void ts_item(const BatchOpDev& o,
-                                        const BatchItemDev& it, double* red,
-                                        double* out) {
-  constexpr int U = P > 1 ? 8 : ((MT + NT) >= 8 ? 2 : ((MT + NT) >= 4 ? 4 : 8));
-  using Core = TsCore<double, P, MT, NT, U, false>;
-  typename Core::V4 acc[MT][NT];
-  Core::zero(acc);
-  const int wave = threadIdx.x >> 6;
-  Core::accumulate(o.A + it.r0 * o.lda, o.lda, o.B + it.r0 * o.ldb, o.ldb, o.M,
-                   o.N, 0, 0, min(o.M, Core::DM), min(o.N, Core::DN),
-                   it.r1 - it.r0, wave, 4, acc);
-  Core::wg_partial(acc, red, out, o.M, o.N, o.outM, o.outN);
-}
-
-__device__ void run_ts_item(const BatchOpDev& o, const BatchItemDev& it,
-                            double* red, double* out) {
-  switch (o.cls) {
-    case 0: ts_item<16, 1, 1>(o, it, red, out); break;
-    case 1: ts_item<8, 1, 1>(o, it, red, out); break;
-    case 2: ts_item<5, 1, 1>(o, it, red, out); break;
-    case 3: ts_item<4, 1, 1>(o, it, red, out); break;
-    case 4: ts_item<3, 1, 1>(o, it, red, out); break;
-    case 5: ts_item<2, 1, 1>(o, it, red, out); break;
-    case 6: ts_item<1, 1, 1>(o, it, red, out); break;
-    case 7: ts_item<1, 1, 2>(o, it, red, out); break;
-    case 8: ts_item<1, 1, 4>(o, it, red, out); break;
-    case 9: ts_item<1, 2, 1>(o, it, red, out); break;
-    case 10: ts_item<1, 2, 2>(o, it, red, out); break;
-    case 11: ts_item<1, 2, 4>(o, it, red, out); break;
-    case 12: ts_item<1, 4, 1>(o, it, red, out); break;
-    case 13: ts_item<1, 4, 2>(o, it, red, out); break;
-    default: ts_item<1, 4, 4>(o, it, red, out); break;
-  }
-}
-
-// ---- TSMM: rows [r0, r1) of one op by the four waves --------------------
-// classes: S in {1, 2, 4, 8, 16} x NT in {1, 2, 4}  ->  3 * iS + iN;
-// 15, 16: M, N <= 2 (M = 1, 2), one thread per row
-static int tm_class(int64_t M, int64_t N) {
-  if (M <= 2 && N <= 2) return M <= 1 ? 15 : 16;
-  const int64_t S = (M + 3) / 4;
-  const int iS = S <= 1 ? 0 : S <= 2 ? 1 : S <= 4 ? 2 : S <= 8 ? 3 : 4;
-  const int64_t nt = (N + 15) / 16;
-  const int iN = nt <= 1 ? 0 : (nt <= 2 ? 1 : 2);
-  return 3 * iS + iN;
-}
-
-template <int S, int NT>
-__device__ void tm_item(const BatchOpDev& o, const BatchItemDev& it) {
-  using V4 = Mfma<double>::v4;
-  const int lane = threadIdx.x & 63;
-  const int li = lane & 15, lg = lane >> 4;
-  const int wave = threadIdx.x >> 6;
-  const int M = o.M, N = o.N;
-  double bop[S][NT];
-#pragma unroll
-  for (int s = 0; s < S; s++)
-#pragma unroll
-    for (int u = 0; u < NT; u++) {
-      const int m = lg * S + s, n = 16 * u + li;
-      bop[s][u] = (m < M && n < N) ? o.B[(int64_t)n * o.ldb + m] : 0.0;
-    }
-  const int mbeg = lg * S;
-  const bool beta0 = o.beta == 0.0;
-  // rows [r0, r1) in 16-row blocks (r0 is a multiple of 16); a wave takes RB
-  // consecutive blocks per step, all their loads in flight before the first
-  // MFMA (about 8 loads per lane); wave w owns blocks w RB .. w RB + RB - 1
-  // of every group of 4 RB, so in place needs no barrier
-  constexpr int RB = S >= 16 ? 1 : (S >= 8 ? 2 : (S >= 4 ? 4 : 8));
-  const int64_t nblk = (it.r1 - it.r0 + 15) >> 4;
-  for (int64_t b0 = (int64_t)wave * RB; b0 < nblk; b0 += 4 * RB) {
-    double av[RB][S];
-#pragma unroll
-    for (int r = 0; r < RB; r++) {
-      const int64_t row = it.r0 + (b0 + r) * 16 + li;
-      const bool ok = row < it.r1;
-      const double* ar = o.A + row * o.lda + mbeg;
-#pragma unroll
-      for (int s = 0; s < S; s++) av[r][s] = (ok && mbeg + s < M) ? ar[s] : 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < RB; r++) {
-      V4 acc[NT];
-#pragma unroll
-      for (int u = 0; u < NT; u++) acc[u] = V4{0, 0, 0, 0};
-#pragma unroll
-      for (int s = 0; s < S; s++)
-#pragma unroll
-        for (int u = 0; u < NT; u++) acc[u] = Mfma<double>::mac(av[r][s], bop[s][u], acc[u]);
-#pragma unroll
-      for (int u = 0; u < NT; u++) {
-        const int col = 16 * u + li;
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-          const int64_t rr = it.r0 + (b0 + r) * 16 + Mfma<double>::drow(lg, v);
-          if (rr < it.r1 && col < N) {
-#pragma clang fp contract(off)
-            double* c = o.C + rr * o.ldc + col;
-            const double sres = acc[u][v];
-            *c = beta0 ? o.alpha * sres : o.beta * *c + o.alpha * sres;
-          }
-        }
-      }
-    }
-  }
-}
-
-// M, N <= 2: a thread owns whole rows (R rows in flight), B in registers;
-// every load of a row precedes its stores, so C may alias A
-template <int MC>
-__device__ void tm_rows_item(const BatchOpDev& o, const BatchItemDev& it) {
-  constexpr int R = 4;
-  const int M = o.M, N = o.N;
-  const bool beta0 = o.beta == 0.0;
-  double b[2][2];
-#pragma unroll
-  for (int m = 0; m < 2; m++)
-#pragma unroll
-    for (int n = 0; n < 2; n++)
-      b[m][n] = (m < M && n < N) ? o.B[(int64_t)n * o.ldb + m] : 0.0;
-  const int64_t nrows = it.r1 - it.r0;
-  for (int64_t base = threadIdx.x; base < nrows; base += 256 * R) {
-    double av[R][MC];
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      const int64_t q = base + 256 * r;
-      const double* ar = o.A + (it.r0 + q) * o.lda;
-#pragma unroll
-      for (int m = 0; m < MC; m++) av[r][m] = (q < nrows && m < M) ? ar[m] : 0.0;
-    }
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-      const int64_t q = base + 256 * r;
-      if (q >= nrows) continue;
-      double* cr = o.C + (it.r0 + q) * o.ldc;
-#pragma unroll
-      for (int n = 0; n < 2; n++) {
-        if (n >= N) continue;
-        double sres = 0.0;
-#pragma unroll
-        for (int m = 0; m < MC; m++) sres = __builtin_fma(av[r][m], b[m][n], sres);
-        {
-#pragma clang fp contract(off)
-          cr[n] = beta0 ? o.alpha * sres : o.beta * cr[n] + o.alpha * sres;
-        }
-      }
-    }
-  }
-}
-
-__device__ void run_tm_item(const BatchOpDev& o, const BatchItemDev& it) {
-  switch (o.cls) {
-    case 15: tm_rows_item<1>(o, it); break;
-    case 16: tm_rows_item<2>(o, it); break;
-    case 0: tm_item<1, 1>(o, it); break;
-    case 1: tm_item<1, 2>(o, it); break;
-    case 2: tm_item<1, 4>(o, it); break;
-    case 3: tm_item<2, 1>(o, it); break;
-    case 4: tm_item<2, 2>(o, it); break;
-    case 5: tm_item<2, 4>(o, it); break;
-    case 6: tm_item<4, 1>(o, it); break;
-    case 7: tm_item<4, 2>(o, it); break;
-    case 8: tm_item<4, 4>(o, it); break;
-    case 9: tm_item<8, 1>(o, it); break;
-    case 10: tm_item<8, 2>(o, it); break;
-    case 11: tm_item<8, 4>(o, it); break;
-    case 12: tm_item<16, 1>(o, it); break;
-    case 13: tm_item<16, 2>(o, it); break;
-    default: tm_item<16, 4>(o, it); break;
-  }
-}
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     batch_kernel(BatchArgs a) {
@@ -358,67 +132,12 @@ skinny_status_t skinny_dbatch_create(skinny_handle_t h, int nops,
     return fail(SKINNY_INVALID_ARG, "batch: no ops");
   h = resolve(h);
   if (!h) return no_handle("batch");
-  if (item_bytes <= 0) item_bytes = 8LL << 20;
-  std::vector<BatchOpDev> dops(nops);
+  std::vector<BatchOpDev> dops;
   std::vector<BatchItemDev> items;
   size_t part_doubles = 0;
-  for (int i = 0; i < nops; i++) {
-    const skinny_batch_op& o = ops[i];
-    if (o.op != 0 && o.op != 1) return fail(SKINNY_INVALID_ARG, "batch: bad op kind");
-    if (o.M < 1 || o.N < 1 || o.K < 0 || o.M > 64 || o.N > 64)
-      return fail(SKINNY_UNSUPPORTED, "batch: 1 <= M, N <= 64 only");
-    if (!o.A || !o.B || !o.C) return fail(SKINNY_INVALID_ARG, "batch: null matrix");
-    BatchOpDev d{};
-    d.op = o.op;
-    d.M = (int)o.M;
-    d.N = (int)o.N;
-    d.K = o.K;
-    d.lda = o.lda;
-    d.ldb = o.ldb;
-    d.ldc = o.ldc;
-    d.A = o.A;
-    d.B = o.B;
-    d.C = o.C;
-    d.alpha = o.alpha;
-    d.beta = o.beta;
-    d.cnt = i;
-    const int64_t row_bytes = 8 * (o.op == 0 ? (o.M + o.N) : (o.M + o.N));
-    if (o.op == 0) {
-      if (o.lda < o.M || o.ldb < o.N || o.ldc < o.M)
-        return fail(SKINNY_INVALID_ARG, "batch: leading dimension too small");
-      int rows_min;
-      d.cls = ts_class(o.M, o.N);
-      ts_class_shape(d.cls, o.M, o.N, &d.outM, &d.outN, &rows_min);
-      int64_t rows = std::max<int64_t>(item_bytes / row_bytes, rows_min);
-      rows = std::max<int64_t>(rows, (o.K + 63) / 64);  // at most 64 partials
-      rows = (rows + rows_min - 1) / rows_min * rows_min;
-      const int64_t np = std::max<int64_t>(1, (o.K + rows - 1) / rows);
-      d.nparts = (int)np;
-      d.part_off = (int64_t)part_doubles;
-      part_doubles += (size_t)np * d.outM * d.outN;
-      for (int64_t p = 0; p < np; p++)
-        items.push_back({i, (int)p, p * rows, std::min(o.K, (p + 1) * rows)});
-    } else {
-      if (o.lda < o.M || o.ldb < o.M || o.ldc < o.N)
-        return fail(SKINNY_INVALID_ARG, "batch: leading dimension too small");
-      if (o.A == o.C && o.lda != o.ldc)
-        return fail(SKINNY_UNSUPPORTED, "batch: in place needs lda == ldc");
-      d.cls = tm_class(o.M, o.N);
-      int64_t rows = std::max<int64_t>(item_bytes / row_bytes, 64);
-      rows = (rows + 63) / 64 * 64;
-      for (int64_t r = 0; r < o.K; r += rows)
-        items.push_back({i, 0, r, std::min(o.K, r + rows)});
-      d.nparts = 0;
-    }
-    dops[i] = d;
-  }
-  // largest items first: the tail of the queue is the small remainders
-  std::stable_sort(items.begin(), items.end(),
-                   [&](const BatchItemDev& x, const BatchItemDev& y) {
-                     const BatchOpDev& a = dops[x.op];
-                     const BatchOpDev& b = dops[y.op];
-                     return (x.r1 - x.r0) * (a.M + a.N) > (y.r1 - y.r0) * (b.M + b.N);
-                   });
+  skinny_status_t ps = batch_plan(nops, ops, item_bytes, 2 * h->cu_count, dops, items,
+                                  part_doubles);
+  if (ps != SKINNY_OK) return ps;
   skinny_batch_t b = new skinny_batch_s();
   b->device = h->device;
   b->nops = nops;

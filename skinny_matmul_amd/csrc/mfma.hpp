@@ -5,12 +5,29 @@
 
 namespace skinny {
 
+// Global-memory view of a pointer into device memory.  Pointers that are
+// read from memory (the fields of a work-item record) are generic, and
+// accesses through them compile to flat_* instructions, which need 64-bit
+// vector addresses and count against both the vector-memory and the LDS
+// wait counters; through this view they compile to global_* instructions.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
+// LDS view of a pointer into a workgroup's shared memory (ds_* instead of
+// flat_* instructions when the pointer was passed through a call).
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(3))) T* lmem(T* p) {
+  return (__attribute__((address_space(3))) T*)p;
+}
+
 // Streaming load with the nontemporal hint (global_load ... nt): inputs that
 // are read once do not displace cached data.  On MI355X a read stream with
 // the hint runs at about 7.1 TB/s against 6.4 TB/s without it.
 template <typename T>
 __device__ __forceinline__ T ld_stream(const T* p) {
-  return __builtin_nontemporal_load(p);
+  return __builtin_nontemporal_load(gmem(p));
 }
 
 // 16 x 16 x 4 MFMA, fp64 and fp32.  Operands: lane l supplies

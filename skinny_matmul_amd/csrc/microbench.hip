@@ -147,6 +147,98 @@ __global__ void __launch_bounds__(256) mfma_i8_kernel(int* out, int iters,
   if (s == 12345) out[threadIdx.x] = s;
 }
 
+// ---- memory hierarchy: latency and bandwidth by working-set size ---------
+// (the role of microbenches/instruction_rates.cu:30-148: LDS throughput and
+// the L1 / L2 latency sweep, measured here for gfx950's L1, L2, Infinity
+// Cache and HBM)
+
+// Pointer chain over the 128-byte lines of a buffer: line l holds the word
+// offset of line (l + P) mod n_lines, P odd and near 0.618 n_lines, so the
+// chain visits every line (n_lines is a power of two) in a scattered order.
+__global__ void __launch_bounds__(256)
+    chain_init_kernel(unsigned* a, int64_t n_lines, int64_t P) {
+  for (int64_t l = (int64_t)blockIdx.x * 256 + threadIdx.x; l < n_lines;
+       l += (int64_t)gridDim.x * 256) {
+    const int64_t nxt = (l + P) & (n_lines - 1);
+    a[l * 32] = (unsigned)(nxt * 32);
+  }
+}
+
+// One lane follows the chain: every load's address is the previous load's
+// value.  Word offsets stay below n_lines * 32 by construction.
+__global__ void chase_kernel(const unsigned* a, unsigned start, int64_t steps,
+                             unsigned* sink) {
+  if (threadIdx.x != 0) return;
+  unsigned idx = start;
+  for (int64_t i = 0; i < steps; i++) idx = a[idx];
+  sink[0] = idx;
+}
+
+// `passes` reads of the whole buffer by the whole grid (grid-stride, 4
+// accesses of 16 bytes in flight per lane); a workgroup reads the same
+// addresses on every pass, so a buffer that fits in a cache level is served
+// from it after the first pass.
+__global__ void __launch_bounds__(256)
+    reread_kernel(const d2* __restrict__ b, int64_t n2, int passes,
+                  double* __restrict__ sink) {
+  d2 acc = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int p = 0; p < passes; p++) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n2; i += 4 * stride) {
+      d2 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t j = i + u * stride;
+        v[u] = j < n2 ? b[j] : d2{0.0, 0.0};
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) acc += v[u];
+    }
+  }
+  if (acc.x + acc.y == 12345.678) sink[threadIdx.x] = acc.x;
+}
+
+// LDS read bandwidth: 16-byte reads, lane l of wave w reading entry l (or
+// 64 + l) of its wave's row, so the 64 lanes of a read are conflict-free.
+// Eight independent chains per lane: each read's index adds a value from
+// the previous read of the same chain (always 0), which keeps the compiler
+// from hoisting the loop-invariant reads.
+template <int ITERS>
+__global__ void __launch_bounds__(256) lds_bw_kernel(unsigned* sink) {
+  __shared__ uint4 buf[4][128];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  buf[w][l] = uint4{(unsigned)l, 1u, 2u, 3u};
+  buf[w][64 + l] = uint4{(unsigned)l, 5u, 6u, 7u};
+  __syncthreads();
+  uint4 acc = {0, 0, 0, 0};
+  unsigned off[8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) off[j] = 0;
+  for (int it = 0; it < ITERS; it++) {
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const uint4 v = buf[w][(j & 1) * 64 + l + off[j]];
+      off[j] = v.y >> 8;  // 0: entries hold 1 or 5 here
+      acc.x += v.x;
+      acc.z += v.z;
+      acc.w ^= v.w;
+    }
+  }
+  if ((acc.x + acc.z + acc.w) == 0xdeadbeefu) sink[threadIdx.x] = acc.x;
+}
+
+// LDS latency: one lane follows a chain of dependent LDS reads.
+__global__ void __launch_bounds__(64) lds_chase_kernel(int steps,
+                                                       unsigned* sink) {
+  __shared__ unsigned buf[1024];
+  for (int i = threadIdx.x; i < 1024; i += 64) buf[i] = (unsigned)((i + 37) & 1023);
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  unsigned idx = 0;
+  for (int i = 0; i < steps; i++) idx = buf[idx & 1023];
+  sink[0] = idx;
+}
+
 }  // namespace
 }  // namespace skinny
 
@@ -229,6 +321,100 @@ skinny_status_t skinny_mb_stream(int kind, size_t bytes_per_array, int reps,
   (void)hipFree(c);
   (void)hipFree(sink);
   if (e != hipSuccess) return hip_fail(e, "skinny_mb_stream launch");
+  return SKINNY_OK;
+}
+
+// kind 0: latency (ns per dependent load) of a chain over `bytes` bytes;
+// 1: aggregate read bandwidth (GB/s) of the whole GPU re-reading a buffer of
+// `bytes` bytes; 2: LDS read bandwidth (GB/s, all CUs; bytes ignored);
+// 3: LDS latency (ns per dependent read; bytes ignored).
+skinny_status_t skinny_mb_cache(int kind, size_t bytes, int reps,
+                                double* out, skinny_stream_t stream) {
+  if (!out || kind < 0 || kind > 3 || reps < 1 ||
+      ((kind == 0 || kind == 1) && (bytes < 4096 || bytes > ((size_t)16 << 30))))
+    return fail(SKINNY_INVALID_ARG, "skinny_mb_cache: bad arguments");
+  skinny_handle_t h = skinny_default_handle();
+  if (!h) return no_handle("microbench");
+  hipStream_t st = (hipStream_t)stream;
+  void* sink = nullptr;
+  hipError_t e = hipMalloc(&sink, 4096);
+  if (e != hipSuccess) return hip_fail(e, "skinny_mb_cache alloc");
+  void* buf = nullptr;
+  if (kind <= 1) {
+    // power-of-two number of 128-byte lines for the chain
+    size_t nb = 128;
+    while (nb * 2 <= bytes) nb *= 2;
+    if (kind == 1) nb = bytes / 16 * 16;
+    e = hipMalloc(&buf, nb);
+    if (e != hipSuccess) {
+      (void)hipFree(sink);
+      return hip_fail(e, "skinny_mb_cache alloc");
+    }
+    bytes = nb;
+  }
+  float ms = 0.0f;
+  double count = 0.0;
+  if (kind == 0) {
+    const int64_t n_lines = (int64_t)(bytes / 128);
+    int64_t P = (int64_t)(0.6180339887 * (double)n_lines) | 1;
+    if (P >= n_lines) P = n_lines > 1 ? n_lines - 1 : 1;
+    if (n_lines == 1) P = 0;
+    const int gb = (int)std::min<int64_t>((n_lines + 255) / 256, 1024);
+    hipLaunchKernelGGL(chain_init_kernel, dim3(gb), dim3(256), 0, st, (unsigned*)buf,
+                       n_lines, P);
+    // Buffers that fit the 256 MiB Infinity Cache are first read by the
+    // whole GPU and once along the chain (up to 64k lines), so the timed
+    // chase finds them in the smallest level that holds them.  Larger ones
+    // are not warmed, and each timed launch starts where the previous one
+    // stopped, so its lines come from HBM.
+    const int64_t steps = 1 << 16;
+    if (bytes <= ((size_t)256 << 20)) {
+      hipLaunchKernelGGL(reread_kernel, dim3(h->cu_count), dim3(256), 0, st,
+                         (const d2*)buf, (int64_t)(bytes / 16), 1, (double*)sink);
+      hipLaunchKernelGGL(chase_kernel, dim3(1), dim3(64), 0, st, (const unsigned*)buf, 0u,
+                         std::min<int64_t>(n_lines, steps), (unsigned*)sink);
+    }
+    int64_t launch = 0;
+    ms = time_ms([&]() {
+      // first line of segment `launch` of the chain: line (launch steps P) mod n
+      const int64_t line = (int64_t)(((unsigned __int128)(launch * steps) * (unsigned __int128)P) %
+                                     (unsigned __int128)n_lines);
+      const unsigned start = bytes <= ((size_t)256 << 20) ? 0u : (unsigned)(line * 32);
+      launch++;
+      hipLaunchKernelGGL(chase_kernel, dim3(1), dim3(64), 0, st, (const unsigned*)buf, start,
+                         steps, (unsigned*)sink);
+    }, st, reps);
+    count = (double)steps;
+    *out = (double)ms * 1e6 / count;
+  } else if (kind == 1) {
+    const int64_t n2 = (int64_t)(bytes / 16);
+    // about 16 GiB of reads per launch (at least one pass)
+    const int blocks = 4 * h->cu_count;
+    int passes = (int)std::max<int64_t>(1, ((int64_t)16 << 30) / (int64_t)bytes);
+    if (passes > 1 << 20) passes = 1 << 20;
+    ms = time_ms([&]() {
+      hipLaunchKernelGGL(reread_kernel, dim3(blocks), dim3(256), 0, st, (const d2*)buf, n2,
+                         passes, (double*)sink);
+    }, st, reps);
+    *out = (double)bytes * passes / (ms * 1e-3) / 1e9;
+  } else if (kind == 2) {
+    constexpr int IT = 4096;
+    const int blocks = 8 * h->cu_count;
+    ms = time_ms([&]() {
+      hipLaunchKernelGGL(lds_bw_kernel<IT>, dim3(blocks), dim3(256), 0, st, (unsigned*)sink);
+    }, st, reps);
+    *out = (double)blocks * 256 * IT * 8 * 16 / (ms * 1e-3) / 1e9;
+  } else {
+    const int steps = 1 << 16;
+    ms = time_ms([&]() {
+      hipLaunchKernelGGL(lds_chase_kernel, dim3(1), dim3(64), 0, st, steps, (unsigned*)sink);
+    }, st, reps);
+    *out = (double)ms * 1e6 / steps;
+  }
+  e = hipGetLastError();
+  (void)hipFree(buf);
+  (void)hipFree(sink);
+  if (e != hipSuccess) return hip_fail(e, "skinny_mb_cache launch");
   return SKINNY_OK;
 }
 

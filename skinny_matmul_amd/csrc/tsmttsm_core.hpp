@@ -42,7 +42,7 @@ struct TsCore {
   template <bool STREAM>
   static __device__ __forceinline__ T ld(const T* p) {
     if (STREAM) return ld_stream(p);
-    return *p;
+    return *gmem(p);
   }
 
   template <bool STREAM = true>
@@ -138,9 +138,10 @@ struct TsCore {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int li = lane & 15, lg = lane >> 4;
-    T* red0 = red;
-    T* red1 = red + NACC * 64;
-    auto slot = [&](T* base, int t, int u, int v) -> T& {
+    using LT = __attribute__((address_space(3))) T;
+    LT* red0 = lmem(red);
+    LT* red1 = red0 + NACC * 64;
+    auto slot = [&](LT* base, int t, int u, int v) -> LT& {
       return base[((t * NT + u) * 4 + v) * 64 + lane];
     };
     if (wave >= 2) {
@@ -183,7 +184,7 @@ struct TsCore {
     __syncthreads();
     // D tile as a DM x DN row-major matrix (MFMA D layout: lane l, element v
     // holds D[drow(l / 16, v)][l % 16])
-    T* Dm = red1;
+    LT* Dm = red1;
     if (wave == 0) {
 #pragma unroll
       for (int t = 0; t < MT; t++)
@@ -205,7 +206,7 @@ struct TsCore {
       } else {
         s = Dm[ml * DN + nl];
       }
-      out[e] = s;
+      *gmem(out + e) = s;
     }
     __syncthreads();
   }

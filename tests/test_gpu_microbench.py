@@ -1,0 +1,41 @@
+"""The HIP microbenchmarks give physically ordered figures on the GPU.
+
+HBM streams and compute peaks (microbenches/membw.cu, roofline.cu) and the
+memory-hierarchy sweep (microbenches/instruction_rates.cu): results must sit
+below the MI355X datasheet peaks, and a working set that fits a cache level
+must be served faster than one that only fits HBM.
+"""
+import pytest
+
+from gpu_util import note
+
+pytestmark = pytest.mark.gpu
+
+
+def test_stream_and_peaks_below_datasheet(gpu):
+    note("microbench: read stream 1 GiB, FP64 matrix-core peak")
+    rd = gpu.mb_stream("read", 1 << 30, 2)
+    assert 2000.0 < rd < 8000.0 * 1.02, rd
+    fp = gpu.mb_peak("fp64_mfma", 1)
+    assert 10.0 < fp < 78.6 * 1.02, fp
+
+
+def test_hierarchy_is_ordered(gpu):
+    note("microbench: latency 16 KiB / 2 MiB / 1 GiB, re-read 8 MiB / 4 GiB, LDS")
+    l1 = gpu.mb_cache("latency", 16 << 10, 1)
+    l2 = gpu.mb_cache("latency", 2 << 20, 1)
+    hbm = gpu.mb_cache("latency", 1 << 30, 1)
+    assert 0 < l1 < l2 < hbm, (l1, l2, hbm)
+    assert hbm < 10000.0, hbm        # a dependent load takes well under 10 us
+    bw_cache = gpu.mb_cache("bandwidth", 8 << 20, 1)
+    bw_hbm = gpu.mb_cache("bandwidth", 4 << 30, 1)
+    assert bw_cache > bw_hbm > 1000.0, (bw_cache, bw_hbm)
+    lds = gpu.mb_cache("lds_bandwidth", 0, 1)
+    assert lds > bw_hbm, (lds, bw_hbm)
+    lds_lat = gpu.mb_cache("lds_latency", 0, 1)
+    assert 0 < lds_lat < l1, (lds_lat, l1)
+
+
+def test_bad_arguments_raise(gpu):
+    with pytest.raises(gpu.SkinnyError):
+        gpu.mb_cache("latency", 100, 1)

@@ -90,3 +90,38 @@ def test_svqb_graph_replays_match_eager(gpu, orc):
         assert np.array_equal(host(dX), want), rep
         del junk
     g.close()
+
+
+@pytest.mark.parametrize("Ms", [(8,), (3, 16), (8, 16, 32, 64)])
+def test_svqb_fused_grid_barrier(gpu, orc, Ms):
+    """One cooperative launch per call: orthogonality as the eager path, the
+    span kept, and bitwise reproducible from call to call."""
+    K = 6000
+    data = []
+    for i, M in enumerate(Ms):
+        X0 = orc.svqb_test_block(M, K + 100 * i, 400 + 10 * i + M, cond_exp=6.0)
+        data.append((M, K + 100 * i, X0))
+    note(f"svqb fused Ms={Ms}")
+    outs = []
+    for rep in range(2):
+        devs = [dev(X0.copy()) for (_, _, X0) in data]
+        gpu.dsvqb_fused([(M, Kb, d, M) for (M, Kb, _), d in zip(data, devs)], sweeps=2)
+        outs.append([host(d) for d in devs])
+    for (M, Kb, X0), got, again in zip(data, outs[0], outs[1]):
+        assert np.array_equal(got, again)
+        fro, _ = orc.gram_error(M, Kb, got, M)
+        assert fro < 1e-11, (M, fro)
+        Xn = got.reshape(Kb, M)
+        R = Xn.T @ X0.reshape(Kb, M)
+        assert np.linalg.norm(X0.reshape(Kb, M) - Xn @ R) <= 1e-9 * np.linalg.norm(X0)
+
+
+def test_svqb_fused_one_sweep_matches_oracle(gpu, orc):
+    M, K = 16, 8192
+    X0 = orc.svqb_test_block(M, K, 170, cond_exp=3.0)
+    dX = dev(X0.copy())
+    note("svqb fused one sweep vs oracle")
+    gpu.dsvqb_fused([(M, K, dX, M)], sweeps=1)
+    Xc = X0.copy()
+    orc.svqb_sweep(M, K, Xc, M)
+    assert _match_columns(host(dX).reshape(K, M), Xc.reshape(K, M)) < 1e-8
