@@ -218,7 +218,10 @@ struct TmBlocks {
 template <typename T, int S, int NT, bool BETA0, bool VEC>
 __global__ void __launch_bounds__(256) tsmm_mfma_kernel(TmArgs<T> a) {
   using V4 = typename Mfma<T>::v4;
-  constexpr int RB = TmBlocks<S>::value;
+  // with beta != 0 the old C tiles of all RB blocks are loaded with A (at
+  // most 64 values per lane), so their latency is not paid once per store
+  constexpr int RB0 = TmBlocks<S>::value;
+  constexpr int RB = BETA0 ? RB0 : (RB0 < 16 / NT ? RB0 : 16 / NT);
   const int lane = threadIdx.x & 63;
   const int li = lane & 15, lg = lane >> 4;
   const int M = a.M, N = a.N;
@@ -264,6 +267,20 @@ __global__ void __launch_bounds__(256) tsmm_mfma_kernel(TmArgs<T> a) {
           av[r][s] = (ok && mbeg + s < M) ? ar[s] : T(0);
       }
     }
+    T cold[BETA0 ? 1 : RB][BETA0 ? 1 : NT][4];
+    if (!BETA0) {
+#pragma unroll
+      for (int r = 0; r < RB; r++)
+#pragma unroll
+        for (int u = 0; u < NT; u++)
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            const int64_t row = (b0 + r) * 16 + Mfma<T>::drow(lg, v);
+            const int col = 16 * u + li;
+            cold[BETA0 ? 0 : r][BETA0 ? 0 : u][v] =
+                (row < K && col < N) ? a.C[row * a.ldc + col] : T(0);
+          }
+    }
 #pragma unroll
     for (int r = 0; r < RB; r++) {
       V4 acc[NT];
@@ -282,7 +299,7 @@ __global__ void __launch_bounds__(256) tsmm_mfma_kernel(TmArgs<T> a) {
           const int64_t row = (b0 + r) * 16 + Mfma<T>::drow(lg, v);
           if (row < K && col < N) {
             T* c = a.C + row * a.ldc + col;
-            T old = BETA0 ? T(0) : *c;
+            const T old = BETA0 ? T(0) : cold[BETA0 ? 0 : r][BETA0 ? 0 : u][v];
             *c = axpby_t(acc[u][v], old, a.alpha, a.beta, BETA0);
           }
         }
