@@ -58,6 +58,11 @@ struct TmArgs {
   // complex units, alpha = 1 and (cre, cim) the complex alpha folded into Bh
   int cplxB;
   T cre, cim;
+  // in place with beta != 0 and N <= M (tsmm_mfma_kernel only): the old C
+  // columns are A's own columns, so C = A (fa B + fb I) with alpha = 1,
+  // beta = 0 and no read of C
+  int fold;
+  T fa, fb;
 };
 
 // Element (m, n) of the real B operand: B itself, or Bh of the complex view.
@@ -236,7 +241,12 @@ __global__ void __launch_bounds__(256) tsmm_mfma_kernel(TmArgs<T> a) {
 #pragma unroll
     for (int u = 0; u < NT; u++) {
       const int m = lg * S + s, n = 16 * u + li;
-      bop[s][u] = (m < M && n < N) ? tm_bval(a, m, n) : T(0);
+      T b = (m < M && n < N) ? tm_bval(a, m, n) : T(0);
+      if (a.fold && m < M && n < N) {
+#pragma clang fp contract(off)
+        b = a.fa * b + (m == n ? a.fb : T(0));
+      }
+      bop[s][u] = b;
     }
   const int mbeg = lg * S;
 
@@ -402,12 +412,23 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
   a.cplxB = 0;
   a.cre = T(1);
   a.cim = T(0);
+  a.fold = 0;
+  a.fa = T(1);
+  a.fb = T(0);
   int blocks = blockCount > 0 ? (int)blockCount
                               : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T));
-  const bool beta0 = (beta == T(0));
+  bool beta0 = (beta == T(0));
   // the matrix-core kernel from 3 columns of A, and for M <= 2 once C is 4
   // or more columns wide (its 16-column stores then beat whole-row threads)
   if ((M >= 3 || N >= 4) && N <= 64) {
+    if (!beta0 && (const void*)A == (const void*)C && lda == ldc && N <= M) {
+      a.fold = 1;
+      a.fa = alpha;
+      a.fb = beta;
+      a.alpha = T(1);
+      a.beta = T(0);
+      beta0 = true;
+    }
     const int S = (int)((M + 3) / 4);
     const int Sc = S <= 1 ? 1 : S <= 2 ? 2 : S <= 4 ? 4 : S <= 8 ? 8 : 16;
     const int nt = (int)((N + 15) / 16);
@@ -468,9 +489,20 @@ skinny_status_t launch_tsmm_cview(skinny_handle_t h, size_t blockCount,
   a.cplxB = 1;
   a.cre = alpha_r;
   a.cim = alpha_i;
+  a.fold = 0;
+  a.fa = T(1);
+  a.fb = T(0);
   int blocks = blockCount > 0 ? (int)blockCount
                               : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T));
-  const bool beta0 = (beta == T(0));
+  bool beta0 = (beta == T(0));
+  // in place, real beta, N <= M: beta I joins the real-view operand (the
+  // complex alpha is already in it)
+  if (!beta0 && (const void*)A == (const void*)C && lda == ldc && N <= M) {
+    a.fold = 1;
+    a.fb = beta;
+    a.beta = T(0);
+    beta0 = true;
+  }
   const int S = (int)((M + 3) / 4);
   const int Sc = S <= 1 ? 1 : S <= 2 ? 2 : S <= 4 ? 4 : S <= 8 ? 8 : 16;
   const int nt = (int)((N + 15) / 16);

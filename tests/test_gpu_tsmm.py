@@ -119,3 +119,19 @@ def test_unsupported_overlap_returns_false(gpu):
     assert gpu.dtsmm(0, 4, 4, 10, buf, 8, 1.0, B, 4, 0.0, buf, 4) is False
     # partially overlapping A and C
     assert gpu.dtsmm(0, 4, 4, 10, buf, 4, 1.0, B, 4, 0.0, buf[2:], 4) is False
+
+
+@pytest.mark.parametrize("M,N", [(4, 4), (8, 3), (16, 16), (33, 20), (64, 64)])
+def test_inplace_beta_uniform(gpu, orc, M, N):
+    """In place with beta != 0 and N <= M: beta * C joins the operand as
+    beta * I (C's old columns are A's own); non-integer alpha, beta within
+    the FP64 tolerance of the oracle's beta C + alpha A B."""
+    K = 1 << 14
+    note(f"tsmm in place beta uniform M={M} N={N} K={K}")
+    C = orc.uniform(K * M, 6, 0)
+    B = orc.uniform(M * N, 6, 1)
+    want = C.copy()
+    orc.tsmm(M, N, K, want, M, 0.75, B, M, -1.25, want, M)
+    ok, got = _run(gpu, 0, M, N, K, C, M, 0.75, B, M, -1.25, C, M, inplace=True)
+    assert ok
+    assert rel_fro(got, want) <= 8 * (M + 1) * EPS64
