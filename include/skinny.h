@@ -372,6 +372,12 @@ skinny_status_t skinny_mb_peak(int kind, int reps, double* tflops,
  * 4 KiB <= bytes <= 16 GiB for kinds 0 and 1. */
 skinny_status_t skinny_mb_cache(int kind, size_t bytes, int reps, double* out,
                                 skinny_stream_t stream);
+/* Microseconds per grid-wide barrier (the barrier of skinny_dsvqb_fused) in a
+ * cooperative grid of per_cu workgroups of 256 threads per CU; replaces
+ * global_sync/sync.cu:38-53.  SKINNY_UNSUPPORTED if per_cu workgroups do not
+ * fit on a CU. */
+skinny_status_t skinny_mb_barrier(int per_cu, int reps, double* us,
+                                  skinny_stream_t stream);
 
 #ifdef __cplusplus
 }

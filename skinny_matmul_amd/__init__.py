@@ -128,6 +128,7 @@ _SIGS = {
     "skinny_mb_stream": (C.c_int, [C.c_int, sz, C.c_int, C.POINTER(C.c_double), vp]),
     "skinny_mb_peak": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double), vp]),
     "skinny_mb_cache": (C.c_int, [C.c_int, sz, C.c_int, C.POINTER(C.c_double), vp]),
+    "skinny_mb_barrier": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double), vp]),
 }
 
 # every symbol include/skinny.h declares (checked by the CPU test suite)
@@ -554,6 +555,14 @@ def mb_cache(kind: str, nbytes: int = 1 << 20, reps: int = 3, stream=None) -> fl
     _check(lib().skinny_mb_cache(CACHE_KINDS[kind], nbytes, reps, C.byref(out),
                                  _stream(stream)), False)
     return out.value
+
+
+def mb_barrier(per_cu: int = 1, reps: int = 3, stream=None):
+    """Microseconds per grid-wide barrier with per_cu workgroups per CU
+    (global_sync/sync.cu); None if that many do not fit on a CU."""
+    out = C.c_double()
+    ok = _check(lib().skinny_mb_barrier(per_cu, reps, C.byref(out), _stream(stream)))
+    return out.value if ok else None
 
 
 def cu_count(handle=None) -> int:

@@ -15,7 +15,8 @@ The analogue of the reference's membw and roofline microbenchmarks
 
 * with --hierarchy: ns per dependent load over working sets from 4 KiB to
   1 GiB, GB/s re-reading buffers from 4 MiB to 8 GiB, LDS GB/s and latency
-  (the role of microbenches/instruction_rates.cu).
+  (the role of microbenches/instruction_rates.cu), and the cost of a
+  grid-wide barrier at 1-8 workgroups per CU (global_sync/sync.cu).
 
 These are the measured denominators of the roofline fractions; the datasheet
 figures (8 TB/s, 78.6 TF FP64, 5 POP/s INT8) are upper bounds only.
@@ -63,6 +64,14 @@ def hierarchy(reps: int = 3) -> dict:
         v = round(mb_cache("bandwidth", b, reps), 1)
         out["bandwidth_GBs"][str(b)] = v
         print(f"bandwidth {b / 2**20:12.0f} MiB {v:9.1f} GB/s", flush=True)
+    from . import mb_barrier
+    out["barrier_us"] = {}
+    for per in (1, 2, 4, 8):
+        v = mb_barrier(per, reps)
+        if v is None:
+            break
+        out["barrier_us"][str(per)] = round(v, 2)
+        print(f"grid barrier, {per} workgroup(s) per CU: {v:.2f} us", flush=True)
     out["lds_GBs"] = round(mb_cache("lds_bandwidth", 0, reps), 1)
     out["lds_latency_ns"] = round(mb_cache("lds_latency", 0, reps), 2)
     print(f"LDS {out['lds_GBs']:.1f} GB/s, {out['lds_latency_ns']:.2f} ns per dependent read",

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "grid_barrier.hpp"
 #include "internal.hpp"
 
 namespace skinny {
@@ -239,6 +240,14 @@ __global__ void __launch_bounds__(64) lds_chase_kernel(int steps,
   sink[0] = idx;
 }
 
+// Grid barriers back to back (the cost global_sync/sync.cu measures with its
+// atomic arrival counter): n barriers, then the exit that resets the counters.
+__global__ void __launch_bounds__(256)
+    barrier_kernel(unsigned* bar, int n, unsigned spin_limit) {
+  for (int i = 1; i <= n; i++) grid_barrier(bar, (unsigned)i * gridDim.x, spin_limit);
+  grid_exit(bar);
+}
+
 }  // namespace
 }  // namespace skinny
 
@@ -415,6 +424,47 @@ skinny_status_t skinny_mb_cache(int kind, size_t bytes, int reps,
   (void)hipFree(buf);
   (void)hipFree(sink);
   if (e != hipSuccess) return hip_fail(e, "skinny_mb_cache launch");
+  return SKINNY_OK;
+}
+
+// Microseconds per grid barrier for a cooperative grid of per_cu workgroups
+// of 256 threads per CU (1 <= per_cu <= the occupancy limit).
+skinny_status_t skinny_mb_barrier(int per_cu, int reps, double* us,
+                                  skinny_stream_t stream) {
+  if (!us || per_cu < 1 || reps < 1)
+    return fail(SKINNY_INVALID_ARG, "skinny_mb_barrier: bad arguments");
+  skinny_handle_t h = skinny_default_handle();
+  if (!h) return no_handle("microbench");
+  int maxper = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&maxper, barrier_kernel, 256, 0);
+  if (e != hipSuccess) return hip_fail(e, "skinny_mb_barrier occupancy");
+  if (per_cu > maxper)
+    return fail(SKINNY_UNSUPPORTED, "skinny_mb_barrier: more workgroups per CU than fit");
+  hipStream_t st = (hipStream_t)stream;
+  unsigned* bar = nullptr;
+  e = hipMalloc(&bar, 4 * sizeof(unsigned));
+  if (e == hipSuccess) e = hipMemsetAsync(bar, 0, 4 * sizeof(unsigned), st);
+  if (e != hipSuccess) {
+    (void)hipFree(bar);
+    return hip_fail(e, "skinny_mb_barrier alloc");
+  }
+  const int grid = per_cu * h->cu_count;
+  int n = 1000;
+  unsigned spin_limit = 1u << 25;
+  void* args[] = {&bar, &n, &spin_limit};
+  hipError_t le = hipSuccess;
+  const float ms = time_ms([&]() {
+    const hipError_t x = hipLaunchCooperativeKernel((const void*)barrier_kernel, dim3(grid),
+                                                    dim3(256), args, 0, st);
+    if (x != hipSuccess) le = x;
+  }, st, reps);
+  unsigned flag = 0;
+  e = le;
+  if (e == hipSuccess) e = hipMemcpy(&flag, bar + 2, sizeof(unsigned), hipMemcpyDeviceToHost);
+  (void)hipFree(bar);
+  if (e != hipSuccess) return hip_fail(e, "skinny_mb_barrier launch");
+  if (flag) return fail(SKINNY_HIP_ERROR, "skinny_mb_barrier: a barrier timed out");
+  *us = (double)ms * 1e3 / n;
   return SKINNY_OK;
 }
 

@@ -15,18 +15,17 @@
 // result does not depend on timing and is bitwise reproducible for a fixed
 // grid.
 //
-// The barrier: thread 0 of each workgroup releases its writes at agent
-// scope, adds 1 to a counter that only grows within a launch, and polls it
-// until it reaches (barrier number) x (grid size), then acquires.  The
-// cooperative launch guarantees that all workgroups are resident, so every
-// wait ends; a poll limit (about 2 s) still bounds it, flags the call as
-// failed and lets the workgroup go on, so a broken launch cannot hang the
-// device.  The last workgroup to leave resets the counters for the next
-// launch (no memset).
+// The barrier (grid_barrier.hpp): a monotonic arrival counter with agent-
+// scope release/acquire.  The cooperative launch guarantees that all
+// workgroups are resident, so every wait ends; a poll limit (about 2 s)
+// still bounds it, flags the call as failed and lets the workgroup go on,
+// so a broken launch cannot hang the device.  The last workgroup to leave
+// resets the counters for the next launch (no memset).
 #include <algorithm>
 #include <vector>
 
 #include "batch_items.hpp"
+#include "grid_barrier.hpp"
 #include "svqb_eig.hpp"
 
 namespace skinny {
@@ -40,25 +39,6 @@ struct FusedArgs {
   unsigned* bar;  // [0] arrivals, [1] exits, [2] poll-limit flag
   unsigned spin_limit;
 };
-
-__device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned target,
-                                             unsigned spin_limit) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned spins = 0;
-    while (__hip_atomic_load(&bar[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(2);
-      if (++spins > spin_limit) {
-        __hip_atomic_store(&bar[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-}
 
 constexpr int kFusedLds =
     kEigLdsDoubles > TsCore<double, 1, 4, 4, 2, false>::RED_ELEMS
@@ -104,16 +84,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
       run_tm_item(a.ops[it.op], it);
     }
   }
-  // leave; the last workgroup out resets the arrival and exit counters
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned t = __hip_atomic_fetch_add(&a.bar[1], 1u, __ATOMIC_ACQ_REL,
-                                              __HIP_MEMORY_SCOPE_AGENT);
-    if (t == G - 1) {
-      __hip_atomic_store(&a.bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&a.bar[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
+  grid_exit(a.bar);
 }
 
 }  // namespace skinny

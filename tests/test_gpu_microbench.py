@@ -39,3 +39,10 @@ def test_hierarchy_is_ordered(gpu):
 def test_bad_arguments_raise(gpu):
     with pytest.raises(gpu.SkinnyError):
         gpu.mb_cache("latency", 100, 1)
+
+
+def test_grid_barrier_cost(gpu):
+    note("microbench: grid barrier")
+    us = gpu.mb_barrier(1, 1)
+    assert us is not None and 0.1 < us < 1000.0, us
+    assert gpu.mb_barrier(1000, 1) is None      # cannot be resident: refused, not launched
