@@ -318,6 +318,13 @@ __global__ void __launch_bounds__(256) tsmm_mfma_kernel(TmArgs<T> a) {
   }
 }
 
+// Slice length per lane group (columns of A per MFMA k-lane) rounded up to
+// an instantiated class: 1, 2, 4, 6, 8, 12, 16.  6 and 12 keep widths
+// 21..24 and 41..48 from padding a third of their matrix-core steps.
+static int tm_slice_class(int S) {
+  return S <= 1 ? 1 : S <= 2 ? 2 : S <= 4 ? 4 : S <= 6 ? 6 : S <= 8 ? 8 : S <= 12 ? 12 : 16;
+}
+
 template <typename T, int S, int NT>
 static void launch_mfma_s_nt(const TmArgs<T>& a, int blocks, bool beta0,
                              bool vec, hipStream_t st) {
@@ -430,7 +437,7 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
       beta0 = true;
     }
     const int S = (int)((M + 3) / 4);
-    const int Sc = S <= 1 ? 1 : S <= 2 ? 2 : S <= 4 ? 4 : S <= 8 ? 8 : 16;
+    const int Sc = tm_slice_class(S);
     const int nt = (int)((N + 15) / 16);
     // unmasked 16-byte row slices need every slice start 16-byte aligned
     const bool vecm = sizeof(T) == 8 && (M % 4 == 0) && (S % 2 == 0) &&
@@ -439,7 +446,9 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
       case 1: launch_mfma_s<T, 1>(a, blocks, nt, beta0, false, stream); break;
       case 2: launch_mfma_s<T, 2>(a, blocks, nt, beta0, vecm, stream); break;
       case 4: launch_mfma_s<T, 4>(a, blocks, nt, beta0, vecm, stream); break;
+      case 6: launch_mfma_s<T, 6>(a, blocks, nt, beta0, vecm, stream); break;
       case 8: launch_mfma_s<T, 8>(a, blocks, nt, beta0, vecm, stream); break;
+      case 12: launch_mfma_s<T, 12>(a, blocks, nt, beta0, vecm, stream); break;
       default: launch_mfma_s<T, 16>(a, blocks, nt, beta0, vecm, stream); break;
     }
     hipError_t e = hipGetLastError();
@@ -504,7 +513,7 @@ skinny_status_t launch_tsmm_cview(skinny_handle_t h, size_t blockCount,
     beta0 = true;
   }
   const int S = (int)((M + 3) / 4);
-  const int Sc = S <= 1 ? 1 : S <= 2 ? 2 : S <= 4 ? 4 : S <= 8 ? 8 : 16;
+  const int Sc = tm_slice_class(S);
   const int nt = (int)((N + 15) / 16);
   const bool vecm = sizeof(T) == 8 && (M % 4 == 0) && (S % 2 == 0) &&
                     (Sc == S) && (a.lda % 2 == 0) && ((uintptr_t)A % 16 == 0);
@@ -512,7 +521,9 @@ skinny_status_t launch_tsmm_cview(skinny_handle_t h, size_t blockCount,
     case 1: launch_mfma_s<T, 1>(a, blocks, nt, beta0, false, stream); break;
     case 2: launch_mfma_s<T, 2>(a, blocks, nt, beta0, vecm, stream); break;
     case 4: launch_mfma_s<T, 4>(a, blocks, nt, beta0, vecm, stream); break;
+    case 6: launch_mfma_s<T, 6>(a, blocks, nt, beta0, vecm, stream); break;
     case 8: launch_mfma_s<T, 8>(a, blocks, nt, beta0, vecm, stream); break;
+    case 12: launch_mfma_s<T, 12>(a, blocks, nt, beta0, vecm, stream); break;
     default: launch_mfma_s<T, 16>(a, blocks, nt, beta0, vecm, stream); break;
   }
   hipError_t e = hipGetLastError();

@@ -325,9 +325,11 @@ TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
     G = te.G;
     p.uvar = te.uvar;
   } else {
-    // measured on MI355X at K = 2^26 (fp64): one workgroup per CU for tiles
-    // up to 16 x 16, two for the 32 x 32 and 64 x 64 tiles
-    int64_t per_cu = (p.MT * p.NT >= 4) ? 2 : 1;
+    // measured on MI355X (fp64): two workgroups per CU.  The autotuner
+    // picked two for most of a 12 x 12 grid of widths 1..64; one per CU
+    // left narrow asymmetric products (M = 12, N = 1: 3.1 TB/s) short of
+    // loads in flight.
+    int64_t per_cu = 2;
     int64_t target = (per_cu * h->cu_count) / ntiles;
     if (target < 1) target = 1;
     int64_t byrows = K / (CH * kWaves * 2);
