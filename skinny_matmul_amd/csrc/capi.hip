@@ -122,8 +122,9 @@ skinny_status_t tsmm_cplx(const char* fn, skinny_handle_t h, size_t blockCount,
   h = resolve(h);
   if (!h) return no_handle(fn);
   // real beta, 2..32 complex columns: the real matrix-core kernel on the
-  // interleaved view (alpha folded into the operand)
-  if (beta.y == 0 && M >= 2 && M <= 32 && N <= 32)
+  // interleaved view (alpha folded into the operand); M = N = 1: the real
+  // whole-row kernel on the same view
+  if (beta.y == 0 && ((M >= 2 && M <= 32 && N <= 32) || (M == 1 && N == 1)))
     return launch_tsmm_cview<T>(h, blockCount, M, N, K, (const T*)A, lda, alpha.x,
                                 alpha.y, (const T*)B, ldb, beta.x, (T*)C, ldc,
                                 (hipStream_t)stream);

@@ -115,8 +115,10 @@ skinny_status_t launch_tsmttsm_real(skinny_handle_t h, const TsmttsmPlan& p,
                                     int64_t ldc, hipStream_t stream);
 
 // ---- TSMM ---------------------------------------------------------------
+// Default grid of a TSMM call; real = false for the complex kernels, whose
+// whole-row variant covers only M, N <= 2 of the real view.
 int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
-                      int elem_bytes);
+                      int elem_bytes, bool real = true);
 
 template <typename T>
 skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,

@@ -54,8 +54,9 @@ struct TmArgs {
   int64_t K, lda, ldb, ldc;
   int M, N;
   T alpha, beta;
-  // complex view (tsmm_mfma_kernel only): B is complex M/2 x N/2 with ldb in
-  // complex units, alpha = 1 and (cre, cim) the complex alpha folded into Bh
+  // complex view (tsmm_mfma_kernel, tsmm_kernel with CPLX): B is complex
+  // M/2 x N/2 with ldb in complex units, alpha = 1 and (cre, cim) the
+  // complex alpha folded into Bh
   int cplxB;
   T cre, cim;
   // in place with beta != 0 and N <= M (tsmm_mfma_kernel only): the old C
@@ -104,7 +105,9 @@ struct TmRows {
   static constexpr int value = MC <= 2 ? 4 : (MC <= 8 ? 2 : 1);
 };
 
-template <typename T, int MC, bool BETA0, bool VEC>
+// CPLX: B is the real view Bh of a complex B (tm_bval), for complex
+// products one column wide (real view M = N = 2)
+template <typename T, int MC, bool BETA0, bool VEC, bool CPLX = false>
 __global__ void __launch_bounds__(256) tsmm_kernel(TmArgs<T> a) {
   constexpr int R = TmRows<MC>::value;
   const int M = a.M, N = a.N;
@@ -156,7 +159,8 @@ __global__ void __launch_bounds__(256) tsmm_kernel(TmArgs<T> a) {
 #pragma unroll
       for (int m = 0; m < MC; m++) {
         if (m < M) {
-          T x0 = b0[m], x1 = b1[m];
+          T x0 = CPLX ? tm_bval(a, m, n) : b0[m];
+          T x1 = CPLX ? tm_bval(a, m, n + 1) : b1[m];
 #pragma unroll
           for (int r = 0; r < R; r++) {
             s0[r] = fma_t(av[r][m], x0, s0[r]);
@@ -193,7 +197,7 @@ __global__ void __launch_bounds__(256) tsmm_kernel(TmArgs<T> a) {
 #pragma unroll
       for (int m = 0; m < MC; m++) {
         if (m < M) {
-          T x0 = b0[m];
+          T x0 = CPLX ? tm_bval(a, m, n) : b0[m];
 #pragma unroll
           for (int r = 0; r < R; r++) s0[r] = fma_t(av[r][m], x0, s0[r]);
         }
@@ -356,17 +360,34 @@ static void launch_mfma_s(const TmArgs<T>& a, int blocks, int nt, bool beta0,
   }
 }
 
+// Whole-row threads (tsmm_kernel) or the matrix cores (tsmm_mfma_kernel), for
+// M, N <= 64.  Measured on MI355X with K (M + N) = 2^28 doubles: a C at most
+// two columns wide uses 2 of the 16 lanes of each matrix-core store, and whole
+// rows win up to M = 6 (out of place: M = 5, N = 1 4246 -> 4999 GB/s) or
+// M = 8 (in place); three columns up to M = 3.  In place with beta != 0 and C
+// wider than A, the matrix-core kernel's old-C tile loads are mostly masked
+// lanes, and whole rows win for M <= 3 up to N = 4 (M = 1, N = 2: 1385 ->
+// 3793 GB/s).
+static bool tm_rows_wins(int64_t M, int64_t N, bool inplace, bool beta0) {
+  if (N <= 2) return M <= (inplace ? 8 : 6);
+  if (N == 3) return M <= 3;
+  if (inplace && !beta0 && N > M) return M <= 3 && N <= 4;
+  return false;
+}
+
 int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
-                      int elem_bytes) {
+                      int elem_bytes, bool real) {
   TuneEntry te;
   if (elem_bytes == 8 && tune_lookup(h, 1, 0, M, N, K, &te)) return te.G;
   // measured on MI355X at K = 2^26 (fp64): the one-row-per-thread kernel
-  // (M, N <= 2) wants four workgroups per CU for M = 1 and two for M = 2; the
-  // matrix-core kernel wants eight when A is at most 4 columns wide (few
-  // loads per lane) and two otherwise
+  // wants four workgroups per CU for M = 1 and two otherwise; the matrix-core
+  // kernel wants eight when A is at most 4 columns wide (few loads per lane)
+  // and two otherwise
   int64_t want = (K + 255) / 256;
   int64_t per_cu;
-  if (M <= 2 && N <= 2) per_cu = M <= 1 ? 4 : 2;
+  const bool rows = real ? (M <= 64 && N <= 64 && tm_rows_wins(M, N, false, true))
+                         : (M <= 2 && N <= 2);
+  if (rows) per_cu = M <= 1 ? 4 : 2;
   else per_cu = M <= 4 ? 8 : 2;
   int64_t cap = per_cu * h->cu_count;
   if (want > cap) want = cap;
@@ -374,25 +395,25 @@ int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
   return (int)want;
 }
 
-template <typename T, int MC, bool BETA0, bool VEC>
+template <typename T, int MC, bool BETA0, bool VEC, bool CPLX>
 static void launch_tm(const TmArgs<T>& a, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL((tsmm_kernel<T, MC, BETA0, VEC>), dim3(blocks), dim3(256),
-                     0, s, a);
+  hipLaunchKernelGGL((tsmm_kernel<T, MC, BETA0, VEC, CPLX>), dim3(blocks),
+                     dim3(256), 0, s, a);
 }
 
-template <typename T, int MC>
+template <typename T, int MC, bool CPLX = false>
 static void launch_mc(const TmArgs<T>& a, int blocks, bool beta0, bool vec,
                       hipStream_t s) {
   if (beta0) {
     if (vec)
-      launch_tm<T, MC, true, true>(a, blocks, s);
+      launch_tm<T, MC, true, true, CPLX>(a, blocks, s);
     else
-      launch_tm<T, MC, true, false>(a, blocks, s);
+      launch_tm<T, MC, true, false, CPLX>(a, blocks, s);
   } else {
     if (vec)
-      launch_tm<T, MC, false, true>(a, blocks, s);
+      launch_tm<T, MC, false, true, CPLX>(a, blocks, s);
     else
-      launch_tm<T, MC, false, false>(a, blocks, s);
+      launch_tm<T, MC, false, false, CPLX>(a, blocks, s);
   }
 }
 
@@ -425,9 +446,8 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
   int blocks = blockCount > 0 ? (int)blockCount
                               : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T));
   bool beta0 = (beta == T(0));
-  // the matrix-core kernel from 3 columns of A, and for M <= 2 once C is 4
-  // or more columns wide (its 16-column stores then beat whole-row threads)
-  if ((M >= 3 || N >= 4) && N <= 64) {
+  const bool inplace = (const void*)A == (const void*)C;
+  if (!tm_rows_wins(M, N, inplace, beta0)) {
     if (!beta0 && (const void*)A == (const void*)C && lda == ldc && N <= M) {
       a.fold = 1;
       a.fa = alpha;
@@ -481,8 +501,9 @@ skinny_status_t launch_tsmm_cview(skinny_handle_t h, size_t blockCount,
                                   int64_t ldb, T beta, T* C, int64_t ldc,
                                   hipStream_t stream) {
   const int64_t M = 2 * Mc, N = 2 * Nc;
-  if (Mc < 2 || M > 64 || N > 64)
-    return fail(SKINNY_UNSUPPORTED, "tsmm complex view: 2 <= M <= 32, N <= 32");
+  if ((Mc < 2 && !(Mc == 1 && Nc == 1)) || M > 64 || N > 64)
+    return fail(SKINNY_UNSUPPORTED,
+                "tsmm complex view: 2 <= M <= 32, N <= 32, or M = N = 1");
   TmArgs<T> a;
   a.A = A;
   a.B = B;
@@ -502,8 +523,18 @@ skinny_status_t launch_tsmm_cview(skinny_handle_t h, size_t blockCount,
   a.fa = T(1);
   a.fb = T(0);
   int blocks = blockCount > 0 ? (int)blockCount
-                              : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T));
+                              : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T), false);
   bool beta0 = (beta == T(0));
+  if (Mc == 1) {
+    // one complex column: the real view is 2 x 2, whole rows per thread
+    // (a row is one 16-byte complex value for double)
+    const bool vec = sizeof(T) == 8 && ((uintptr_t)A % 16 == 0) &&
+                     ((uintptr_t)C % 16 == 0);
+    launch_mc<T, 2, true>(a, blocks, beta0, vec, stream);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "tsmm (complex view) launch");
+    return SKINNY_OK;
+  }
   // in place, real beta, N <= M: beta I joins the real-view operand (the
   // complex alpha is already in it)
   if (!beta0 && (const void*)A == (const void*)C && lda == ldc && N <= M) {

@@ -165,6 +165,8 @@ inline std::string device_name() {
   HIP_CHECK(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+  // the marketing name is empty where libdrm's device table is not installed
+  if (prop.name[0] == '\0') return prop.gcnArchName;
   return std::string(prop.name) + " (" + prop.gcnArchName + ")";
 }
 

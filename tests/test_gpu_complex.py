@@ -112,6 +112,32 @@ def test_fuzz_ztsmm(gpu, orc, M):
     _fuzz_tsmm(gpu, orc, rng, M, rng.randint(1, 64), np.complex128, 4000)
 
 
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_tsmm_one_column_real_beta(gpu, orc, dt):
+    """M = N = 1 with a real beta runs on the real whole-row kernel over the
+    interleaved view; complex alpha, padded strides, in and out of place."""
+    rng = random.Random(91)
+    for inplace in (False, True):
+        for beta in (0.0, 1.0, -1.0):
+            for bc in (0, 7, 300):
+                lda = 1 + rng.randint(0, 2)
+                ldc = lda if inplace else 1 + rng.randint(0, 2)
+                K = rng.randint(2000, 9000)
+                alpha = complex(rng.randint(-2, 2), rng.randint(-2, 2))
+                seed = rng.randint(0, 1 << 30)
+                note(f"ztsmm one column {np.dtype(dt).name} K={K} lda={lda} ldc={ldc} "
+                     f"inplace={inplace} alpha={alpha} beta={beta} bc={bc} seed={seed}")
+                B = cint(orc, 1, seed, 1).astype(dt)
+                C = cint(orc, K * ldc, seed, 2).astype(dt)
+                A = C.copy() if inplace else cint(orc, K * lda, seed, 0).astype(dt)
+                want = C.copy()
+                orc.tsmm(1, 1, K, want if inplace else A, lda, alpha, B, 1, beta, want, ldc)
+                dC = dev(C)
+                dA = dC if inplace else dev(A)
+                assert gpu.tsmm(bc, 1, 1, K, dA, lda, alpha, dev(B), 1, beta, dC, ldc)
+                assert np.array_equal(host(dC), want), (K, lda, ldc, inplace, beta, bc)
+
+
 @pytest.mark.parametrize("M,N", [(1, 1), (4, 4), (8, 8), (16, 16), (32, 32), (5, 9)])
 def test_fuzz_complex64(gpu, orc, M, N):
     rng = random.Random(M * 13 + N)

@@ -80,6 +80,37 @@ def test_fuzz_rectangular_fp64(gpu, orc):
         _fuzz_one(gpu, orc, rng, rng.randint(1, 64), rng.randint(1, 64), np.float64, 20000)
 
 
+@pytest.mark.parametrize("inplace", [False, True])
+def test_narrow_grid_fp64(gpu, orc, inplace):
+    """Every (M, N) in 1..9 x 1..5, both sides of the switch between
+    whole-row threads and the matrix cores (which depends on M, N, in place
+    and beta), beta in {0, 1}, the default grid and a random one."""
+    rng = random.Random(77 + inplace)
+    for M in range(1, 10):
+        for N in range(1, 6):
+            for beta in (0.0, 1.0):
+                if inplace:
+                    ldc = max(M, N) + rng.randint(0, 2)
+                    lda = ldc
+                else:
+                    lda, ldc = M + rng.randint(0, 2), N + rng.randint(0, 2)
+                K = rng.randint(1000, 5000)
+                seed = rng.randint(0, 1 << 30)
+                for bc in (0, rng.randint(1, 300)):
+                    note(f"tsmm narrow M={M} N={N} K={K} lda={lda} ldc={ldc} "
+                         f"inplace={inplace} beta={beta} bc={bc} seed={seed}")
+                    B = orc.integers(N * M, seed, 1)
+                    C = orc.integers(K * ldc, seed, 2)
+                    A = C.copy() if inplace else orc.integers(K * lda, seed, 0)
+                    want = C.copy()
+                    orc.tsmm(M, N, K, want if inplace else A, lda, 1.0, B, M, beta,
+                             want, ldc)
+                    ok, got = _run(gpu, bc, M, N, K, A, lda, 1.0, B, M, beta, C, ldc,
+                                   inplace)
+                    assert ok
+                    assert np.array_equal(got, want), (M, N, K, lda, ldc, bc, beta)
+
+
 @pytest.mark.parametrize("M,N", [(1, 1), (4, 4), (8, 8), (5, 11), (16, 16), (32, 32), (64, 64)])
 def test_fuzz_fp32(gpu, orc, M, N):
     rng = random.Random(M * 17 + N)
