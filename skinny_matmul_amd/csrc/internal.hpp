@@ -92,6 +92,7 @@ struct TsmttsmPlan {
   int ngroups = 1;
   int outM = 1, outN = 1;  // entries of one partial (outM x outN)
   int uvar = 0;            // unroll variant: 0 base, 1 doubled
+  bool tuned = false;      // G and uvar come from the autotuner table
   size_t scratch_bytes = 0;
 };
 
@@ -126,8 +127,9 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
                                  int64_t lda, T alpha, const T* B, int64_t ldb,
                                  T beta, T* C, int64_t ldc, hipStream_t stream);
 
-// Complex TSMM with a real beta on the real matrix-core kernel (Mc, Nc
-// complex widths, 2 <= Mc, Nc <= 32; lda, ldb, ldc in complex units).
+// Complex TSMM with a real beta on the real kernels over the interleaved view
+// (Mc, Nc complex widths: 2 <= Mc <= 32 with Nc <= 32 on the matrix cores, or
+// Mc = Nc = 1 on whole rows; lda, ldb, ldc in complex units).
 template <typename T>
 skinny_status_t launch_tsmm_cview(skinny_handle_t h, size_t blockCount,
                                   int64_t Mc, int64_t Nc, int64_t K, const T* A,

@@ -324,6 +324,7 @@ TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
   } else if (elem_bytes == 8 && tune_lookup(h, 0, 0, M, N, K, &te)) {
     G = te.G;
     p.uvar = te.uvar;
+    p.tuned = true;
   } else {
     // measured on MI355X (fp64): two workgroups per CU.  The autotuner
     // picked two for most of a 12 x 12 grid of widths 1..64; one per CU
@@ -454,8 +455,13 @@ skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
   const bool self = (A == B) && (lda == ldb) && (M == N) && ntiles == 1;
   const int blocks = p.G * ntiles;
   a.two_pass = p.G > 1 ? 1 : 0;
-  bool ok = self ? dispatch<T, true>(p, a, blocks, stream)
-                 : dispatch<T, false>(p, a, blocks, stream);
+  // a Gram product loads one operand per step, half the loads in flight of
+  // the general product at the same unroll: untuned, it takes the doubled
+  // unroll (M = 24 at K = 2^22: 3.4 TB/s with the base unroll)
+  TsmttsmPlan q = p;
+  if (self && !p.tuned) q.uvar = 1;
+  bool ok = self ? dispatch<T, true>(q, a, blocks, stream)
+                 : dispatch<T, false>(q, a, blocks, stream);
   if (!ok) return fail(SKINNY_UNSUPPORTED, "tsmttsm: no kernel for this shape");
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "tsmttsm launch");

@@ -17,9 +17,19 @@
 
 namespace skinny {
 
+// SELF: a Gram product (B == A, one diagonal tile, MT == NT).  Only the
+// subtiles t <= u are accumulated; wg_partial mirrors them into the lower
+// ones.  Entry (i, j) of subtile (u, t) would be the same products in the
+// same order as entry (j, i) of subtile (t, u), so the mirror is bitwise
+// what the full computation gives, with MT (MT + 1) / 2 instead of MT^2
+// matrix-core steps per row group (10 of 16 at 64 columns).
 template <typename T, int P, int MT, int NT, int U, bool SELF>
 struct TsCore {
   using V4 = typename Mfma<T>::v4;
+  // the dispatch instantiates every (MT, NT) for SELF; only square ones run
+  static constexpr bool SYM = SELF && MT == NT;
+  // subtile (t, u) is accumulated
+  static constexpr bool live(int t, int u) { return !SYM || u >= t; }
   static constexpr int RS = 4 * P;   // rows per MFMA step
   static constexpr int CH = RS * U;  // rows per chunk
   static constexpr int DM = 16 * MT, DN = 16 * NT;
@@ -98,8 +108,9 @@ struct TsCore {
         for (int t = 0; t < MT; t++)
 #pragma unroll
           for (int u = 0; u < NT; u++)
-            acc[t][u] = Mfma<T>::mac(av[s][t], SELF ? av[s][u] : bv[s][u],
-                                     acc[t][u]);
+            if (live(t, u))
+              acc[t][u] = Mfma<T>::mac(av[s][t], SELF ? av[s][u] : bv[s][u],
+                                       acc[t][u]);
     }
 
     // ragged tail: the one partial chunk, taken by the wave whose turn it is
@@ -124,7 +135,7 @@ struct TsCore {
         for (int t = 0; t < MT; t++)
 #pragma unroll
           for (int u = 0; u < NT; u++)
-            acc[t][u] = Mfma<T>::mac(av[t], bv[u], acc[t][u]);
+            if (live(t, u)) acc[t][u] = Mfma<T>::mac(av[t], bv[u], acc[t][u]);
       }
     }
   }
@@ -151,7 +162,7 @@ struct TsCore {
         for (int u = 0; u < NT; u++)
 #pragma unroll
           for (int v = 0; v < 4; v++)
-            slot(wave == 2 ? red0 : red1, t, u, v) = acc[t][u][v];
+            if (live(t, u)) slot(wave == 2 ? red0 : red1, t, u, v) = acc[t][u][v];
     }
     __syncthreads();
     if (wave < 2) {
@@ -161,7 +172,7 @@ struct TsCore {
         for (int u = 0; u < NT; u++)
 #pragma unroll
           for (int v = 0; v < 4; v++)
-            acc[t][u][v] += slot(wave == 0 ? red0 : red1, t, u, v);
+            if (live(t, u)) acc[t][u][v] += slot(wave == 0 ? red0 : red1, t, u, v);
     }
     __syncthreads();
     if (wave == 1) {
@@ -170,7 +181,8 @@ struct TsCore {
 #pragma unroll
         for (int u = 0; u < NT; u++)
 #pragma unroll
-          for (int v = 0; v < 4; v++) slot(red0, t, u, v) = acc[t][u][v];
+          for (int v = 0; v < 4; v++)
+            if (live(t, u)) slot(red0, t, u, v) = acc[t][u][v];
     }
     __syncthreads();
     if (wave == 0) {
@@ -179,7 +191,8 @@ struct TsCore {
 #pragma unroll
         for (int u = 0; u < NT; u++)
 #pragma unroll
-          for (int v = 0; v < 4; v++) acc[t][u][v] += slot(red0, t, u, v);
+          for (int v = 0; v < 4; v++)
+            if (live(t, u)) acc[t][u][v] += slot(red0, t, u, v);
     }
     __syncthreads();
     // D tile as a DM x DN row-major matrix (MFMA D layout: lane l, element v
@@ -191,8 +204,13 @@ struct TsCore {
 #pragma unroll
         for (int u = 0; u < NT; u++)
 #pragma unroll
-          for (int v = 0; v < 4; v++)
+          for (int v = 0; v < 4; v++) {
+            if (!live(t, u)) continue;
             Dm[(16 * t + Mfma<T>::drow(lg, v)) * DN + 16 * u + li] = acc[t][u][v];
+            // SELF: the transposed copy fills subtile (u, t)
+            if (SYM && u > t)
+              Dm[(16 * u + li) * DN + 16 * t + Mfma<T>::drow(lg, v)] = acc[t][u][v];
+          }
     }
     __syncthreads();
     const int PS = outM * outN;

@@ -109,6 +109,41 @@ def test_uniform_fp64_vs_exact(gpu, orc, M):
     assert np.array_equal(got, again)
 
 
+@pytest.mark.parametrize("M", [2, 17, 32, 33, 48, 64])
+def test_gram_mirror_matches_general(gpu, orc, M):
+    """A Gram product (B is A) accumulates only the subtiles on and above the
+    diagonal and mirrors them.  On uniform data the result is exactly
+    symmetric and meets the exact oracle's bound; with the unroll and grid
+    pinned through the tuning table it equals the general kernel on a copy
+    of A bitwise (the same products summed in the same order)."""
+    import torch
+    K = 1 << 16
+    A = orc.uniform(K * M, 12, M)
+    dA, dA2 = dev(A), dev(A.copy())
+    hi, lo = orc.xtsmttsm(M, M, K, A, M, A, M)
+    h = gpu.Handle()
+    try:
+        for uvar, G in ((None, None), (0, 37), (1, 37), (1, 300)):
+            if uvar is not None:
+                gpu.tune_clear(handle=h)
+                gpu.tune_import(f"0 0 {M} {M} 16 {uvar} {G}\n", handle=h)
+            note(f"tsmttsm gram mirror M={M} K={K} uvar={uvar} G={G}")
+            out = []
+            for dB in (dA, dA2):
+                dC = dev(np.zeros(M * M))
+                assert gpu.dtsmttsm(0, M, M, K, dA, M, 1.0, dB, M, 0.0, dC, M, handle=h)
+                torch.cuda.synchronize()
+                out.append(host(dC))
+            gram, gen = out
+            G2 = gram.reshape(M, M)
+            assert np.array_equal(G2, G2.T)
+            assert rel_fro(gram - lo, hi) <= 8 * K * EPS64
+            if uvar is not None:
+                assert np.array_equal(gram, gen)
+    finally:
+        h.close()
+
+
 def test_c1_config(gpu, orc):
     """K = 2^16, M = N = 8, alpha = 1, beta = 0, seed 1 (streams 0 and 1)."""
     M = N = 8
