@@ -36,7 +36,9 @@ struct BatchItemDev {
 };
 
 // ---- TSMTTSM tile classes ------------------------------------------------
-// 0..5: packed P = 16, 8, 5, 4, 3, 2;  6..14: (MT, NT) in {1, 2, 4}^2
+// 0..5: packed P = 16, 8, 5, 4, 3, 2;  6..14: (MT, NT) in {1, 2, 4}^2;
+// 15..17: Gram products (B is A, M == N) of the square classes 6, 10, 14,
+// which accumulate the subtiles on and above the diagonal only
 static int ts_class(int64_t M, int64_t N) {
   const int64_t W = std::max(M, N);
   if (W <= 8) {
@@ -52,8 +54,12 @@ static int ts_class(int64_t M, int64_t N) {
   auto c3 = [](int64_t x) { return x <= 16 ? 0 : (x <= 32 ? 1 : 2); };
   return 6 + 3 * c3(M) + c3(N);
 }
+static int ts_gram_class(int cls) {
+  return cls == 6 ? 15 : cls == 10 ? 16 : cls == 14 ? 17 : cls;
+}
 static void ts_class_shape(int cls, int64_t M, int64_t N, int* outM, int* outN,
                            int* rows_min) {
+  if (cls >= 15) cls = cls == 15 ? 6 : cls == 16 ? 10 : 14;
   if (cls < 6) {
     static const int P[6] = {16, 8, 5, 4, 3, 2};
     *outM = (int)M;
@@ -68,12 +74,16 @@ static void ts_class_shape(int cls, int64_t M, int64_t N, int* outM, int* outN,
   *rows_min = 4 * 8;
 }
 
-template <int P, int MT, int NT>
+// SELF (a Gram class): one operand load per step, so twice the unroll of
+// the general product, at most 8 (chunks of at most 32 rows, the item
+// granularity)
+template <int P, int MT, int NT, bool SELF = false>
 __device__ __forceinline__ void ts_item(const BatchOpDev& o,
                                         const BatchItemDev& it, double* red,
                                         double* out) {
-  constexpr int U = P > 1 ? 8 : ((MT + NT) >= 8 ? 2 : ((MT + NT) >= 4 ? 4 : 8));
-  using Core = TsCore<double, P, MT, NT, U, false>;
+  constexpr int U0 = P > 1 ? 8 : ((MT + NT) >= 8 ? 2 : ((MT + NT) >= 4 ? 4 : 8));
+  constexpr int U = SELF ? (2 * U0 < 8 ? 2 * U0 : 8) : U0;
+  using Core = TsCore<double, P, MT, NT, U, SELF>;
   typename Core::V4 acc[MT][NT];
   Core::zero(acc);
   const int wave = threadIdx.x >> 6;
@@ -100,6 +110,9 @@ __device__ void run_ts_item(const BatchOpDev& o, const BatchItemDev& it,
     case 11: ts_item<1, 2, 4>(o, it, red, out); break;
     case 12: ts_item<1, 4, 1>(o, it, red, out); break;
     case 13: ts_item<1, 4, 2>(o, it, red, out); break;
+    case 15: ts_item<1, 1, 1, true>(o, it, red, out); break;
+    case 16: ts_item<1, 2, 2, true>(o, it, red, out); break;
+    case 17: ts_item<1, 4, 4, true>(o, it, red, out); break;
     default: ts_item<1, 4, 4>(o, it, red, out); break;
   }
 }
@@ -337,6 +350,7 @@ inline skinny_status_t batch_plan(int nops, const skinny_batch_op* ops,
         return fail(SKINNY_INVALID_ARG, "batch: leading dimension too small");
       int rows_min;
       d.cls = ts_class(o.M, o.N);
+      if (o.A == o.B && o.lda == o.ldb && o.M == o.N) d.cls = ts_gram_class(d.cls);
       ts_class_shape(d.cls, o.M, o.N, &d.outM, &d.outN, &rows_min);
       int64_t rows = std::max<int64_t>(item_bytes / row_bytes, rows_min);
       rows = std::max<int64_t>(rows, (o.K + 63) / 64);  // at most 64 partials

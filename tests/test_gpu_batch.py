@@ -76,6 +76,32 @@ def test_batch_integer_bitwise(gpu, orc):
     b.close()
 
 
+def test_batch_gram_ops(gpu, orc):
+    """Gram products (B is A) in a batch take the symmetric item classes:
+    integer data, bitwise against the oracle, symmetric results."""
+    rng = random.Random(5)
+    ops, refs = [], []
+    for M in (1, 8, 12, 16, 20, 32, 33, 48, 64):
+        K = rng.randint(3000, 40000)
+        lda = M + rng.randint(0, 3)
+        seed = rng.randint(0, 1 << 30)
+        A = orc.integers(K * lda, seed, 0)
+        C = orc.integers(M * M, seed, 2)
+        beta = float(rng.choice((0.0, 1.0)))
+        want = C.copy()
+        orc.tsmttsm(M, M, K, A, lda, 1.0, A, lda, beta, want, M)
+        dA, dC = dev(A), dev(C)
+        ops.append(("tsmttsm", M, M, K, dA, lda, 1.0, dA, lda, beta, dC, M))
+        refs.append((M, K, want, dC))
+    note(f"batch gram: {len(ops)} ops")
+    b = gpu.Batch(ops, item_bytes=1 << 18)
+    b.run()
+    for (M, K, want, dC) in refs:
+        got = host(dC)
+        assert np.array_equal(got, want), (M, K)
+    b.close()
+
+
 def test_batch_relaunch_and_reproducible(gpu, orc):
     rng = random.Random(7)
     ops, refs = _make_ops(orc, rng, 40, 40000, integer=False, beta_choices=(0.0,))
