@@ -117,9 +117,11 @@ skinny_status_t launch_tsmttsm_real(skinny_handle_t h, const TsmttsmPlan& p,
 
 // ---- TSMM ---------------------------------------------------------------
 // Default grid of a TSMM call; real = false for the complex kernels, whose
-// whole-row variant covers only M, N <= 2 of the real view.
+// whole-row variant covers only M, N <= 2 of the real view.  beta0 and
+// inplace select between the real kernels as the call does.
 int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
-                      int elem_bytes, bool real = true);
+                      int elem_bytes, bool real = true, bool beta0 = true,
+                      bool inplace = false);
 
 template <typename T>
 skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,

@@ -367,8 +367,14 @@ static void launch_mfma_s(const TmArgs<T>& a, int blocks, int nt, bool beta0,
 // M = 8 (in place); three columns up to M = 3.  In place with beta != 0 and C
 // wider than A, the matrix-core kernel's old-C tile loads are mostly masked
 // lanes, and whole rows win for M <= 3 up to N = 4 (M = 1, N = 2: 1385 ->
-// 3793 GB/s).
+// 3793 GB/s).  Out of place with beta != 0 the matrix-core kernel loads the
+// old C tiles with N of every 16 lanes, and whole rows win further: up to
+// M = 12 for N <= 2 and M = 7 for N <= 4 (M = 5, N = 1: 1990 -> 4932 GB/s).
 static bool tm_rows_wins(int64_t M, int64_t N, bool inplace, bool beta0) {
+  if (!beta0 && !inplace) {
+    if (N <= 2) return M <= 12;
+    if (N <= 4) return M <= 7 || (N == 3 && M <= 8);
+  }
   if (N <= 2) return M <= (inplace ? 8 : 6);
   if (N == 3) return M <= 3;
   if (inplace && !beta0 && N > M) return M <= 3 && N <= 4;
@@ -376,18 +382,18 @@ static bool tm_rows_wins(int64_t M, int64_t N, bool inplace, bool beta0) {
 }
 
 int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
-                      int elem_bytes, bool real) {
+                      int elem_bytes, bool real, bool beta0, bool inplace) {
   TuneEntry te;
   if (elem_bytes == 8 && tune_lookup(h, 1, 0, M, N, K, &te)) return te.G;
   // measured on MI355X at K = 2^26 (fp64): the one-row-per-thread kernel
-  // wants four workgroups per CU for M = 1 and two otherwise; the matrix-core
-  // kernel wants eight when A is at most 4 columns wide (few loads per lane)
-  // and two otherwise
+  // wants four workgroups per CU for M = 1 or beta != 0 (old C loaded too)
+  // and two otherwise; the matrix-core kernel wants eight when A is at most
+  // 4 columns wide (few loads per lane) and two otherwise
   int64_t want = (K + 255) / 256;
   int64_t per_cu;
-  const bool rows = real ? (M <= 64 && N <= 64 && tm_rows_wins(M, N, false, true))
+  const bool rows = real ? (M <= 64 && N <= 64 && tm_rows_wins(M, N, inplace, beta0))
                          : (M <= 2 && N <= 2);
-  if (rows) per_cu = M <= 1 ? 4 : 2;
+  if (rows) per_cu = (M <= 1 || !beta0) ? 4 : 2;
   else per_cu = M <= 4 ? 8 : 2;
   int64_t cap = per_cu * h->cu_count;
   if (want > cap) want = cap;
@@ -443,10 +449,11 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
   a.fold = 0;
   a.fa = T(1);
   a.fb = T(0);
-  int blocks = blockCount > 0 ? (int)blockCount
-                              : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T));
   bool beta0 = (beta == T(0));
   const bool inplace = (const void*)A == (const void*)C;
+  int blocks = blockCount > 0
+                   ? (int)blockCount
+                   : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T), true, beta0, inplace);
   if (!tm_rows_wins(M, N, inplace, beta0)) {
     if (!beta0 && (const void*)A == (const void*)C && lda == ldc && N <= M) {
       a.fold = 1;

@@ -15,6 +15,12 @@ product kernel dispatch is one call, and a TSMTTSM call may be followed by
 its partial-sum kernel (tsmttsm_reduce_kernel).  Prints a markdown table of
 average durations per call and GB/s of algorithmic bytes (perf.cu:209-231
 accounting: (M + N) K 8 bytes for either product).
+
+With --svqb the trace is of the whole benchmark (`python3 bench.py --steps 1
+--warmup 1 --no-cpu`) and the summary covers its SVQB measurement instead:
+the hipGraph replays of two sweeps over 64 blocks (K = 2^22, M = 8, 16, 32,
+64 in turn), i.e. the library's kernels between the batched launch and the
+first fused launch, per replay (one eigensolve launch per sweep).
 """
 from __future__ import annotations
 
@@ -27,8 +33,62 @@ REDUCE = "tsmttsm_reduce_kernel"
 
 
 def short(name: str) -> str:
-    name = name.replace("void skinny::", "")
+    for pre in ("void skinny::", "skinny::"):
+        if name.startswith(pre):
+            name = name[len(pre):]
     return name.split("(")[0]
+
+
+def load_trace(path: str):
+    """Kernel dispatches in start order, from a rocprofv3 kernel-trace CSV or
+    from its default SQLite output (`<prefix>_results.db`, `kernels` view).
+    A run with a cooperative launch can end the profiler before it writes the
+    CSV files; the database is written as the run goes."""
+    if path.endswith(".db"):
+        import sqlite3
+        con = sqlite3.connect(path)
+        rows = [{"Kernel_Name": n, "Start_Timestamp": s0, "End_Timestamp": e0}
+                for n, s0, e0 in con.execute("select name, start, end from kernels")]
+        con.close()
+    else:
+        with open(path, newline="") as f:
+            rows = [r for r in csv.DictReader(f)
+                    if r.get("Kind", "KERNEL_DISPATCH") == "KERNEL_DISPATCH"]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    return rows
+
+
+SVQB_KERNELS = ("tsmttsm_kernel", "tsmttsm_reduce_kernel", "tsmm_", "svqb_")
+
+
+def svqb_summary(rows) -> int:
+    names = [short(r["Kernel_Name"]) for r in rows]
+    last_batch = max((i for i, n in enumerate(names) if n.startswith("batch_kernel")),
+                     default=-1)
+    first_fused = next((i for i, n in enumerate(names)
+                        if n.startswith("svqb_fused_kernel")), len(names))
+    win = [(names[i], int(rows[i]["End_Timestamp"]) - int(rows[i]["Start_Timestamp"]))
+           for i in range(last_batch + 1, first_fused)
+           if names[i].startswith(SVQB_KERNELS) and "htsmttsm" not in names[i]]
+    neig = sum(1 for n, _ in win if n.startswith("svqb_eig_kernel"))
+    if neig < 2 or neig % 2:
+        print(f"found {neig} eigensolve launches, expected two per replay", file=sys.stderr)
+        return 1
+    reps = neig // 2
+    agg = {}
+    for n, ns in win:
+        c, t = agg.get(n, (0, 0))
+        agg[n] = (c + 1, t + ns)
+    print(f"{reps} replays of the two-sweep graph; per replay:\n")
+    print("| kernel | launches | ms | share |")
+    print("|---|---|---|---|")
+    total = sum(t for _, t in agg.values()) / reps
+    for n, (c, t) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print(f"| `{n}` | {c // reps} | {t / reps / 1e6:.3f} | {t / reps / total:.1%} |")
+    nbytes = sum(2 * 3 * M * (1 << 22) * 8 for M in (8, 16, 32, 64) for _ in range(16))
+    print(f"\nKernel time per replay: {total / 1e6:.3f} ms; {nbytes} algorithmic bytes "
+          f"(Gram K M 8 + update 2 K M 8 per block and sweep): {nbytes / total:.0f} GB/s.")
+    return 0
 
 
 def main() -> int:
@@ -37,11 +97,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--logK", type=int, default=26)
+    ap.add_argument("--svqb", action="store_true")
     a = ap.parse_args()
 
-    with open(a.trace, newline="") as f:
-        rows = [r for r in csv.DictReader(f) if r.get("Kind", "KERNEL_DISPATCH") == "KERNEL_DISPATCH"]
-    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    rows = load_trace(a.trace)
+    if a.svqb:
+        return svqb_summary(rows)
     disp = [(short(r["Kernel_Name"]), int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
             for r in rows
             if "tsmttsm" in r["Kernel_Name"] or "tsmm" in r["Kernel_Name"]]
