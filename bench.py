@@ -198,6 +198,15 @@ def extra_configs(sk, torch, logK: int) -> dict:
     del pool
     torch.cuda.empty_cache()
 
+    out["c5"] = svqb_config(sk, torch)
+    return out
+
+
+def svqb_config(sk, torch, fused: bool = True) -> dict:
+    """C5: two SVQB sweeps over 64 blocks (K = 2^22, M cycling 8, 16, 32,
+    64) replayed from a hipGraph, and (fused) the same sweeps as one
+    cooperative launch with grid barriers."""
+    dev = torch.device("cuda:0")
     # ---- C5: SVQB, 64 blocks, K = 2^22, two sweeps from a hipGraph -------
     log("C5 svqb")
     K5 = 1 << 22
@@ -225,24 +234,25 @@ def extra_configs(sk, torch, logK: int) -> dict:
     errs = [sk.orth_error(M5, K5, X, M5) for (M5, K5, X, _) in blocks]
     svqb_bytes = sum(2 * 3 * M5 * K5 * 8 for (M5, K5, _, _) in blocks)
     ms_graph = max(ms_all - ms_copy, 1e-6)
-    # the same two sweeps as one cooperative launch with grid barriers
-    # (synchronous call: its per-call buffers are included in the time)
-    def fused():
-        for X, X0 in bufs:
-            X.copy_(X0)
-        sk.dsvqb_fused(blocks, sweeps=2)
-    ms_fused = max(_time_ms(torch, fused, 3) - ms_copy, 1e-6)
-    errs_fused = [sk.orth_error(M5, K5, X, M5) for (M5, K5, X, _) in blocks]
-    out["c5"] = {"blocks": 64, "K": K5, "sweeps": 2, "ms": round(ms_graph, 3),
-                 "ms_per_block": round(ms_graph / 64, 4),
-                 "GBs": round(svqb_bytes / (ms_graph * 1e-3) / 1e9, 1),
-                 "max_orth_err_fro": max(errs),
-                 "fused_grid_barrier_ms": round(ms_fused, 3),
-                 "fused_max_orth_err_fro": max(errs_fused)}
+    c5 = {"blocks": 64, "K": K5, "sweeps": 2, "ms": round(ms_graph, 3),
+          "ms_per_block": round(ms_graph / 64, 4),
+          "GBs": round(svqb_bytes / (ms_graph * 1e-3) / 1e9, 1),
+          "max_orth_err_fro": max(errs)}
+    if fused:
+        # the same two sweeps as one cooperative launch with grid barriers
+        # (synchronous call: its per-call buffers are included in the time)
+        def fused_run():
+            for X, X0 in bufs:
+                X.copy_(X0)
+            sk.dsvqb_fused(blocks, sweeps=2)
+        ms_fused = max(_time_ms(torch, fused_run, 3) - ms_copy, 1e-6)
+        errs_fused = [sk.orth_error(M5, K5, X, M5) for (M5, K5, X, _) in blocks]
+        c5["fused_grid_barrier_ms"] = round(ms_fused, 3)
+        c5["fused_max_orth_err_fro"] = max(errs_fused)
     graph.close()
     del bufs, blocks
     torch.cuda.empty_cache()
-    return out
+    return c5
 
 
 def cpu_baseline(K_cpu: int, min_seconds: float, threads: int):
@@ -292,6 +302,10 @@ def main() -> int:
                     help="skip the untimed autotuning of each shape")
     ap.add_argument("--headline-only", action="store_true",
                     help="skip the C2-C5 measurements reported beside the headline")
+    ap.add_argument("--svqb-only", action="store_true",
+                    help="run only the C5 SVQB measurement and print its JSON")
+    ap.add_argument("--no-fused", action="store_true",
+                    help="with --svqb-only: skip the cooperative (fused) SVQB launch")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -306,6 +320,9 @@ def main() -> int:
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     sk.lib()
+    if args.svqb_only:
+        print(json.dumps({"c5": svqb_config(sk, torch, fused=not args.no_fused)}), flush=True)
+        return 0
     dev = torch.device("cuda:0")
     K = 1 << args.logK
     Mmax = max(WIDTHS)

@@ -16,11 +16,14 @@ its partial-sum kernel (tsmttsm_reduce_kernel).  Prints a markdown table of
 average durations per call and GB/s of algorithmic bytes (perf.cu:209-231
 accounting: (M + N) K 8 bytes for either product).
 
-With --svqb the trace is of the whole benchmark (`python3 bench.py --steps 1
---warmup 1 --no-cpu`) and the summary covers its SVQB measurement instead:
-the hipGraph replays of two sweeps over 64 blocks (K = 2^22, M = 8, 16, 32,
-64 in turn), i.e. the library's kernels between the batched launch and the
-first fused launch, per replay (one eigensolve launch per sweep).
+With --svqb the summary covers the SVQB measurement instead: the hipGraph
+replays of two sweeps over 64 blocks (K = 2^22, M = 8, 16, 32, 64 in turn),
+per replay (one eigensolve launch per sweep).  The trace is of
+
+    python3 bench.py --svqb-only --no-fused
+
+or of the whole benchmark, where the library's kernels between the batched
+launch and the first fused launch are taken.
 """
 from __future__ import annotations
 
