@@ -144,6 +144,36 @@ def test_gram_mirror_matches_general(gpu, orc, M):
         h.close()
 
 
+@pytest.mark.parametrize("fp", [np.float64, np.float32])
+def test_narrow_asymmetric(gpu, orc, fp):
+    """One width 1..4 and the other 5..16 (both orientations), where the
+    matrix-core tiles are mostly masked lanes: padded and unpadded leading
+    dimensions, integer data, bitwise against the oracle, with the default
+    grid and random ones, beta in {0, 1}."""
+    rng = random.Random(321 + (fp is np.float32))
+    shapes = [(lo, hi) for lo in (1, 2, 3, 4) for hi in (5, 8, 9, 12, 13, 16)]
+    shapes += [(hi, lo) for lo, hi in shapes]
+    for M, N in shapes:
+        cls = lambda w: 1 if w <= 1 else 2 if w <= 2 else 4 if w <= 4 else 8 if w <= 8 \
+            else 12 if w <= 12 else 16
+        for lda, ldb in ((M, N), (cls(M) + 4, cls(N) + rng.randint(0, 4)),
+                         (cls(M) + 5, N + 1)):
+            K = rng.randint(1, 3000) if rng.random() < 0.3 else rng.randint(3000, 70000)
+            beta = float(rng.randint(0, 1))
+            bc = rng.choice((0, rng.randint(1, 300)))
+            seed = rng.randint(0, 1 << 30)
+            note(f"tsmttsm narrow {np.dtype(fp).name} M={M} N={N} K={K} lda={lda} ldb={ldb} "
+                 f"bc={bc} beta={beta} seed={seed}")
+            A = orc.integers(K * lda, seed, 0).astype(fp)
+            B = orc.integers(K * ldb, seed, 1).astype(fp)
+            C = orc.integers(N * M, seed, 2).astype(fp)
+            want = C.copy()
+            orc.tsmttsm(M, N, K, A, lda, 1.0, B, ldb, beta, want, M)
+            ok, got = _run(gpu, bc, M, N, K, A, lda, 1.0, B, ldb, beta, C, M)
+            assert ok
+            assert np.array_equal(got, want), (M, N, K, lda, ldb, bc, beta)
+
+
 def test_c1_config(gpu, orc):
     """K = 2^16, M = N = 8, alpha = 1, beta = 0, seed 1 (streams 0 and 1)."""
     M = N = 8
