@@ -93,6 +93,7 @@ struct TsmttsmPlan {
   int outM = 1, outN = 1;  // entries of one partial (outM x outN)
   int uvar = 0;            // unroll variant: 0 base, 1 doubled
   bool tuned = false;      // G and uvar come from the autotuner table
+  bool fixedG = false;     // G given by the caller or the autotuner table
   size_t scratch_bytes = 0;
 };
 

@@ -281,6 +281,104 @@ __global__ void __launch_bounds__(256) tsmttsm_reduce_kernel(TsArgs<T> a, int E)
 }
 
 // ---------------------------------------------------------------------------
+// one wide operand (9..16 columns) and one narrow (1..8): paired rows
+// ---------------------------------------------------------------------------
+// Here a 16-lane matrix-core tile takes one row per step, and the kernel is
+// bound by the issue of load instructions (about 13 cycles per 64-lane load
+// per CU, whatever the active lanes), not by bytes: M = 9, N = 1 issues as
+// many loads as M = N = 16 and runs at 3.7 TB/s.  This kernel halves the
+// load instructions per row.  Lane (c, g) with c = 8 h + c' loads columns
+// 2c', 2c' + 1 of the wide operand's row 2g + h of an 8-row step with one
+// 16-byte load, and column c' of the narrow operand's same row.  Two MFMAs
+// take the even and the odd columns: D_p(i, j) = sum_g W(row, 2i' + p)
+// Q(row, j') is a valid partial product where i and j lie in the same half
+// (the same row), i.e. the 8 x 8 diagonal blocks; C(2c + p, q) = D_p(c, q) +
+// D_p(8 + c, 8 + q).  SWAP: B is the wide operand (C = (B^T A)^T).
+// ALIGNED: the wide rows allow 16-byte loads (even leading dimension,
+// aligned base); otherwise the pair is two 8-byte loads (3 loads per 8 rows
+// against 4 for the general kernel).
+template <int U, bool SWAP, bool ALIGNED>
+__global__ void __launch_bounds__(256) tsmttsm_pair_kernel(TsArgs<double> a) {
+  using V4 = Mfma<double>::v4;
+  typedef double d2v __attribute__((ext_vector_type(2)));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int h = li >> 3, c = li & 7;
+  const double* W = SWAP ? a.B : a.A;
+  const int64_t ldw = SWAP ? a.ldb : a.lda;
+  const int MW = SWAP ? a.N : a.M;
+  const double* Q = SWAP ? a.A : a.B;
+  const int64_t ldq = SWAP ? a.lda : a.ldb;
+  const int NQ = SWAP ? a.M : a.N;
+  const int rl = 2 * lg + h;  // the lane's row within an 8-row step
+  const bool w2 = 2 * c + 1 < MW, w1 = 2 * c < MW, qok = c < NQ;
+  const double* Wb = W + (int64_t)rl * ldw + 2 * c;
+  const double* Qb = Q + (int64_t)rl * ldq + c;
+  const int64_t K = a.K;
+  constexpr int CH = 8 * U;
+  const int64_t nfull = K / CH;
+  const int64_t w0 = (int64_t)blockIdx.x * 4 + wave, ws = (int64_t)gridDim.x * 4;
+  V4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+  auto ldw2 = [&](const double* p, bool ok) -> d2v {
+    if (ALIGNED) {
+      if (ok && w2) return __builtin_nontemporal_load(gmem(reinterpret_cast<const d2v*>(p)));
+      return d2v{(ok && w1) ? ld_stream(p) : 0.0, 0.0};
+    }
+    return d2v{(ok && w1) ? ld_stream(p) : 0.0, (ok && w2) ? ld_stream(p + 1) : 0.0};
+  };
+  for (int64_t ch = w0; ch < nfull; ch += ws) {
+    const double* pw = Wb + ch * CH * ldw;
+    const double* pq = Qb + ch * CH * ldq;
+    d2v wv[U];
+    double qv[U];
+#pragma unroll
+    for (int s = 0; s < U; s++) {
+      wv[s] = ldw2(pw + (int64_t)s * 8 * ldw, true);
+      qv[s] = qok ? ld_stream(pq + (int64_t)s * 8 * ldq) : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < U; s++) {
+      acc0 = Mfma<double>::mac(wv[s].x, qv[s], acc0);
+      acc1 = Mfma<double>::mac(wv[s].y, qv[s], acc1);
+    }
+  }
+  // the one partial chunk, taken by the wave whose turn it is
+  if (nfull * CH < K && (nfull % ws) == w0) {
+    const int64_t r0 = nfull * CH;
+#pragma unroll
+    for (int s = 0; s < U; s++) {
+      const bool ok = r0 + s * 8 + rl < K;
+      const d2v wv = ldw2(Wb + (r0 + s * 8) * ldw, ok);
+      const double qv = (ok && qok) ? ld_stream(Qb + (r0 + s * 8) * ldq) : 0.0;
+      acc0 = Mfma<double>::mac(wv.x, qv, acc0);
+      acc1 = Mfma<double>::mac(wv.y, qv, acc1);
+    }
+  }
+  // the four waves' D tiles in LDS: red[wave][p][lane][v]
+  __shared__ double red[4 * 2 * 64 * 4];
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    red[((wave * 2 + 0) * 64 + lane) * 4 + v] = acc0[v];
+    red[((wave * 2 + 1) * 64 + lane) * 4 + v] = acc1[v];
+  }
+  __syncthreads();
+  // D_p(i, j) sits in lane 16 (i % 4) + j, element i / 4
+  auto dsum = [&](int p, int i, int j) -> double {
+    const int at = (16 * (i & 3) + j) * 4 + (i >> 2);
+    const double* r = red + p * 256 + at;
+    return (r[0] + r[2 * 512]) + (r[512] + r[3 * 512]);
+  };
+  const int M = a.M, N = a.N, PS = M * N;
+  double* out = a.part1 + (int64_t)blockIdx.x * PS;
+  for (int e = threadIdx.x; e < PS; e += 256) {
+    const int m = e % M, n = e / M;
+    const int wi = SWAP ? n : m, qi = SWAP ? m : n;
+    const int p = wi & 1, cw = wi >> 1;
+    *gmem(out + e) = dsum(p, cw, qi) + dsum(p, 8 + cw, 8 + qi);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // planning
 // ---------------------------------------------------------------------------
 static int isqrt_ceil(int x) {
@@ -321,10 +419,12 @@ TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
   TuneEntry te;
   if (blockCount > 0) {
     G = (int64_t)blockCount;
+    p.fixedG = true;
   } else if (elem_bytes == 8 && tune_lookup(h, 0, 0, M, N, K, &te)) {
     G = te.G;
     p.uvar = te.uvar;
     p.tuned = true;
+    p.fixedG = true;
   } else {
     // measured on MI355X (fp64): two workgroups per CU.  The autotuner
     // picked two for most of a 12 x 12 grid of widths 1..64; one per CU
@@ -403,6 +503,77 @@ skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
                                int64_t ldb, T beta, T beta_i, T* C,
                                int64_t ldc, bool cplx, hipStream_t stream);
 
+// FP64 products with one operand 9..16 columns wide and the other at most 8
+// run on tsmttsm_pair_kernel.
+static bool pair_wins(int64_t M, int64_t N, const void* A, int64_t lda, const void* B,
+                      int64_t ldb) {
+  const bool wa = M >= 9 && M <= 16 && N <= 8, wb = N >= 9 && N <= 16 && M <= 8;
+  return wa || wb;
+}
+
+static skinny_status_t launch_tsmttsm_pair(skinny_handle_t h, const TsmttsmPlan& p,
+                                           int64_t M, int64_t N, int64_t K, const double* A,
+                                           int64_t lda, double alpha, const double* B,
+                                           int64_t ldb, double beta, double* C, int64_t ldc,
+                                           hipStream_t stream) {
+  constexpr int U = 8;
+  int64_t G = p.G;
+  if (!p.fixedG) {
+    // two workgroups per CU, no more than two 8 U-row chunks per wave
+    G = 2 * (int64_t)h->cu_count;
+    const int64_t byrows = K / (8 * U * 4 * 2);
+    if (G > byrows) G = byrows;
+    if (G < 1) G = 1;
+  }
+  const int PS = (int)(M * N);
+  Workspace w;
+  skinny_status_t st = acquire_workspace(h, stream, (size_t)G * PS * sizeof(double), &w);
+  if (st != SKINNY_OK) return st;
+  TsArgs<double> a{};
+  a.A = A;
+  a.B = B;
+  a.C = C;
+  a.K = K;
+  a.lda = lda;
+  a.ldb = ldb;
+  a.ldc = ldc;
+  a.M = (int)M;
+  a.N = (int)N;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.cplx = 0;
+  a.part1 = static_cast<double*>(ws_scratch(w));
+  a.part2 = a.part1;
+  a.cnt = ws_counters(w);
+  a.G = (int)G;
+  a.GS = (int)G;
+  a.ngroups = 1;
+  a.tilesM = a.tilesN = 1;
+  a.outM = (int)M;
+  a.outN = (int)N;
+  a.two_pass = 1;
+  const bool swap = M < 9;
+  const bool aligned = swap ? (ldb % 2 == 0 && (uintptr_t)B % 16 == 0)
+                            : (lda % 2 == 0 && (uintptr_t)A % 16 == 0);
+  if (!swap && aligned)
+    hipLaunchKernelGGL((tsmttsm_pair_kernel<U, false, true>), dim3(G), dim3(256), 0, stream, a);
+  else if (!swap)
+    hipLaunchKernelGGL((tsmttsm_pair_kernel<U, false, false>), dim3(G), dim3(256), 0, stream, a);
+  else if (aligned)
+    hipLaunchKernelGGL((tsmttsm_pair_kernel<U, true, true>), dim3(G), dim3(256), 0, stream, a);
+  else
+    hipLaunchKernelGGL((tsmttsm_pair_kernel<U, true, false>), dim3(G), dim3(256), 0, stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "tsmttsm launch");
+  int E = G <= 32 ? 64 : 16;
+  while (E > 1 && E / 2 >= PS) E >>= 1;
+  const int nsl = (PS + E - 1) / E;
+  hipLaunchKernelGGL(tsmttsm_reduce_kernel<double>, dim3(nsl), dim3(256), 0, stream, a, E);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "tsmttsm reduce launch");
+  return SKINNY_OK;
+}
+
 template <typename T>
 skinny_status_t launch_tsmttsm_real(skinny_handle_t h, const TsmttsmPlan& p,
                                     int64_t M, int64_t N, int64_t K,
@@ -419,6 +590,10 @@ skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
                                int64_t lda, T alpha, T alpha_i, const T* B,
                                int64_t ldb, T beta, T beta_i, T* C,
                                int64_t ldc, bool cplx, hipStream_t stream) {
+  if (!cplx && sizeof(T) == 8 && pair_wins(M, N, A, lda, B, ldb))
+    return launch_tsmttsm_pair(h, p, M, N, K, (const double*)A, lda, (double)alpha,
+                               (const double*)B, ldb, (double)beta, (double*)C, ldc,
+                               stream);
   const int ntiles = p.tilesM * p.tilesN;
   if ((size_t)ntiles * (p.ngroups + 1) > (size_t)kMaxCounters)
     return fail(SKINNY_UNSUPPORTED, "tsmttsm: too many partial groups");

@@ -146,12 +146,15 @@ def test_gram_mirror_matches_general(gpu, orc, M):
 
 @pytest.mark.parametrize("fp", [np.float64, np.float32])
 def test_narrow_asymmetric(gpu, orc, fp):
-    """One width 1..4 and the other 5..16 (both orientations), where the
-    matrix-core tiles are mostly masked lanes: padded and unpadded leading
-    dimensions, integer data, bitwise against the oracle, with the default
-    grid and random ones, beta in {0, 1}."""
+    """One width 1..8 and the other 5..16 (both orientations), where the
+    matrix-core tiles are mostly masked lanes; FP64 products with a 9..16
+    wide operand of even leading dimension pair rows in 16-byte loads.
+    Padded and unpadded leading dimensions (odd ones take the general
+    kernel), integer data, bitwise against the oracle, with the default grid
+    and random ones, beta in {0, 1}."""
     rng = random.Random(321 + (fp is np.float32))
     shapes = [(lo, hi) for lo in (1, 2, 3, 4) for hi in (5, 8, 9, 12, 13, 16)]
+    shapes += [(lo, hi) for lo in (5, 7, 8) for hi in (9, 10, 16)]
     shapes += [(hi, lo) for lo, hi in shapes]
     for M, N in shapes:
         cls = lambda w: 1 if w <= 1 else 2 if w <= 2 else 4 if w <= 4 else 8 if w <= 8 \
