@@ -296,26 +296,30 @@ __global__ void __launch_bounds__(256) tsmttsm_reduce_kernel(TsArgs<T> a, int E)
 // D_p(8 + c, 8 + q).  SWAP: B is the wide operand (C = (B^T A)^T).
 // ALIGNED: the wide rows allow 16-byte loads (even leading dimension,
 // aligned base); otherwise the pair is two 8-byte loads (3 loads per 8 rows
-// against 4 for the general kernel).
-template <int U, bool SWAP, bool ALIGNED>
+// against 4 for the general kernel).  R = 4 is the same for a wide operand
+// of 5..8 columns and a narrow one of at most 4: four rows per 16-lane group
+// (lanes 4h + c'), the 4 x 4 diagonal blocks summed, 2 loads per 16 rows
+// where the packed general kernel needs 2 per 8 or 12.
+template <int U, int R, bool SWAP, bool ALIGNED>
 __global__ void __launch_bounds__(256) tsmttsm_pair_kernel(TsArgs<double> a) {
+  constexpr int L = 16 / R;  // lanes per row
   using V4 = Mfma<double>::v4;
   typedef double d2v __attribute__((ext_vector_type(2)));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lg = lane >> 4;
-  const int h = li >> 3, c = li & 7;
+  const int h = li / L, c = li % L;
   const double* W = SWAP ? a.B : a.A;
   const int64_t ldw = SWAP ? a.ldb : a.lda;
   const int MW = SWAP ? a.N : a.M;
   const double* Q = SWAP ? a.A : a.B;
   const int64_t ldq = SWAP ? a.lda : a.ldb;
   const int NQ = SWAP ? a.M : a.N;
-  const int rl = 2 * lg + h;  // the lane's row within an 8-row step
+  const int rl = R * lg + h;  // the lane's row within a 4 R-row step
   const bool w2 = 2 * c + 1 < MW, w1 = 2 * c < MW, qok = c < NQ;
   const double* Wb = W + (int64_t)rl * ldw + 2 * c;
   const double* Qb = Q + (int64_t)rl * ldq + c;
   const int64_t K = a.K;
-  constexpr int CH = 8 * U;
+  constexpr int CH = 4 * R * U;
   const int64_t nfull = K / CH;
   const int64_t w0 = (int64_t)blockIdx.x * 4 + wave, ws = (int64_t)gridDim.x * 4;
   V4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
@@ -333,8 +337,8 @@ __global__ void __launch_bounds__(256) tsmttsm_pair_kernel(TsArgs<double> a) {
     double qv[U];
 #pragma unroll
     for (int s = 0; s < U; s++) {
-      wv[s] = ldw2(pw + (int64_t)s * 8 * ldw, true);
-      qv[s] = qok ? ld_stream(pq + (int64_t)s * 8 * ldq) : 0.0;
+      wv[s] = ldw2(pw + (int64_t)s * 4 * R * ldw, true);
+      qv[s] = qok ? ld_stream(pq + (int64_t)s * 4 * R * ldq) : 0.0;
     }
 #pragma unroll
     for (int s = 0; s < U; s++) {
@@ -347,9 +351,9 @@ __global__ void __launch_bounds__(256) tsmttsm_pair_kernel(TsArgs<double> a) {
     const int64_t r0 = nfull * CH;
 #pragma unroll
     for (int s = 0; s < U; s++) {
-      const bool ok = r0 + s * 8 + rl < K;
-      const d2v wv = ldw2(Wb + (r0 + s * 8) * ldw, ok);
-      const double qv = (ok && qok) ? ld_stream(Qb + (r0 + s * 8) * ldq) : 0.0;
+      const bool ok = r0 + s * 4 * R + rl < K;
+      const d2v wv = ldw2(Wb + (r0 + s * 4 * R) * ldw, ok);
+      const double qv = (ok && qok) ? ld_stream(Qb + (r0 + s * 4 * R) * ldq) : 0.0;
       acc0 = Mfma<double>::mac(wv.x, qv, acc0);
       acc1 = Mfma<double>::mac(wv.y, qv, acc1);
     }
@@ -374,7 +378,10 @@ __global__ void __launch_bounds__(256) tsmttsm_pair_kernel(TsArgs<double> a) {
     const int m = e % M, n = e / M;
     const int wi = SWAP ? n : m, qi = SWAP ? m : n;
     const int p = wi & 1, cw = wi >> 1;
-    *gmem(out + e) = dsum(p, cw, qi) + dsum(p, 8 + cw, 8 + qi);
+    double v = dsum(p, cw, qi);
+#pragma unroll
+    for (int hh = 1; hh < R; hh++) v += dsum(p, L * hh + cw, L * hh + qi);
+    *gmem(out + e) = v;
   }
 }
 
@@ -504,11 +511,19 @@ skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
                                int64_t ldc, bool cplx, hipStream_t stream);
 
 // FP64 products with one operand 9..16 columns wide and the other at most 8
-// run on tsmttsm_pair_kernel.
-static bool pair_wins(int64_t M, int64_t N, const void* A, int64_t lda, const void* B,
-                      int64_t ldb) {
-  const bool wa = M >= 9 && M <= 16 && N <= 8, wb = N >= 9 && N <= 16 && M <= 8;
-  return wa || wb;
+// (two rows per lane group), or 5..8 wide and at most 4 (four rows) run on
+// tsmttsm_pair_kernel; returns the rows per group, 0 for the general kernel.
+// A 5-wide operand with odd rows stays on the general kernel: its packing (3
+// rows in 15 lanes) needs fewer loads per row than two 8-byte loads per
+// pair (measured: M = 5, N = 1 5762 against 5315 GB/s).
+static int pair_rows(int64_t M, int64_t N, const void* A, int64_t lda, const void* B,
+                     int64_t ldb) {
+  const int64_t lo = M < N ? M : N, hi = M < N ? N : M;
+  const bool aligned = M < N ? (ldb % 2 == 0 && (uintptr_t)B % 16 == 0)
+                             : (lda % 2 == 0 && (uintptr_t)A % 16 == 0);
+  if (hi >= 9 && hi <= 16 && lo <= 8) return 2;
+  if (hi >= 5 && hi <= 8 && lo <= 4 && (hi > 5 || aligned)) return 4;
+  return 0;
 }
 
 static skinny_status_t launch_tsmttsm_pair(skinny_handle_t h, const TsmttsmPlan& p,
@@ -517,11 +532,12 @@ static skinny_status_t launch_tsmttsm_pair(skinny_handle_t h, const TsmttsmPlan&
                                            int64_t ldb, double beta, double* C, int64_t ldc,
                                            hipStream_t stream) {
   constexpr int U = 8;
+  const int R = pair_rows(M, N, A, lda, B, ldb);
   int64_t G = p.G;
   if (!p.fixedG) {
-    // two workgroups per CU, no more than two 8 U-row chunks per wave
+    // two workgroups per CU, no more than two 4 R U-row chunks per wave
     G = 2 * (int64_t)h->cu_count;
-    const int64_t byrows = K / (8 * U * 4 * 2);
+    const int64_t byrows = K / (4 * R * U * 4 * 2);
     if (G > byrows) G = byrows;
     if (G < 1) G = 1;
   }
@@ -552,17 +568,16 @@ static skinny_status_t launch_tsmttsm_pair(skinny_handle_t h, const TsmttsmPlan&
   a.outM = (int)M;
   a.outN = (int)N;
   a.two_pass = 1;
-  const bool swap = M < 9;
+  const bool swap = M < N;  // B is the wide operand
   const bool aligned = swap ? (ldb % 2 == 0 && (uintptr_t)B % 16 == 0)
                             : (lda % 2 == 0 && (uintptr_t)A % 16 == 0);
-  if (!swap && aligned)
-    hipLaunchKernelGGL((tsmttsm_pair_kernel<U, false, true>), dim3(G), dim3(256), 0, stream, a);
-  else if (!swap)
-    hipLaunchKernelGGL((tsmttsm_pair_kernel<U, false, false>), dim3(G), dim3(256), 0, stream, a);
-  else if (aligned)
-    hipLaunchKernelGGL((tsmttsm_pair_kernel<U, true, true>), dim3(G), dim3(256), 0, stream, a);
-  else
-    hipLaunchKernelGGL((tsmttsm_pair_kernel<U, true, false>), dim3(G), dim3(256), 0, stream, a);
+#define SK_PAIR(r, sw, al)                                                            \
+  if (R == (r) && swap == (sw) && aligned == (al))                                   \
+    hipLaunchKernelGGL((tsmttsm_pair_kernel<U, r, sw, al>), dim3(G), dim3(256), 0, stream, a);
+  SK_PAIR(2, false, true) SK_PAIR(2, false, false) SK_PAIR(2, true, true)
+  SK_PAIR(2, true, false) SK_PAIR(4, false, true) SK_PAIR(4, false, false)
+  SK_PAIR(4, true, true) SK_PAIR(4, true, false)
+#undef SK_PAIR
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "tsmttsm launch");
   int E = G <= 32 ? 64 : 16;
@@ -590,7 +605,7 @@ skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
                                int64_t lda, T alpha, T alpha_i, const T* B,
                                int64_t ldb, T beta, T beta_i, T* C,
                                int64_t ldc, bool cplx, hipStream_t stream) {
-  if (!cplx && sizeof(T) == 8 && pair_wins(M, N, A, lda, B, ldb))
+  if (!cplx && sizeof(T) == 8 && pair_rows(M, N, A, lda, B, ldb) > 0)
     return launch_tsmttsm_pair(h, p, M, N, K, (const double*)A, lda, (double)alpha,
                                (const double*)B, ldb, (double)beta, (double*)C, ldc,
                                stream);
