@@ -171,7 +171,12 @@ int skinny_tuned_blockcount_tsmttsm(skinny_handle_t h, int64_t M, int64_t N,
                                     int64_t K, int elem_bytes) {
   h = resolve(h);
   if (!h || M <= 0 || N <= 0) return 0;
-  return plan_tsmttsm(h, 0, M, N, K, elem_bytes, elem_bytes).G;
+  const TsmttsmPlan p = plan_tsmttsm(h, 0, M, N, K, elem_bytes, elem_bytes);
+  if (!p.fixedG && elem_bytes == 8) {
+    const int g = tsmttsm_pair_default_blocks(h, M, N, K);
+    if (g > 0) return g;
+  }
+  return p.G;
 }
 
 int skinny_tuned_blockcount_tsmm(skinny_handle_t h, int64_t M, int64_t N,

@@ -100,6 +100,10 @@ struct TsmttsmPlan {
 TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
                          int64_t N, int64_t K, int elem_bytes, int partial_bytes);
 
+// Default grid of a real FP64 TSMTTSM that runs on the paired-row kernel
+// (one operand 5..16 wide, the other narrow), 0 for other shapes.
+int tsmttsm_pair_default_blocks(skinny_handle_t h, int64_t M, int64_t N, int64_t K);
+
 // complex (cplx = true): A, B, C point at interleaved (re, im) data and M, N,
 // lda, ldb are the real-view sizes (2x the complex ones); ldc stays complex.
 template <typename T>

@@ -526,21 +526,32 @@ static int pair_rows(int64_t M, int64_t N, const void* A, int64_t lda, const voi
   return 0;
 }
 
+constexpr int kPairUnroll = 8;
+
+// Default grid of tsmttsm_pair_kernel: two workgroups per CU, no more than
+// two chunks of 4 R U rows per wave.  Never above the general kernel's
+// default for the same shape, so its workspace size covers this one.
+static int64_t pair_grid(skinny_handle_t h, int R, int64_t K) {
+  int64_t G = 2 * (int64_t)h->cu_count;
+  const int64_t byrows = K / (4 * R * kPairUnroll * 4 * 2);
+  if (G > byrows) G = byrows;
+  return G < 1 ? 1 : G;
+}
+
+int tsmttsm_pair_default_blocks(skinny_handle_t h, int64_t M, int64_t N, int64_t K) {
+  // rows of 16-byte aligned, even leading dimension assumed
+  const int R = pair_rows(M, N, nullptr, 2, nullptr, 2);
+  return R > 0 ? (int)pair_grid(h, R, K) : 0;
+}
+
 static skinny_status_t launch_tsmttsm_pair(skinny_handle_t h, const TsmttsmPlan& p,
                                            int64_t M, int64_t N, int64_t K, const double* A,
                                            int64_t lda, double alpha, const double* B,
                                            int64_t ldb, double beta, double* C, int64_t ldc,
                                            hipStream_t stream) {
-  constexpr int U = 8;
+  constexpr int U = kPairUnroll;
   const int R = pair_rows(M, N, A, lda, B, ldb);
-  int64_t G = p.G;
-  if (!p.fixedG) {
-    // two workgroups per CU, no more than two 4 R U-row chunks per wave
-    G = 2 * (int64_t)h->cu_count;
-    const int64_t byrows = K / (4 * R * U * 4 * 2);
-    if (G > byrows) G = byrows;
-    if (G < 1) G = 1;
-  }
+  const int64_t G = p.fixedG ? p.G : pair_grid(h, R, K);
   const int PS = (int)(M * N);
   Workspace w;
   skinny_status_t st = acquire_workspace(h, stream, (size_t)G * PS * sizeof(double), &w);
