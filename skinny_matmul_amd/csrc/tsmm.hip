@@ -224,7 +224,12 @@ struct TmBlocks {
   static constexpr int value = S >= 16 ? 1 : (S >= 8 ? 2 : (S >= 4 ? 4 : 8));
 };
 
-template <typename T, int S, int NT, bool BETA0, bool VEC>
+// TR: the operands swap places and the MFMA produces the transposed tile:
+// lane (i, g) then holds C(row0 + i, 16 u + drow(g, v)), so one store
+// instruction writes 16 whole rows of up to 4 columns from all 64 lanes,
+// where the untransposed tile needs 4 stores with N of every 16 lanes (for
+// C at most 8 wide: 1 or 2 stores per 16 rows against 4).
+template <typename T, int S, int NT, bool BETA0, bool VEC, bool TR = false>
 __global__ void __launch_bounds__(256) tsmm_mfma_kernel(TmArgs<T> a) {
   using V4 = typename Mfma<T>::v4;
   // with beta != 0 the old C tiles of all RB blocks are loaded with A (at
@@ -289,8 +294,8 @@ __global__ void __launch_bounds__(256) tsmm_mfma_kernel(TmArgs<T> a) {
         for (int u = 0; u < NT; u++)
 #pragma unroll
           for (int v = 0; v < 4; v++) {
-            const int64_t row = (b0 + r) * 16 + Mfma<T>::drow(lg, v);
-            const int col = 16 * u + li;
+            const int64_t row = (b0 + r) * 16 + (TR ? li : Mfma<T>::drow(lg, v));
+            const int col = 16 * u + (TR ? Mfma<T>::drow(lg, v) : li);
             cold[BETA0 ? 0 : r][BETA0 ? 0 : u][v] =
                 (row < K && col < N) ? a.C[row * a.ldc + col] : T(0);
           }
@@ -304,13 +309,14 @@ __global__ void __launch_bounds__(256) tsmm_mfma_kernel(TmArgs<T> a) {
       for (int s = 0; s < S; s++)
 #pragma unroll
         for (int u = 0; u < NT; u++)
-          acc[u] = Mfma<T>::mac(av[r][s], bop[s][u], acc[u]);
+          acc[u] = TR ? Mfma<T>::mac(bop[s][u], av[r][s], acc[u])
+                      : Mfma<T>::mac(av[r][s], bop[s][u], acc[u]);
 #pragma unroll
       for (int u = 0; u < NT; u++) {
-        const int col = 16 * u + li;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-          const int64_t row = (b0 + r) * 16 + Mfma<T>::drow(lg, v);
+          const int64_t row = (b0 + r) * 16 + (TR ? li : Mfma<T>::drow(lg, v));
+          const int col = 16 * u + (TR ? Mfma<T>::drow(lg, v) : li);
           if (row < K && col < N) {
             T* c = a.C + row * a.ldc + col;
             const T old = BETA0 ? T(0) : cold[BETA0 ? 0 : r][BETA0 ? 0 : u][v];
@@ -329,24 +335,38 @@ static int tm_slice_class(int S) {
   return S <= 1 ? 1 : S <= 2 ? 2 : S <= 4 ? 4 : S <= 6 ? 6 : S <= 8 ? 8 : S <= 12 ? 12 : 16;
 }
 
-template <typename T, int S, int NT>
-static void launch_mfma_s_nt(const TmArgs<T>& a, int blocks, bool beta0,
-                             bool vec, hipStream_t st) {
+template <typename T, int S, int NT, bool TR>
+static void launch_mfma_s_nt_tr(const TmArgs<T>& a, int blocks, bool beta0,
+                                bool vec, hipStream_t st) {
   if (vec && S % 2 == 0 && sizeof(T) == 8) {
     if (beta0)
-      hipLaunchKernelGGL((tsmm_mfma_kernel<T, S, NT, true, (S % 2 == 0)>),
+      hipLaunchKernelGGL((tsmm_mfma_kernel<T, S, NT, true, (S % 2 == 0), TR>),
                          dim3(blocks), dim3(256), 0, st, a);
     else
-      hipLaunchKernelGGL((tsmm_mfma_kernel<T, S, NT, false, (S % 2 == 0)>),
+      hipLaunchKernelGGL((tsmm_mfma_kernel<T, S, NT, false, (S % 2 == 0), TR>),
                          dim3(blocks), dim3(256), 0, st, a);
   } else {
     if (beta0)
-      hipLaunchKernelGGL((tsmm_mfma_kernel<T, S, NT, true, false>),
+      hipLaunchKernelGGL((tsmm_mfma_kernel<T, S, NT, true, false, TR>),
                          dim3(blocks), dim3(256), 0, st, a);
     else
-      hipLaunchKernelGGL((tsmm_mfma_kernel<T, S, NT, false, false>),
+      hipLaunchKernelGGL((tsmm_mfma_kernel<T, S, NT, false, false, TR>),
                          dim3(blocks), dim3(256), 0, st, a);
   }
+}
+
+template <typename T, int S, int NT>
+static void launch_mfma_s_nt(const TmArgs<T>& a, int blocks, bool beta0,
+                             bool vec, hipStream_t st) {
+  // measured on MI355X: the transposed tile wins for C at most 4 wide
+  // (M = 12, N = 1: 4817 -> 5108 GB/s) and, with beta != 0, up to 8 wide
+  // (old C loaded coalesced too: M = 5, N = 5, beta = 1: 3712 -> 5146 GB/s);
+  // with beta = 0 and 6..8 columns the untransposed tile is faster
+  const bool tr = NT == 1 && (a.N <= 4 || (!beta0 && a.N <= 8));
+  if (tr)
+    launch_mfma_s_nt_tr<T, S, 1, true>(a, blocks, beta0, vec, st);
+  else
+    launch_mfma_s_nt_tr<T, S, NT, false>(a, blocks, beta0, vec, st);
 }
 
 template <typename T, int S>
@@ -367,13 +387,14 @@ static void launch_mfma_s(const TmArgs<T>& a, int blocks, int nt, bool beta0,
 // M = 8 (in place); three columns up to M = 3.  In place with beta != 0 and C
 // wider than A, the matrix-core kernel's old-C tile loads are mostly masked
 // lanes, and whole rows win for M <= 3 up to N = 4 (M = 1, N = 2: 1385 ->
-// 3793 GB/s).  Out of place with beta != 0 the matrix-core kernel loads the
-// old C tiles with N of every 16 lanes, and whole rows win further: up to
-// M = 12 for N <= 2 and M = 7 for N <= 4 (M = 5, N = 1: 1990 -> 4932 GB/s).
+// 3793 GB/s).  Out of place with beta != 0, whole rows win up to M = 12 for
+// N <= 2 (M = 5, N = 1: 1990 -> 4932 GB/s) and at M <= 4 for N = 4; from
+// N = 3 on the matrix cores' transposed tile (coalesced old-C loads) wins
+// otherwise (M = 8, N = 3: 3919 -> 5084 GB/s).
 static bool tm_rows_wins(int64_t M, int64_t N, bool inplace, bool beta0) {
   if (!beta0 && !inplace) {
     if (N <= 2) return M <= 12;
-    if (N <= 4) return M <= 7 || (N == 3 && M <= 8);
+    return N == 4 && M <= 4;
   }
   if (N <= 2) return M <= (inplace ? 8 : 6);
   if (N == 3) return M <= 3;
