@@ -152,6 +152,24 @@ def test_unsupported_overlap_returns_false(gpu):
     assert gpu.dtsmm(0, 4, 4, 10, buf, 4, 1.0, B, 4, 0.0, buf[2:], 4) is False
 
 
+@pytest.mark.parametrize("M,N", [(7, 1), (12, 4), (5, 5), (10, 8), (40, 3), (64, 2)])
+def test_transposed_tile_alpha_beta(gpu, orc, M, N):
+    """Narrow C on the matrix cores' transposed tile (N <= 4, or N <= 8 with
+    beta != 0): alpha = 1.5, beta in {0, -0.5} on U[-1,1) data, out of
+    place, within the FP64 tolerance of the oracle."""
+    K = (1 << 15) + 7
+    for beta in (0.0, -0.5):
+        note(f"tsmm transposed tile M={M} N={N} K={K} beta={beta}")
+        A = orc.uniform(K * M, 31, M)
+        B = orc.uniform(M * N, 32, N)
+        C = orc.uniform(K * N, 33, 0)
+        want = C.copy()
+        orc.tsmm(M, N, K, A, M, 1.5, B, M, beta, want, N)
+        ok, got = _run(gpu, 0, M, N, K, A, M, 1.5, B, M, beta, C, N)
+        assert ok
+        assert rel_fro(got, want) <= 8 * (M + 1) * EPS64
+
+
 @pytest.mark.parametrize("M,N", [(4, 4), (8, 3), (16, 16), (33, 20), (64, 64)])
 def test_inplace_beta_uniform(gpu, orc, M, N):
     """In place with beta != 0 and N <= M: beta * C joins the operand as

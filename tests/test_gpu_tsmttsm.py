@@ -177,6 +177,23 @@ def test_narrow_asymmetric(gpu, orc, fp):
             assert np.array_equal(got, want), (M, N, K, lda, ldb, bc, beta)
 
 
+@pytest.mark.parametrize("M,N", [(12, 1), (16, 8), (9, 3), (3, 13), (6, 1), (1, 8), (5, 4)])
+def test_narrow_asymmetric_alpha_beta(gpu, orc, M, N):
+    """Paired-row shapes with alpha = 1.5, beta = -0.5 on U[-1,1) data: within
+    8 K eps of the exact oracle (the epilogue is beta C + alpha sum)."""
+    K = 1 << 16
+    for lda, ldb in ((M, N), (M + 1, N + 2)):
+        note(f"tsmttsm narrow alpha/beta M={M} N={N} K={K} lda={lda} ldb={ldb}")
+        A = orc.uniform(K * lda, 21, M)
+        B = orc.uniform(K * ldb, 22, N)
+        C0 = orc.uniform(M * N, 23, 0)
+        hi, lo = orc.xtsmttsm(M, N, K, A, lda, B, ldb)
+        want = 1.5 * (hi + lo) - 0.5 * C0
+        ok, got = _run(gpu, 0, M, N, K, A, lda, 1.5, B, ldb, -0.5, C0.copy(), M)
+        assert ok
+        assert rel_fro(got, want) <= 8 * K * EPS64
+
+
 def test_c1_config(gpu, orc):
     """K = 2^16, M = N = 8, alpha = 1, beta = 0, seed 1 (streams 0 and 1)."""
     M = N = 8
