@@ -14,8 +14,9 @@
 //    address in all lanes, so it is fetched with scalar loads into SGPRs and
 //    broadcast for free as the scalar operand of v_fma (the reference stages B
 //    through shared memory, fix3.cuh:10-18).
-//  * Row loads and stores use 16-byte vector accesses when the leading
-//    dimension and base pointer allow it.
+//  * Row loads and stores use two-element vector accesses (16 bytes for
+//    double, 8 for float) when the leading dimension and base pointer allow
+//    it.
 //  * The row length is rounded up to a compile-time class MC in
 //    {1, 2, 4, 8, 16, 32, 64} and masked with uniform branches; N is a runtime
 //    loop unrolled by two, so one instantiation serves a range of widths.
@@ -26,7 +27,7 @@
 //    four columns of A: lane (i, g) = (lane % 16, lane / 16) supplies
 //    A(row0 + i, m_g) and B(m_g, n0 + i).
 //  * Lane (i, g) owns the contiguous slice m = g*S .. g*S + S - 1 of row
-//    row0 + i (S = ceil(M / 4)), read with 16-byte loads; MFMA step s uses
+//    row0 + i (S = ceil(M / 4)), read two elements per load; MFMA step s uses
 //    m_g = g*S + s.  The 16 rows x M columns of a block are one contiguous
 //    region when lda == M.
 //  * B is the same for every row: each lane keeps its S x NT operand values
@@ -64,6 +65,13 @@ struct TmArgs {
   // beta = 0 and no read of C
   int fold;
   T fa, fb;
+};
+
+// Two consecutive elements as one load or store (16 bytes for double, 8 for
+// float): half the memory instructions of element-wise access.
+template <typename T>
+struct Vec2 {
+  typedef T type __attribute__((ext_vector_type(2)));
 };
 
 // Element (m, n) of the real B operand: B itself, or Bh of the complex view.
@@ -123,14 +131,15 @@ __global__ void __launch_bounds__(256) tsmm_kernel(TmArgs<T> a) {
       const int64_t row = base + (int64_t)r * nthreads;
       const bool ok = row < K;
       const T* ar = a.A + row * a.lda;
-      if (VEC && (MC % 2 == 0) && sizeof(T) == 8) {
+      if (VEC && (MC % 2 == 0)) {
 #pragma unroll
         for (int m = 0; m < MC; m += 2) {
           if (m < M) {
             if (ok && m + 1 < M) {
-              double2 v = *reinterpret_cast<const double2*>(ar + m);
-              av[r][m] = (T)v.x;
-              av[r][m + 1] = (T)v.y;
+              const typename Vec2<T>::type v =
+                  *reinterpret_cast<const typename Vec2<T>::type*>(ar + m);
+              av[r][m] = v.x;
+              av[r][m + 1] = v.y;
             } else {
               av[r][m] = ok ? ar[m] : T(0);
               av[r][m + 1] = T(0);
@@ -177,11 +186,8 @@ __global__ void __launch_bounds__(256) tsmm_kernel(TmArgs<T> a) {
           T c1 = BETA0 ? T(0) : cr[1];
           T o0 = axpby_t(s0[r], c0, a.alpha, a.beta, BETA0);
           T o1 = axpby_t(s1[r], c1, a.alpha, a.beta, BETA0);
-          if (VEC && sizeof(T) == 8) {
-            double2 v;
-            v.x = (double)o0;
-            v.y = (double)o1;
-            *reinterpret_cast<double2*>(cr) = v;
+          if (VEC) {
+            *reinterpret_cast<typename Vec2<T>::type*>(cr) = typename Vec2<T>::type{o0, o1};
           } else {
             cr[0] = o0;
             cr[1] = o1;
@@ -268,11 +274,12 @@ __global__ void __launch_bounds__(256) tsmm_mfma_kernel(TmArgs<T> a) {
       const bool ok = row < K;
       const T* ar = a.A + row * a.lda + mbeg;
       if (VEC) {
-        // S even, lda even, A 16-byte aligned, M a multiple of 4: no masks
+        // S even, lda even, A aligned to two elements, M a multiple of 4: no masks
 #pragma unroll
         for (int s = 0; s < S; s += 2) {
           if (ok) {
-            double2 v = *reinterpret_cast<const double2*>(ar + s);
+            const typename Vec2<T>::type v =
+                *reinterpret_cast<const typename Vec2<T>::type*>(ar + s);
             av[r][s] = v.x;
             av[r][s + 1] = v.y;
           } else {
@@ -338,7 +345,7 @@ static int tm_slice_class(int S) {
 template <typename T, int S, int NT, bool TR>
 static void launch_mfma_s_nt_tr(const TmArgs<T>& a, int blocks, bool beta0,
                                 bool vec, hipStream_t st) {
-  if (vec && S % 2 == 0 && sizeof(T) == 8) {
+  if (vec && S % 2 == 0) {
     if (beta0)
       hipLaunchKernelGGL((tsmm_mfma_kernel<T, S, NT, true, (S % 2 == 0), TR>),
                          dim3(blocks), dim3(256), 0, st, a);
@@ -487,9 +494,10 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
     const int S = (int)((M + 3) / 4);
     const int Sc = tm_slice_class(S);
     const int nt = (int)((N + 15) / 16);
-    // unmasked 16-byte row slices need every slice start 16-byte aligned
-    const bool vecm = sizeof(T) == 8 && (M % 4 == 0) && (S % 2 == 0) &&
-                      (Sc == S) && (lda % 2 == 0) && ((uintptr_t)A % 16 == 0);
+    // unmasked two-element row slices need every slice start aligned to two
+    // elements
+    const bool vecm = (M % 4 == 0) && (S % 2 == 0) && (Sc == S) && (lda % 2 == 0) &&
+                      ((uintptr_t)A % (2 * sizeof(T)) == 0);
     switch (Sc) {
       case 1: launch_mfma_s<T, 1>(a, blocks, nt, beta0, false, stream); break;
       case 2: launch_mfma_s<T, 2>(a, blocks, nt, beta0, vecm, stream); break;
@@ -503,9 +511,9 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
     if (e != hipSuccess) return hip_fail(e, "tsmm launch");
     return SKINNY_OK;
   }
-  // 16-byte row accesses need 16-byte aligned rows for both A and C
-  const bool vec = sizeof(T) == 8 && (lda % 2 == 0) && (ldc % 2 == 0) &&
-                   ((uintptr_t)A % 16 == 0) && ((uintptr_t)C % 16 == 0);
+  // two-element row accesses need rows aligned to two elements in A and C
+  const bool vec = (lda % 2 == 0) && (ldc % 2 == 0) && ((uintptr_t)A % (2 * sizeof(T)) == 0) &&
+                   ((uintptr_t)C % (2 * sizeof(T)) == 0);
   int MC = M <= 1 ? 1 : M <= 2 ? 2 : M <= 4 ? 4 : M <= 8 ? 8 : M <= 16 ? 16
            : M <= 32 ? 32 : 64;
   switch (MC) {
@@ -555,9 +563,9 @@ skinny_status_t launch_tsmm_cview(skinny_handle_t h, size_t blockCount,
   bool beta0 = (beta == T(0));
   if (Mc == 1) {
     // one complex column: the real view is 2 x 2, whole rows per thread
-    // (a row is one 16-byte complex value for double)
-    const bool vec = sizeof(T) == 8 && ((uintptr_t)A % 16 == 0) &&
-                     ((uintptr_t)C % 16 == 0);
+    // (a row is one complex value, one two-element access)
+    const bool vec = ((uintptr_t)A % (2 * sizeof(T)) == 0) &&
+                     ((uintptr_t)C % (2 * sizeof(T)) == 0);
     launch_mc<T, 2, true>(a, blocks, beta0, vec, stream);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "tsmm (complex view) launch");
@@ -574,8 +582,8 @@ skinny_status_t launch_tsmm_cview(skinny_handle_t h, size_t blockCount,
   const int S = (int)((M + 3) / 4);
   const int Sc = tm_slice_class(S);
   const int nt = (int)((N + 15) / 16);
-  const bool vecm = sizeof(T) == 8 && (M % 4 == 0) && (S % 2 == 0) &&
-                    (Sc == S) && (a.lda % 2 == 0) && ((uintptr_t)A % 16 == 0);
+  const bool vecm = (M % 4 == 0) && (S % 2 == 0) && (Sc == S) && (a.lda % 2 == 0) &&
+                    ((uintptr_t)A % (2 * sizeof(T)) == 0);
   switch (Sc) {
     case 1: launch_mfma_s<T, 1>(a, blocks, nt, beta0, false, stream); break;
     case 2: launch_mfma_s<T, 2>(a, blocks, nt, beta0, vecm, stream); break;
