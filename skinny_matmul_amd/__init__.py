@@ -393,18 +393,35 @@ def get_enabled_tsmm_versions(type_key: str = "DR") -> List[Tuple[MatmulFunction
 # ---------------------------------------------------------------------------
 # device data generation
 # ---------------------------------------------------------------------------
+def _fill_target(out, n, itemsize: int) -> int:
+    """Device address of a fill target.  A tensor must hold at least n
+    elements of the routine's element size (8 bytes for the double fills, 4
+    for the float one): the C routines write n elements from the address and
+    cannot see the buffer's extent."""
+    if hasattr(out, "element_size") and hasattr(out, "numel"):
+        if out.element_size() != itemsize:
+            raise SkinnyError(INVALID_ARG, f"fill: the tensor's elements are {out.element_size()} "
+                                           f"bytes, the routine writes {itemsize}-byte elements")
+        if n > out.numel():
+            raise SkinnyError(INVALID_ARG, f"fill: {n} elements into a tensor of {out.numel()}")
+    return _addr(out)
+
+
 def fill_uniform(out, n, seed, stream_id, offset=0, stream=None) -> None:
-    _check(lib().skinny_fill_uniform(_addr(out), n, seed, stream_id, offset,
+    """n doubles U[-1,1) from the counter RNG (float64 tensors or addresses)."""
+    _check(lib().skinny_fill_uniform(_fill_target(out, n, 8), n, seed, stream_id, offset,
                                      _stream(stream)), False)
 
 
 def fill_uniform_f(out, n, seed, stream_id, offset=0, stream=None) -> None:
-    _check(lib().skinny_fill_uniform_f(_addr(out), n, seed, stream_id, offset,
+    """n floats U[-1,1) from the counter RNG (float32 tensors or addresses)."""
+    _check(lib().skinny_fill_uniform_f(_fill_target(out, n, 4), n, seed, stream_id, offset,
                                        _stream(stream)), False)
 
 
 def fill_int(out, n, seed, stream_id, lo=-2, hi=2, offset=0, stream=None) -> None:
-    _check(lib().skinny_fill_int(_addr(out), n, seed, stream_id, offset, lo, hi,
+    """n integer-valued doubles in [lo, hi] (float64 tensors or addresses)."""
+    _check(lib().skinny_fill_int(_fill_target(out, n, 8), n, seed, stream_id, offset, lo, hi,
                                  _stream(stream)), False)
 
 

@@ -66,3 +66,20 @@ def test_no_oracle_in_product():
     import skinny_matmul_amd as sk
     out = subprocess.run(["ldd", sk.library_path()], capture_output=True, text=True).stdout
     assert "oracle" not in out
+
+
+def test_fill_rejects_mismatched_tensors():
+    """The fill routines write n elements of their own type from an address;
+    the binding refuses a tensor of another element size or fewer than n
+    elements before anything reaches the device (a float32 tensor passed to
+    the double fill would be written past its end)."""
+    torch = pytest.importorskip("torch")
+    import skinny_matmul_amd as sk
+    f32 = torch.empty(16, dtype=torch.float32)
+    f64 = torch.empty(16, dtype=torch.float64)
+    with pytest.raises(sk.SkinnyError, match="4 bytes"):
+        sk.fill_uniform(f32, 16, 1, 0)
+    with pytest.raises(sk.SkinnyError, match="8 bytes"):
+        sk.fill_uniform_f(f64, 16, 1, 0)
+    with pytest.raises(sk.SkinnyError, match="into a tensor of 16"):
+        sk.fill_int(f64, 17, 1, 0)
