@@ -438,6 +438,9 @@ TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
     // left narrow asymmetric products (M = 12, N = 1: 3.1 TB/s) short of
     // loads in flight.
     int64_t per_cu = 2;
+    // FP32 with one column on each side: 4-byte lane loads keep too few bytes
+    // in flight at two per CU (K = 2^30: 2339 GB/s; sixteen per CU: 4922)
+    if (elem_bytes == 4 && M == 1 && N == 1) per_cu = 16;
     int64_t target = (per_cu * h->cu_count) / ntiles;
     if (target < 1) target = 1;
     int64_t byrows = K / (CH * kWaves * 2);
