@@ -98,6 +98,13 @@ int skinny_tuned_blockcount_tsmttsm(skinny_handle_t handle, int64_t M,
                                     int64_t N, int64_t K, int elem_bytes);
 int skinny_tuned_blockcount_tsmm(skinny_handle_t handle, int64_t M, int64_t N,
                                  int64_t K, int elem_bytes);
+/* The grid a real TSMM call (elem_bytes 4 or 8) with blockCount 0 launches,
+ * for beta == 0 (beta_zero != 0) or not and in place (C == A) or not: the
+ * kernel choice and its workgroups per CU depend on both.  The reference
+ * fixes its grids per variant in the perf driver (perf.cu:186-204). */
+int skinny_tuned_blockcount_tsmm_ex(skinny_handle_t handle, int64_t M,
+                                    int64_t N, int64_t K, int elem_bytes,
+                                    int beta_zero, int in_place);
 
 /* ---- TSMTTSM: C = alpha A^T B + beta C --------------------------------- */
 /* replaces GENV1..8/SPECSMALL::tsmttsm<T,...> and tsmttsm_cublas<T>

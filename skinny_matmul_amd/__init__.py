@@ -92,6 +92,7 @@ _SIGS = {
     "skinny_cu_count": (C.c_int, [vp]),
     "skinny_tuned_blockcount_tsmttsm": (C.c_int, [vp, i64, i64, i64, C.c_int]),
     "skinny_tuned_blockcount_tsmm": (C.c_int, [vp, i64, i64, i64, C.c_int]),
+    "skinny_tuned_blockcount_tsmm_ex": (C.c_int, [vp, i64, i64, i64, C.c_int, C.c_int, C.c_int]),
     "skinny_dtsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, vp]),
     "skinny_stsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, flt, vp, i64, flt, vp, i64, vp]),
     "skinny_dtsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, vp]),
@@ -394,16 +395,25 @@ def get_enabled_tsmm_versions(type_key: str = "DR") -> List[Tuple[MatmulFunction
 # device data generation
 # ---------------------------------------------------------------------------
 def _fill_target(out, n, itemsize: int) -> int:
-    """Device address of a fill target.  A tensor must hold at least n
-    elements of the routine's element size (8 bytes for the double fills, 4
-    for the float one): the C routines write n elements from the address and
-    cannot see the buffer's extent."""
+    """Device address of a fill target.  A tensor must be contiguous and hold
+    at least n elements of the routine's element size (8 bytes for the double
+    fills, 4 for the float one) in its own storage: the C routines write n
+    consecutive elements from the address and cannot see the buffer's extent
+    (an expanded view has more elements than bytes behind it, a strided view
+    would have its neighbours overwritten)."""
     if hasattr(out, "element_size") and hasattr(out, "numel"):
         if out.element_size() != itemsize:
             raise SkinnyError(INVALID_ARG, f"fill: the tensor's elements are {out.element_size()} "
                                            f"bytes, the routine writes {itemsize}-byte elements")
         if n > out.numel():
             raise SkinnyError(INVALID_ARG, f"fill: {n} elements into a tensor of {out.numel()}")
+        if n > 1 and not out.is_contiguous():
+            raise SkinnyError(INVALID_ARG, "fill: the tensor is not contiguous (the routine "
+                                           "writes consecutive elements)")
+        room = out.untyped_storage().nbytes() - out.storage_offset() * itemsize
+        if n * itemsize > room:
+            raise SkinnyError(INVALID_ARG, f"fill: {n * itemsize} bytes into {room} bytes "
+                                           "of storage")
     return _addr(out)
 
 
@@ -590,5 +600,9 @@ def tuned_blockcount_tsmttsm(M, N, K, elem_bytes=8, handle=None) -> int:
     return int(lib().skinny_tuned_blockcount_tsmttsm(_h(handle), M, N, K, elem_bytes))
 
 
-def tuned_blockcount_tsmm(M, N, K, elem_bytes=8, handle=None) -> int:
-    return int(lib().skinny_tuned_blockcount_tsmm(_h(handle), M, N, K, elem_bytes))
+def tuned_blockcount_tsmm(M, N, K, elem_bytes=8, handle=None, beta0=True,
+                          inplace=False) -> int:
+    """The grid a real TSMM call with blockCount 0 launches (beta == 0 or not,
+    C == A or not)."""
+    return int(lib().skinny_tuned_blockcount_tsmm_ex(_h(handle), M, N, K, elem_bytes,
+                                                     int(bool(beta0)), int(bool(inplace))))

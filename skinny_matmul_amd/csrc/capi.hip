@@ -186,6 +186,15 @@ int skinny_tuned_blockcount_tsmm(skinny_handle_t h, int64_t M, int64_t N,
   return tuned_blocks_tsmm(h, M, N, K, elem_bytes);
 }
 
+int skinny_tuned_blockcount_tsmm_ex(skinny_handle_t h, int64_t M, int64_t N,
+                                    int64_t K, int elem_bytes, int beta_zero,
+                                    int in_place) {
+  h = resolve(h);
+  if (!h || M < 0 || N < 0 || K < 0) return 0;
+  return tuned_blocks_tsmm(h, M, N, K, elem_bytes, true, beta_zero != 0,
+                           in_place != 0);
+}
+
 skinny_status_t skinny_dtsmttsm(skinny_handle_t h, size_t blockCount,
                                 int64_t M, int64_t N, int64_t K,
                                 const double* A, int64_t lda, double alpha,

@@ -421,11 +421,12 @@ int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
   int64_t per_cu;
   const bool rows = real ? (M <= 64 && N <= 64 && tm_rows_wins(M, N, inplace, beta0))
                          : (M <= 2 && N <= 2);
-  if (rows) per_cu = (M <= 1 || !beta0) ? 4 : 2;
   // FP32 rows carry half the bytes per row in flight: eight per CU
   // (M = 3, N = 1: 3332 -> 4512 GB/s; M = 4, N = 2: 4178 -> 5138 GB/s)
-  if (rows && elem_bytes == 4) per_cu = 8;
-  else per_cu = M <= 4 ? 8 : 2;
+  if (rows)
+    per_cu = elem_bytes == 4 ? 8 : ((M <= 1 || !beta0) ? 4 : 2);
+  else
+    per_cu = M <= 4 ? 8 : 2;
   int64_t cap = per_cu * h->cu_count;
   if (want > cap) want = cap;
   if (want < 1) want = 1;

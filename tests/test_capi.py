@@ -83,3 +83,13 @@ def test_fill_rejects_mismatched_tensors():
         sk.fill_uniform_f(f64, 16, 1, 0)
     with pytest.raises(sk.SkinnyError, match="into a tensor of 16"):
         sk.fill_int(f64, 17, 1, 0)
+    # an expanded view: 16 elements over 8 bytes of storage
+    with pytest.raises(sk.SkinnyError, match="not contiguous"):
+        sk.fill_uniform(torch.empty(1, dtype=torch.float64).expand(16), 16, 1, 0)
+    # a strided column view: its neighbours would be overwritten
+    with pytest.raises(sk.SkinnyError, match="not contiguous"):
+        sk.fill_uniform(torch.empty(8, 4, dtype=torch.float64)[:, 0], 8, 1, 0)
+    # a contiguous view near the end of its storage
+    tail = torch.empty(16, dtype=torch.float64)[12:]
+    with pytest.raises(sk.SkinnyError, match="into a tensor of 4"):
+        sk.fill_uniform(tail, 5, 1, 0)

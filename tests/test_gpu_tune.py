@@ -41,3 +41,38 @@ def test_autotune_roundtrip_and_parity(gpu, orc):
     assert gpu.tune_export(handle=h2) == text
     gpu.tune_clear(handle=h2)
     assert gpu.tune_export(handle=h2) == ""
+
+
+# (elem_bytes, M, N, beta0, inplace) -> workgroups per CU of the untuned grid.
+# Whole-row kernel (narrow C): FP64 four per CU for M = 1 or beta != 0, two
+# otherwise; FP32 eight.  Matrix-core kernel: eight for A at most 4 columns
+# wide, two otherwise.
+DEFAULT_GRID_CASES = [
+    (8, 5, 1, False, False, 4),   # whole rows, beta != 0 (M = 5, N = 1, beta = 1)
+    (8, 5, 1, True, False, 2),    # whole rows, beta == 0, M > 1
+    (8, 1, 1, True, False, 4),    # whole rows, M = 1
+    (8, 2, 2, True, True, 2),     # whole rows in place
+    (8, 2, 4, False, True, 4),    # whole rows in place, beta != 0, C wider than A
+    (8, 2, 5, False, True, 8),    # matrix cores in place, beta != 0, A 2 wide
+    (8, 4, 4, True, False, 8),    # matrix cores, A 4 wide
+    (8, 8, 8, True, False, 2),    # matrix cores, A 8 wide
+    (8, 64, 64, True, False, 2),  # matrix cores, A 64 wide
+    (8, 8, 3, False, False, 2),   # matrix cores (transposed tile), beta != 0
+    (4, 3, 1, True, False, 8),    # FP32 whole rows
+    (4, 4, 4, True, False, 8),    # FP32 matrix cores, A 4 wide
+    (4, 16, 16, True, False, 2),  # FP32 matrix cores
+]
+
+
+@pytest.mark.parametrize("eb,M,N,beta0,inplace,per_cu", DEFAULT_GRID_CASES)
+def test_default_tsmm_grid(gpu, eb, M, N, beta0, inplace, per_cu):
+    """The untuned TSMM grid follows the measured per-CU rule for each kernel
+    class; a fresh handle holds no autotuned entries."""
+    h = gpu.Handle()
+    cus = gpu.cu_count(handle=h)
+    K = 1 << 26
+    got = gpu.tuned_blockcount_tsmm(M, N, K, eb, handle=h, beta0=beta0, inplace=inplace)
+    assert got == per_cu * cus, (eb, M, N, beta0, inplace, got, cus)
+    # a short K caps the grid at one workgroup per 256 rows
+    assert gpu.tuned_blockcount_tsmm(M, N, 1000, eb, handle=h, beta0=beta0,
+                                     inplace=inplace) == 4
