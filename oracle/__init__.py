@@ -54,7 +54,6 @@ def lib():
             "orc_stsmm": (None, [i64, i64, i64, vp, i64, flt, vp, i64, flt, vp, i64]),
             "orc_ztsmm": (None, [i64, i64, i64, vp, i64, vp, vp, i64, vp, vp, i64]),
             "orc_ctsmm": (None, [i64, i64, i64, vp, i64, vp, vp, i64, vp, vp, i64]),
-            "orc_dtsmm_rows": (None, [i64, i64, vp, i64, dbl, vp, i64, dbl, vp, vp]),
             "orc_dtsmttsm_par": (None, [C.c_int, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64]),
             "orc_dtsmm_par": (None, [C.c_int, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64]),
             "orc_max_threads": (C.c_int, []),

@@ -290,19 +290,6 @@ void orc_ctsmm(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
   free(crow);
 }
 
-void orc_dtsmm_rows(int64_t M, int64_t N, const double* Arows, int64_t nrows,
-                    double alpha, const double* B, int64_t ldb, double beta,
-                    const double* Cold, double* Cout) {
-  for (int64_t r = 0; r < nrows; r++) {
-    for (int64_t n = 0; n < N; n++) {
-      double s = 0.0;
-      for (int64_t m = 0; m < M; m++) s += Arows[r * M + m] * B[n * ldb + m];
-      Cout[r * N + n] =
-          (beta == 0.0) ? alpha * s : beta * Cold[r * N + n] + alpha * s;
-    }
-  }
-}
-
 /* ------------------------------------------------------------------------ */
 /* OpenMP CPU baseline                                                      */
 /* ------------------------------------------------------------------------ */

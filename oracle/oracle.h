@@ -68,11 +68,6 @@ void orc_ctsmm(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
                const float* alpha, const float* B, int64_t ldb,
                const float* beta, float* C, int64_t ldc);
 
-/* sampled TSMM rows: C rows listed in `rows` only (for K too large to copy) */
-void orc_dtsmm_rows(int64_t M, int64_t N, const double* Arows, int64_t nrows,
-                    double alpha, const double* B, int64_t ldb, double beta,
-                    const double* Cold, double* Cout);
-
 /* ---- OpenMP CPU baseline (FP64): K-chunks with private accumulators ---- */
 void orc_dtsmttsm_par(int nthreads, int64_t M, int64_t N, int64_t K,
                       const double* A, int64_t lda, double alpha,

@@ -48,6 +48,79 @@ def test_oracle_tsmm_golden(orc, case):
     assert np.array_equal(C, np.array(case["C"], dtype=np.float64))
 
 
+@pytest.mark.parametrize("case", load("tsmttsm_int.json"), ids=lambda c: f"{c['M']}x{c['N']}x{c['K']}")
+def test_oracle_stsmttsm_golden(orc, case):
+    """orc_stsmttsm (FR) on the same fixtures held as float32: small integers
+    and their products and sums are exact in float32, so the result must
+    equal the stored C bit for bit (the FR type of types.hpp:29-37)."""
+    A = np.array(case["A"], dtype=np.float32)
+    B = A if case["self"] else np.array(case["B"], dtype=np.float32)
+    C = np.array(case["C0"], dtype=np.float32)
+    assert np.abs(np.array(case["C"])).max() < 2 ** 24
+    orc.tsmttsm(case["M"], case["N"], case["K"], A, case["lda"], case["alpha"], B,
+                case["ldb"], case["beta"], C, case["ldc"])
+    assert C.dtype == np.float32
+    assert np.array_equal(C, np.array(case["C"], dtype=np.float32))
+
+
+@pytest.mark.parametrize("case", load("tsmm_int.json"), ids=lambda c: f"{c['M']}x{c['N']}x{c['K']}{'ip' if c['inplace'] else ''}")
+def test_oracle_stsmm_golden(orc, case):
+    """orc_stsmm (FR) on the TSMM fixtures as float32, in place included:
+    bitwise equal to the stored C."""
+    B = np.array(case["B"], dtype=np.float32)
+    C = np.array(case["C0"], dtype=np.float32)
+    A = C if case["inplace"] else np.array(case["A"], dtype=np.float32)
+    orc.tsmm(case["M"], case["N"], case["K"], A, case["lda"], case["alpha"], B,
+             case["ldb"], case["beta"], C, case["ldc"])
+    assert C.dtype == np.float32
+    assert np.array_equal(C, np.array(case["C"], dtype=np.float32))
+
+
+def _ulps32(got, want):
+    """Distance in float32 units in the last place (same-sign finite values)."""
+    g = got.astype(np.float32).view(np.int32).astype(np.int64)
+    w = want.astype(np.float32).view(np.int32).astype(np.int64)
+    return np.abs(g - w)
+
+
+def test_oracle_fp32_accumulates_in_double(orc):
+    """On non-integer float32 data the FR oracle is the double-precision sum
+    of the exact float32 products, rounded once to float32: within one
+    float32 ulp of numpy's float64 product (another summation order, error
+    far below half a float32 ulp) and equal in almost every entry.  A
+    float32 running sum, the plausible slip, misses by many ulps at this K."""
+    M, N, K = 6, 5, 4096
+    A = orc.uniform_f(K * M, 3, 0)
+    B = orc.uniform_f(K * N, 3, 1)
+    C0 = orc.uniform_f(M * N, 3, 2)
+    C = C0.copy()
+    orc.tsmttsm(M, N, K, A, M, 0.75, B, N, -0.5, C, M)
+    S = A.reshape(K, M).astype(np.float64).T @ B.reshape(K, N).astype(np.float64)
+    want = (0.75 * S - 0.5 * C0.reshape(N, M).T.astype(np.float64)).astype(np.float32)
+    got = C.reshape(N, M).T
+    d = _ulps32(got, want)
+    assert d.max() <= 1 and (d == 0).mean() >= 0.9
+    # the same sums accumulated in float32 fail this bound
+    f32 = np.zeros((M, N), dtype=np.float32)
+    Af, Bf = A.reshape(K, M), B.reshape(K, N)
+    for k in range(K):
+        f32 += np.outer(Af[k], Bf[k]).astype(np.float32)
+    slip = (np.float32(0.75) * f32 - np.float32(0.5) * C0.reshape(N, M).T).astype(np.float32)
+    assert _ulps32(slip, want).max() > 4
+
+    # TSMM: rows of A times B (column-major M x N), beta != 0
+    Kr = 513
+    A = orc.uniform_f(Kr * M, 4, 0)
+    Bm = orc.uniform_f(M * N, 4, 1)
+    C0 = orc.uniform_f(Kr * N, 4, 2)
+    C = C0.copy()
+    orc.tsmm(M, N, Kr, A, M, 1.25, Bm, M, 0.5, C, N)
+    S = A.reshape(Kr, M).astype(np.float64) @ Bm.reshape(N, M).T.astype(np.float64)
+    want = (1.25 * S + 0.5 * C0.reshape(Kr, N).astype(np.float64)).astype(np.float32)
+    d = _ulps32(C.reshape(Kr, N), want)
+    assert d.max() <= 1 and (d == 0).mean() >= 0.9
+
+
 def _cz(v):
     return np.array([complex(a, b) for a, b in v], dtype=np.complex128)
 
