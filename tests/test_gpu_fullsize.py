@@ -126,3 +126,52 @@ def test_uniform_full_k_within_tolerance(gpu):
     assert float(err) <= 8 * M * EPS64, float(err)
     del A, B, C, T, ref
     _free()
+
+
+@pytest.mark.parametrize("M,N", [(8, 8), (64, 64), (5, 1)])
+def test_tsmm_full_k_sampled_rows_from_rng(gpu, orc, M, N):
+    """U[-1,1) data at K = 2^26: rows of C sampled across the whole range
+    (first, last and 4094 random) are recomputed on the host from the
+    counter RNG (the oracle regenerates each row of A from its offset) and
+    must agree with the device within the dot-product error bound
+    2 M eps sum_m |a_m b_mn| per entry (both sides sum M exact products in
+    FP64, in different orders).  Out of place with beta = 0, and in place
+    with beta != 0 (C == A, old C = the row of A)."""
+    import torch
+    K = K_FULL
+    seed = 17
+    rng = np.random.default_rng(M * 100 + N)
+    rows = np.unique(np.concatenate([[0, K - 1], rng.integers(0, K, 4094)]))
+    Bh = orc.uniform(M * N, seed, 1)  # column-major M x N, ldb = M
+    Bm = Bh.reshape(N, M).T  # Bm[m, n]
+    Ah = np.concatenate([orc.uniform(M, seed, 0, int(r) * M) for r in rows]).reshape(-1, M)
+    bound_base = np.abs(Ah) @ np.abs(Bm)  # sum_m |a_m b_mn| per sampled row
+    idx = torch.from_numpy(rows).cuda()
+
+    note(f"fullsize tsmm sampled rows M={M} N={N} K=2^26 out of place")
+    A = torch.empty(K * M, dtype=torch.float64, device="cuda")
+    gpu.fill_uniform(A, A.numel(), seed, 0)
+    dB = torch.from_numpy(Bh).cuda()
+    C = torch.full((K * N,), float("nan"), dtype=torch.float64, device="cuda")
+    assert gpu.dtsmm(0, M, N, K, A, M, 1.0, dB, M, 0.0, C, N)
+    got = C.view(K, N).index_select(0, idx).cpu().numpy()
+    want = np.zeros(len(rows) * N)
+    orc.tsmm(M, N, len(rows), Ah.ravel(), M, 1.0, Bh, M, 0.0, want, N)
+    want = want.reshape(-1, N)
+    assert np.all(np.abs(got - want) <= 2 * M * EPS64 * bound_base)
+    del C
+    _free()
+
+    if N <= M:
+        note(f"fullsize tsmm sampled rows M={M} N={N} K=2^26 in place, beta != 0")
+        alpha, beta = 0.75, -1.5
+        assert gpu.dtsmm(0, M, N, K, A, M, alpha, dB, M, beta, A, M)
+        got = A.view(K, M)[:, :N].index_select(0, idx).cpu().numpy()
+        want = Ah[:, :N].copy().ravel()
+        orc.tsmm(M, N, len(rows), Ah.ravel(), M, alpha, Bh, M, beta, want, N)
+        want = want.reshape(-1, N)
+        # the device folds beta I into the operand: M + 1 products per entry
+        bound = 2 * (M + 2) * EPS64 * (abs(alpha) * bound_base + abs(beta) * np.abs(Ah[:, :N]))
+        assert np.all(np.abs(got - want) <= bound)
+    del A, dB, idx
+    _free()
