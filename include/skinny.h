@@ -115,6 +115,27 @@ skinny_status_t skinny_dtsmttsm(skinny_handle_t h, size_t blockCount,
                                 const double* A, int64_t lda, double alpha,
                                 const double* B, int64_t ldb, double beta,
                                 double* C, int64_t ldc, skinny_stream_t stream);
+/* No counterpart in the reference: the FP64 TSMTTSM above for callers that
+ * cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, plan, kernels and
+ * results as skinny_dtsmttsm at the same blockCount; the split-K partials and
+ * arrival counters live in a fixed device array inside the library (one per
+ * GPU, SKINNY_NOALLOC_PARTIAL_BYTES of partials, held until the process
+ * ends) instead of a handle's workspace, so no call allocates, frees or
+ * synchronises.  The handle may be NULL (current device, untuned grid; no
+ * default handle is created); a given handle is used for its device and
+ * tuned grids only.  blockCount 0 takes the largest default split whose
+ * partials fit; an explicit blockCount whose partials (blockCount x tiles x
+ * tile entries x 8 bytes) exceed the limit returns SKINNY_INVALID_ARG.
+ * All calls on one GPU share its array: calls that overlap in time on one
+ * GPU (other streams, unordered host threads) race and must not be made. */
+#define SKINNY_NOALLOC_PARTIAL_BYTES (16u << 20)
+skinny_status_t skinny_dtsmttsm_noalloc(skinny_handle_t h, size_t blockCount,
+                                        int64_t M, int64_t N, int64_t K,
+                                        const double* A, int64_t lda,
+                                        double alpha, const double* B,
+                                        int64_t ldb, double beta, double* C,
+                                        int64_t ldc, skinny_stream_t stream);
 skinny_status_t skinny_stsmttsm(skinny_handle_t h, size_t blockCount,
                                 int64_t M, int64_t N, int64_t K,
                                 const float* A, int64_t lda, float alpha,

@@ -94,6 +94,7 @@ _SIGS = {
     "skinny_tuned_blockcount_tsmm": (C.c_int, [vp, i64, i64, i64, C.c_int]),
     "skinny_tuned_blockcount_tsmm_ex": (C.c_int, [vp, i64, i64, i64, C.c_int, C.c_int, C.c_int]),
     "skinny_dtsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, vp]),
+    "skinny_dtsmttsm_noalloc": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, vp]),
     "skinny_stsmttsm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, flt, vp, i64, flt, vp, i64, vp]),
     "skinny_dtsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, dbl, vp, i64, dbl, vp, i64, vp]),
     "skinny_stsmm": (C.c_int, [vp, sz, i64, i64, i64, vp, i64, flt, vp, i64, flt, vp, i64, vp]),
@@ -239,6 +240,18 @@ def dtsmttsm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
     return _check(lib().skinny_dtsmttsm(_h(handle), blockCount, M, N, K, _addr(A), lda,
                                         float(alpha), _addr(B), ldb, float(beta),
                                         _addr(Cm), ldc, _stream(stream)))
+
+
+def dtsmttsm_noalloc(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,
+                     handle=None, stream=None) -> bool:
+    """skinny_dtsmttsm with the partials in the library's fixed per-GPU
+    scratch array (no device allocation; overlapping calls on one GPU race)."""
+    return _check(lib().skinny_dtsmttsm_noalloc(_h(handle), blockCount, M, N, K, _addr(A),
+                                                lda, float(alpha), _addr(B), ldb,
+                                                float(beta), _addr(Cm), ldc, _stream(stream)))
+
+
+NOALLOC_PARTIAL_BYTES = 16 << 20
 
 
 def stsmttsm(blockCount, M, N, K, A, lda, alpha, B, ldb, beta, Cm, ldc,

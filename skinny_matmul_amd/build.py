@@ -46,6 +46,7 @@ SOURCES = [
     "batch.hip",
     "tune.hip",
     "ozaki.hip",
+    "noalloc.hip",
 ]
 
 CFLAGS = [

@@ -106,19 +106,24 @@ int tsmttsm_pair_default_blocks(skinny_handle_t h, int64_t M, int64_t N, int64_t
 
 // complex (cplx = true): A, B, C point at interleaved (re, im) data and M, N,
 // lda, ldb are the real-view sizes (2x the complex ones); ldc stays complex.
+// `fixed` (optional) is a workspace of the same layout that the call uses
+// instead of the stream's; it must hold kCounterBytes + p.scratch_bytes, with
+// its counters at zero.
 template <typename T>
 skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
                                int64_t M, int64_t N, int64_t K, const T* A,
                                int64_t lda, T alpha, T alpha_i, const T* B,
                                int64_t ldb, T beta, T beta_i, T* C,
-                               int64_t ldc, bool cplx, hipStream_t stream);
+                               int64_t ldc, bool cplx, hipStream_t stream,
+                               const Workspace* fixed = nullptr);
 
 template <typename T>
 skinny_status_t launch_tsmttsm_real(skinny_handle_t h, const TsmttsmPlan& p,
                                     int64_t M, int64_t N, int64_t K,
                                     const T* A, int64_t lda, T alpha,
                                     const T* B, int64_t ldb, T beta, T* C,
-                                    int64_t ldc, hipStream_t stream);
+                                    int64_t ldc, hipStream_t stream,
+                                    const Workspace* fixed = nullptr);
 
 // ---- TSMM ---------------------------------------------------------------
 // Default grid of a TSMM call; real = false for the complex kernels, whose

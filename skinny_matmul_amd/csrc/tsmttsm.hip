@@ -506,13 +506,6 @@ static bool dispatch(const TsmttsmPlan& p, const TsArgs<T>& a, int blocks,
 #undef SK_CASE
 }
 
-template <typename T>
-skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
-                               int64_t M, int64_t N, int64_t K, const T* A,
-                               int64_t lda, T alpha, T alpha_i, const T* B,
-                               int64_t ldb, T beta, T beta_i, T* C,
-                               int64_t ldc, bool cplx, hipStream_t stream);
-
 // FP64 products with one operand 9..16 columns wide and the other at most 8
 // (two rows per lane group), or 5..8 wide and at most 4 (four rows) run on
 // tsmttsm_pair_kernel; returns the rows per group, 0 for the general kernel.
@@ -551,14 +544,21 @@ static skinny_status_t launch_tsmttsm_pair(skinny_handle_t h, const TsmttsmPlan&
                                            int64_t M, int64_t N, int64_t K, const double* A,
                                            int64_t lda, double alpha, const double* B,
                                            int64_t ldb, double beta, double* C, int64_t ldc,
-                                           hipStream_t stream) {
+                                           hipStream_t stream, const Workspace* fixed) {
   constexpr int U = kPairUnroll;
   const int R = pair_rows(M, N, A, lda, B, ldb);
   const int64_t G = p.fixedG ? p.G : pair_grid(h, R, K);
   const int PS = (int)(M * N);
+  const size_t need = (size_t)G * PS * sizeof(double);
   Workspace w;
-  skinny_status_t st = acquire_workspace(h, stream, (size_t)G * PS * sizeof(double), &w);
-  if (st != SKINNY_OK) return st;
+  if (fixed) {
+    if (kCounterBytes + need > fixed->bytes)
+      return fail(SKINNY_INVALID_ARG, "tsmttsm: partials exceed the given workspace");
+    w = *fixed;
+  } else {
+    skinny_status_t st = acquire_workspace(h, stream, need, &w);
+    if (st != SKINNY_OK) return st;
+  }
   TsArgs<double> a{};
   a.A = A;
   a.B = B;
@@ -608,9 +608,10 @@ skinny_status_t launch_tsmttsm_real(skinny_handle_t h, const TsmttsmPlan& p,
                                     int64_t M, int64_t N, int64_t K,
                                     const T* A, int64_t lda, T alpha,
                                     const T* B, int64_t ldb, T beta, T* C,
-                                    int64_t ldc, hipStream_t stream) {
+                                    int64_t ldc, hipStream_t stream,
+                                    const Workspace* fixed) {
   return launch_tsmttsm<T>(h, p, M, N, K, A, lda, alpha, T(0), B, ldb, beta,
-                           T(0), C, ldc, false, stream);
+                           T(0), C, ldc, false, stream, fixed);
 }
 
 template <typename T>
@@ -618,17 +619,24 @@ skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
                                int64_t M, int64_t N, int64_t K, const T* A,
                                int64_t lda, T alpha, T alpha_i, const T* B,
                                int64_t ldb, T beta, T beta_i, T* C,
-                               int64_t ldc, bool cplx, hipStream_t stream) {
+                               int64_t ldc, bool cplx, hipStream_t stream,
+                               const Workspace* fixed) {
   if (!cplx && sizeof(T) == 8 && pair_rows(M, N, A, lda, B, ldb) > 0)
     return launch_tsmttsm_pair(h, p, M, N, K, (const double*)A, lda, (double)alpha,
                                (const double*)B, ldb, (double)beta, (double*)C, ldc,
-                               stream);
+                               stream, fixed);
   const int ntiles = p.tilesM * p.tilesN;
   if ((size_t)ntiles * (p.ngroups + 1) > (size_t)kMaxCounters)
     return fail(SKINNY_UNSUPPORTED, "tsmttsm: too many partial groups");
   Workspace w;
-  skinny_status_t st = acquire_workspace(h, stream, p.scratch_bytes, &w);
-  if (st != SKINNY_OK) return st;
+  if (fixed) {
+    if (kCounterBytes + p.scratch_bytes > fixed->bytes)
+      return fail(SKINNY_INVALID_ARG, "tsmttsm: partials exceed the given workspace");
+    w = *fixed;
+  } else {
+    skinny_status_t st = acquire_workspace(h, stream, p.scratch_bytes, &w);
+    if (st != SKINNY_OK) return st;
+  }
   TsArgs<T> a;
   a.A = A;
   a.B = B;
@@ -689,18 +697,18 @@ skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
 template skinny_status_t launch_tsmttsm<double>(
     skinny_handle_t, const TsmttsmPlan&, int64_t, int64_t, int64_t,
     const double*, int64_t, double, double, const double*, int64_t, double,
-    double, double*, int64_t, bool, hipStream_t);
+    double, double*, int64_t, bool, hipStream_t, const Workspace*);
 template skinny_status_t launch_tsmttsm<float>(
     skinny_handle_t, const TsmttsmPlan&, int64_t, int64_t, int64_t,
     const float*, int64_t, float, float, const float*, int64_t, float, float,
-    float*, int64_t, bool, hipStream_t);
+    float*, int64_t, bool, hipStream_t, const Workspace*);
 template skinny_status_t launch_tsmttsm_real<double>(
     skinny_handle_t, const TsmttsmPlan&, int64_t, int64_t, int64_t,
     const double*, int64_t, double, const double*, int64_t, double, double*,
-    int64_t, hipStream_t);
+    int64_t, hipStream_t, const Workspace*);
 template skinny_status_t launch_tsmttsm_real<float>(
     skinny_handle_t, const TsmttsmPlan&, int64_t, int64_t, int64_t,
     const float*, int64_t, float, const float*, int64_t, float, float*,
-    int64_t, hipStream_t);
+    int64_t, hipStream_t, const Workspace*);
 
 }  // namespace skinny
