@@ -29,6 +29,7 @@
 #include "internal.hpp"
 #include "mfma.hpp"
 #include "tsmttsm_core.hpp"
+#include "tsmttsm_q4.hpp"
 #include "team_sum.hpp"
 
 namespace skinny {
@@ -116,6 +117,62 @@ __device__ void final_epilogue(const TsArgs<T>& a, const T* fin, int m0,
   }
 }
 
+// Grid reduction after a workgroup wrote its partial (single-launch form,
+// used with G == 1 and by callers that keep two_pass off): the last arrival
+// of each group sums the group, then the last group sums the group sums and
+// applies the epilogue.  `red` is the kernel's LDS of LDSN elements.
+template <typename T, int LDSN>
+__device__ __forceinline__ void grid_finish(const TsArgs<T>& a, T* red, int tile, int kp,
+                                            int ntiles, int m0, int n0, int Mloc,
+                                            int Nloc) {
+  const int PS = a.outM * a.outN;
+  // ---- grid reduction: last arrival per group, then last group ----------
+  // Arrival publishes this workgroup's partial with a release fence at agent
+  // scope (L2 write-back, no invalidate); only the last arrival takes the
+  // acquire side (L2 invalidate) before it reads the other partials.  A full
+  // __threadfence() would invalidate the XCD's L2 in every workgroup.
+  __shared__ int s_last;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  const int grp = kp / a.GS;
+  const int gsz = min(a.GS, a.G - grp * a.GS);
+  unsigned* cnt1 = a.cnt + tile * a.ngroups + grp;
+  unsigned* cnt2 = a.cnt + ntiles * a.ngroups + tile;
+  if (threadIdx.x == 0) {
+    unsigned ticket = __hip_atomic_fetch_add(cnt1, 1u, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (ticket == (unsigned)(gsz - 1));
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+
+  const T* gsrc = a.part1 + ((int64_t)tile * a.G + (int64_t)grp * a.GS) * PS;
+  if (a.ngroups == 1) {
+    team_sum<T>(gsrc, gsz, PS, nullptr, red, LDSN, true);
+    final_epilogue<T>(a, red + LDSN / 2, m0, n0, Mloc, Nloc);
+    if (threadIdx.x == 0) *cnt1 = 0u;
+    return;
+  }
+  T* gdst = a.part2 + ((int64_t)tile * a.ngroups + grp) * PS;
+  team_sum<T>(gsrc, gsz, PS, gdst, red, LDSN, false);
+  if (threadIdx.x == 0) *cnt1 = 0u;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned ticket = __hip_atomic_fetch_add(cnt2, 1u, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (ticket == (unsigned)(a.ngroups - 1));
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  team_sum<T>(a.part2 + (int64_t)tile * a.ngroups * PS, a.ngroups, PS, nullptr,
+              red, LDSN, true);
+  final_epilogue<T>(a, red + LDSN / 2, m0, n0, Mloc, Nloc);
+  if (threadIdx.x == 0) *cnt2 = 0u;
+}
+
 template <typename T, int P, int MT, int NT, int U, bool SELF>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu((MT * NT >= 9) ? 2 : 1)))
@@ -156,53 +213,41 @@ __global__ void __launch_bounds__(256)
   T* mine = a.part1 + ((int64_t)tile * a.G + kp) * PS;
   Core::wg_partial(acc, red, mine, a.M, a.N, a.outM, a.outN);
   if (a.two_pass) return;  // the partials are summed by the next kernel
+  grid_finish<T, Core::RED_ELEMS>(a, red, tile, kp, ntiles, m0, n0, Mloc, Nloc);
+}
 
-  // ---- grid reduction: last arrival per group, then last group ----------
-  // Arrival publishes this workgroup's partial with a release fence at agent
-  // scope (L2 write-back, no invalidate); only the last arrival takes the
-  // acquire side (L2 invalidate) before it reads the other partials.  A full
-  // __threadfence() would invalidate the XCD's L2 in every workgroup.
-  __shared__ int s_last;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  const int grp = kp / a.GS;
-  const int gsz = min(a.GS, a.G - grp * a.GS);
-  unsigned* cnt1 = a.cnt + tile * a.ngroups + grp;
-  unsigned* cnt2 = a.cnt + ntiles * a.ngroups + tile;
-  if (threadIdx.x == 0) {
-    unsigned ticket = __hip_atomic_fetch_add(cnt1, 1u, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (ticket == (unsigned)(gsz - 1));
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-
-  const T* gsrc = a.part1 + ((int64_t)tile * a.G + (int64_t)grp * a.GS) * PS;
-  constexpr int LDSN = TsCore<T, P, MT, NT, U, SELF>::RED_ELEMS;
-  if (a.ngroups == 1) {
-    team_sum<T>(gsrc, gsz, PS, nullptr, red, LDSN, true);
-    final_epilogue<T>(a, red + LDSN / 2, m0, n0, Mloc, Nloc);
-    if (threadIdx.x == 0) *cnt1 = 0u;
-    return;
-  }
-  T* gdst = a.part2 + ((int64_t)tile * a.ngroups + grp) * PS;
-  team_sum<T>(gsrc, gsz, PS, gdst, red, LDSN, false);
-  if (threadIdx.x == 0) *cnt1 = 0u;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned ticket = __hip_atomic_fetch_add(cnt2, 1u, __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (ticket == (unsigned)(a.ngroups - 1));
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  team_sum<T>(a.part2 + (int64_t)tile * a.ngroups * PS, a.ngroups, PS, nullptr,
-              red, LDSN, true);
-  final_epilogue<T>(a, red + LDSN / 2, m0, n0, Mloc, Nloc);
-  if (threadIdx.x == 0) *cnt2 = 0u;
+// FP64 64 x 64 tiles on the 4x4x4 matrix-core instruction (TsQ4); the same
+// tiles, partials and reductions as tsmttsm_kernel with MT = NT = 4.  Two
+// waves per SIMD with U = 1 (64 accumulators and two sets of 16 operands per
+// lane), one with U = 2.
+template <int U, bool FULL, bool ALIGNED>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(U <= 1 ? 2 : 1)))
+    tsmttsm_q4_kernel(TsArgs<double> a) {
+  using Core = TsQ4<U, FULL, ALIGNED>;
+  const int wave = threadIdx.x >> 6;
+  const int ntiles = a.tilesM * a.tilesN;
+  const int tile = blockIdx.x % ntiles;
+  const int kp = blockIdx.x / ntiles;
+  const int m0 = (tile % a.tilesM) * 64;
+  const int n0 = (tile / a.tilesM) * 64;
+  const int Mloc = min(a.M - m0, 64);
+  const int Nloc = min(a.N - n0, 64);
+  double acc[Core::NACC];
+  Core::zero(acc);
+  // one column tile: rows are read by one workgroup only (nontemporal
+  // loads); several tiles share rows through L2
+  if (ntiles == 1)
+    Core::template accumulate<true>(a.A, a.lda, a.B, a.ldb, m0, n0, Mloc, Nloc, a.K,
+                                    (int64_t)kp * kWaves + wave, (int64_t)a.G * kWaves, acc);
+  else
+    Core::template accumulate<false>(a.A, a.lda, a.B, a.ldb, m0, n0, Mloc, Nloc, a.K,
+                                     (int64_t)kp * kWaves + wave, (int64_t)a.G * kWaves, acc);
+  __shared__ double red[Core::RED_ELEMS];
+  const int PS = a.outM * a.outN;
+  double* mine = a.part1 + ((int64_t)tile * a.G + kp) * PS;
+  Core::wg_partial(acc, red, mine, a.outM, a.outN);
+  if (a.two_pass) return;
+  grid_finish<double, Core::RED_ELEMS>(a, red, tile, kp, ntiles, m0, n0, Mloc, Nloc);
 }
 
 // Second pass: sums the G partials of each tile entry in a fixed order and
@@ -477,10 +522,39 @@ static void launch_one(const TsArgs<T>& a, int blocks, int uvar,
                        dim3(blocks), dim3(256), 0, s, a);
 }
 
+template <bool FULL, bool ALIGNED>
+static void launch_q4(const TsArgs<double>& a, int blocks, int uvar, hipStream_t s) {
+  if (uvar == 0)
+    hipLaunchKernelGGL((tsmttsm_q4_kernel<1, FULL, ALIGNED>), dim3(blocks), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((tsmttsm_q4_kernel<2, FULL, ALIGNED>), dim3(blocks), dim3(256), 0, s, a);
+}
+
+// 64 x 64 FP64 tiles of a general product: the 4x4x4 instruction (same box,
+// best grid each, K = 2^26: M = N = 64 11.06 -> 10.60 ms; M = 256 at
+// K = 2^22: 9.51 -> 9.37 ms, its Gram product 9.30 -> 8.93 ms).  A Gram
+// product with one tile stays on 16x16x4 steps, which load its one operand
+// once (M = 64: 6.55 against 6.72 ms).
+static void dispatch_q4(const TsArgs<double>& a, int blocks, int uvar, hipStream_t s) {
+  const bool full = a.M % 64 == 0 && a.N % 64 == 0;
+  const bool aligned = a.lda % 2 == 0 && a.ldb % 2 == 0 && (uintptr_t)a.A % 16 == 0 &&
+                       (uintptr_t)a.B % 16 == 0;
+#define SK_Q4(fu, al) \
+  if (full == (fu) && aligned == (al)) return launch_q4<fu, al>(a, blocks, uvar, s);
+  SK_Q4(true, true) SK_Q4(true, false) SK_Q4(false, true) SK_Q4(false, false)
+#undef SK_Q4
+}
+
 template <typename T, bool SELF>
 static bool dispatch(const TsmttsmPlan& p, const TsArgs<T>& a, int blocks,
                      hipStream_t s) {
   const int uv = p.uvar;
+  if constexpr (sizeof(T) == 8) {
+    if (!SELF && p.P == 1 && p.MT == 4 && p.NT == 4) {
+      dispatch_q4(a, blocks, uv, s);
+      return true;
+    }
+  }
   if (p.P > 1) {
     switch (p.P) {
       case 16: launch_one<T, 16, 1, 1, SELF>(a, blocks, uv, s); return true;

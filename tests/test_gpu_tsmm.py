@@ -184,3 +184,26 @@ def test_inplace_beta_uniform(gpu, orc, M, N):
     ok, got = _run(gpu, 0, M, N, K, C, M, 0.75, B, M, -1.25, C, M, inplace=True)
     assert ok
     assert rel_fro(got, want) <= 8 * (M + 1) * EPS64
+
+
+# A 57..64 columns wide with C 49..64 (the widest matrix-core class): full
+# width with aligned rows, ragged widths and odd leading dimensions, beta !=
+# 0, in place (beta folded into the operand) and ragged K
+@pytest.mark.parametrize("M,N,lda_pad,beta,inplace", [
+    (64, 64, 0, 0.0, False), (64, 64, 0, 1.0, False), (64, 64, 1, 0.0, False),
+    (64, 64, 0, 1.0, True), (64, 64, 0, 0.0, True), (60, 64, 0, 0.0, False),
+    (64, 50, 2, 1.0, False), (57, 49, 0, 1.0, True), (63, 63, 3, 0.0, True)])
+def test_wide64_tsmm(gpu, orc, M, N, lda_pad, beta, inplace):
+    K = 4099
+    ld = max(M, N) + lda_pad if inplace else M + lda_pad
+    ldc = ld if inplace else N + 1
+    note(f"tsmm wide64 M={M} N={N} lda={ld} beta={beta} inplace={inplace}")
+    A = orc.integers(K * ld, 91, 0)
+    Bm = orc.integers(M * N, 91, 1)
+    C0 = A.copy() if inplace else orc.integers(K * ldc, 91, 2)
+    want = C0.copy()
+    orc.tsmm(M, N, K, A.copy() if inplace else A, ld, 1.0, Bm, M, beta, want, ldc)
+    dA = dev(A)
+    dC = dA if inplace else dev(C0)
+    assert gpu.dtsmm(0, M, N, K, dA, ld, 1.0, dev(Bm), M, beta, dC, ldc)
+    assert np.array_equal(host(dC), want), (M, N, ld, beta, inplace)

@@ -85,6 +85,30 @@ def test_fuzz_wide_fp64(gpu, orc, M, N):
         _fuzz_one(gpu, orc, rng, M, N, np.float64, 3000)
 
 
+# 64 x 64 tiles run on the 4x4x4 matrix-core kernel: full and ragged tiles,
+# even and odd leading dimensions (16-byte pair loads or element loads), the
+# Gram product, one workgroup (in-kernel epilogue) and several grid tiles
+@pytest.mark.parametrize("M,N,ld_pad,self_,bc", [
+    (64, 64, 0, False, 0), (64, 64, 1, False, 37), (64, 64, 0, True, 0), (64, 64, 1, True, 5),
+    (50, 50, 0, False, 1), (57, 61, 3, False, 64), (49, 49, 2, True, 1), (63, 63, 1, True, 200),
+    (64, 52, 0, False, 3), (128, 64, 0, False, 0), (128, 128, 1, True, 9), (130, 70, 2, False, 1)])
+def test_q4_tiles(gpu, orc, M, N, ld_pad, self_, bc):
+    K = 4099
+    lda = M + ld_pad
+    ldb = lda if self_ else N + ld_pad
+    ldc = M + 1
+    note(f"tsmttsm q4 M={M} N={N} lda={lda} self={self_} bc={bc}")
+    A = orc.integers(K * lda, 77, 0)
+    B = A if self_ else orc.integers(K * ldb, 77, 1)
+    for beta in (0.0, 1.0):
+        C = orc.integers(N * ldc, 77, 2)
+        want = C.copy()
+        orc.tsmttsm(M, N, K, A, lda, 1.0, B, ldb, beta, want, ldc)
+        ok, got = _run(gpu, bc, M, N, K, A, lda, 1.0, B, ldb, beta, C, ldc, self_)
+        assert ok
+        assert np.array_equal(got, want), (M, N, lda, self_, bc, beta)
+
+
 @pytest.mark.parametrize("M,N", [(1, 1), (4, 4), (8, 8), (5, 11), (16, 16), (32, 32), (64, 64)])
 def test_fuzz_fp32(gpu, orc, M, N):
     rng = random.Random(M * 31 + N)
@@ -115,7 +139,10 @@ def test_gram_mirror_matches_general(gpu, orc, M):
     diagonal and mirrors them.  On uniform data the result is exactly
     symmetric and meets the exact oracle's bound; with the unroll and grid
     pinned through the tuning table it equals the general kernel on a copy
-    of A bitwise (the same products summed in the same order)."""
+    of A bitwise (the same products summed in the same order) where both run
+    on the 16x16x4 instruction.  From 49 columns the general product runs on
+    the 4x4x4 instruction, whose summation order differs: there the two
+    agree within the bound only."""
     import torch
     K = 1 << 16
     A = orc.uniform(K * M, 12, M)
@@ -138,8 +165,9 @@ def test_gram_mirror_matches_general(gpu, orc, M):
             G2 = gram.reshape(M, M)
             assert np.array_equal(G2, G2.T)
             assert rel_fro(gram - lo, hi) <= 8 * K * EPS64
-            if uvar is not None:
+            if uvar is not None and M < 49:
                 assert np.array_equal(gram, gen)
+            assert rel_fro(gen - lo, hi) <= 8 * K * EPS64
     finally:
         h.close()
 
