@@ -379,16 +379,25 @@ skinny_status_t skinny_dorth_error(skinny_handle_t h, int64_t M, int64_t K,
 /* ---- microbenchmarks: this chip's own ceilings ------------------------ */
 /* HBM stream over arrays of `bytes_per_array` (allocated internally, far
  * above the Infinity Cache when >= 1 GiB); kind 0 read, 1 copy, 2 triad,
- * 3 write, 4 update (a += s b); best of `reps` and of eight launch shapes
- * (1, 2, 4 or 8 workgroups per CU, 1 or 4 accesses in flight per thread),
- * in GB/s of array traffic.
+ * 3 write, 4 update (a += s b); best of `reps` and of sixteen launch shapes
+ * (1, 2, 4 or 8 workgroups per CU, 1 or 4 accesses in flight per thread,
+ * loads with or without the nontemporal hint), in GB/s of array traffic.
+ * Kind 5 is a copy over other access patterns: nontemporal stores and/or
+ * loads, and per-wave bursts of 16 KiB to 1 MiB (best of 40 variants).
  * Replaces microbenches/membw.cu:19-164 (copy/scale/update/sum/triad). */
 skinny_status_t skinny_mb_stream(int kind, size_t bytes_per_array, int reps,
                                  double* gbs, skinny_stream_t stream);
-/* kind 0: FP64 vector FMA, 1: FP64 matrix core, 2: INT8 matrix core;
- * TFLOP/s (TOP/s).  Replaces microbenches/roofline.cu:19-66. */
+/* kind 0: FP64 vector FMA, 1: FP64 matrix core (16x16x4), 2: INT8 matrix
+ * core, 3: FP64 matrix core (4x4x4, four blocks); TFLOP/s (TOP/s).
+ * Replaces microbenches/roofline.cu:19-66. */
 skinny_status_t skinny_mb_peak(int kind, int reps, double* tflops,
                                skinny_stream_t stream);
+/* Shader clock in MHz while the peak loop of `kind` runs on `stream`, from
+ * the shader-clock and real-time counters its first workgroup reads at its
+ * start and end (-1: a one-wave spin of about 1 ms, the idle clock).
+ * Synchronous.  The reference normalises by a fixed clock
+ * (membw.cu:227,231); this measures the one the chip runs at. */
+skinny_status_t skinny_mb_clock(int kind, double* mhz, skinny_stream_t stream);
 /* Memory hierarchy by working-set size (replaces the LDS-throughput and
  * L1/L2 latency sweeps of microbenches/instruction_rates.cu:30-148):
  * kind 0: ns per dependent load of one lane chasing a chain over the 128-byte

@@ -129,6 +129,7 @@ _SIGS = {
     "skinny_dorth_error": (C.c_int, [vp, i64, i64, vp, i64, C.POINTER(C.c_double), vp]),
     "skinny_mb_stream": (C.c_int, [C.c_int, sz, C.c_int, C.POINTER(C.c_double), vp]),
     "skinny_mb_peak": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double), vp]),
+    "skinny_mb_clock": (C.c_int, [C.c_int, C.POINTER(C.c_double), vp]),
     "skinny_mb_cache": (C.c_int, [C.c_int, sz, C.c_int, C.POINTER(C.c_double), vp]),
     "skinny_mb_barrier": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_double), vp]),
 }
@@ -564,8 +565,8 @@ def orth_error(M, K, X, ldx, handle=None, stream=None) -> float:
     return out.value
 
 
-STREAM_KINDS = {"read": 0, "copy": 1, "triad": 2, "write": 3, "update": 4}
-PEAK_KINDS = {"fp64_fma": 0, "fp64_mfma": 1, "int8_mfma": 2}
+STREAM_KINDS = {"read": 0, "copy": 1, "triad": 2, "write": 3, "update": 4, "copy_variants": 5}
+PEAK_KINDS = {"fp64_fma": 0, "fp64_mfma": 1, "int8_mfma": 2, "fp64_mfma_4x4x4": 3}
 
 
 def mb_stream(kind: str, bytes_per_array: int = 4 << 30, reps: int = 5, stream=None) -> float:
@@ -580,6 +581,14 @@ def mb_peak(kind: str, reps: int = 3, stream=None) -> float:
     """Measured compute peak in TFLOP/s (TOP/s for int8)."""
     out = C.c_double()
     _check(lib().skinny_mb_peak(PEAK_KINDS[kind], reps, C.byref(out), _stream(stream)), False)
+    return out.value
+
+
+def mb_clock(kind: str = "fp64_mfma", stream=None) -> float:
+    """Shader clock in MHz while the peak loop of `kind` runs ("idle": none)."""
+    out = C.c_double()
+    k = -1 if kind == "idle" else PEAK_KINDS[kind]
+    _check(lib().skinny_mb_clock(k, C.byref(out), _stream(stream)), False)
     return out.value
 
 

@@ -9,9 +9,11 @@ The analogue of the reference's membw and roofline microbenchmarks
 
 * HBM streams over arrays of --gib GiB each (far above the 256 MB Infinity
   Cache): read, copy, triad, write and update (a += s b), GB/s of array
-  traffic, best of --reps;
-* FP64 vector FMA, FP64 matrix-core (16x16x4) and INT8 matrix-core
-  (32x32x32) peaks in TFLOP/s (TOP/s).
+  traffic, best of --reps; and the best of 40 other copy patterns
+  (nontemporal stores, per-wave bursts of 16 KiB to 1 MiB);
+* FP64 vector FMA, FP64 matrix-core (16x16x4 and four-block 4x4x4) and INT8
+  matrix-core (32x32x32) peaks in TFLOP/s (TOP/s), each with the shader
+  clock measured while its loop runs.
 
 * with --hierarchy: ns per dependent load over working sets from 4 KiB to
   1 GiB, GB/s re-reading buffers from 4 MiB to 8 GiB, LDS GB/s and latency
@@ -27,10 +29,10 @@ import argparse
 import json
 import sys
 
-from . import mb_peak, mb_stream
+from . import mb_clock, mb_peak, mb_stream
 
-STREAMS = ("read", "copy", "triad", "write", "update")
-PEAKS = ("fp64_fma", "fp64_mfma", "int8_mfma")
+STREAMS = ("read", "copy", "copy_variants", "triad", "write", "update")
+PEAKS = ("fp64_fma", "fp64_mfma", "fp64_mfma_4x4x4", "int8_mfma")
 
 
 def measure(gib: float = 4.0, reps: int = 5) -> dict:
@@ -38,11 +40,13 @@ def measure(gib: float = 4.0, reps: int = 5) -> dict:
     nbytes = int(gib * (1 << 30))
     for k in STREAMS:
         out[k + "_GBs"] = round(mb_stream(k, nbytes, reps), 1)
-        print(f"{k:8s} {out[k + '_GBs']:9.1f} GB/s", flush=True)
+        print(f"{k:13s} {out[k + '_GBs']:9.1f} GB/s", flush=True)
     for k in PEAKS:
         unit = "TOP/s" if k.startswith("int8") else "TFLOP/s"
         out[k] = round(mb_peak(k, max(1, reps // 2)), 2)
-        print(f"{k:9s} {out[k]:9.2f} {unit}", flush=True)
+        out[k + "_clock_MHz"] = round(mb_clock(k), 0)
+        print(f"{k:15s} {out[k]:9.2f} {unit} at {out[k + '_clock_MHz']:.0f} MHz", flush=True)
+    out["idle_clock_MHz"] = round(mb_clock("idle"), 0)
     return out
 
 

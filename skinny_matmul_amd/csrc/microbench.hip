@@ -66,6 +66,59 @@ __global__ void __launch_bounds__(256)
   if (KIND == 0 && acc.x + acc.y == 12345.678) sink[threadIdx.x] = acc.x;
 }
 
+// Copy variants (kind 5 of skinny_mb_stream): with chunk == 0 a grid-stride
+// copy with U accesses of 16 bytes in flight per thread and the
+// nontemporal hint on the loads (NTL) and/or the stores (NTS); with
+// chunk > 0 each wave copies `chunk` consecutive elements per turn, U loads
+// per lane at a time (longer read and write bursts per wave).
+template <int U, bool NTL, bool NTS>
+__global__ void __launch_bounds__(256)
+    copy_variant_kernel(d2* __restrict__ a, const d2* __restrict__ b, int64_t n2, int64_t chunk) {
+  auto ld = [&](int64_t j) { return NTL ? __builtin_nontemporal_load(&b[j]) : b[j]; };
+  auto st = [&](int64_t j, d2 v) {
+    if (NTS)
+      __builtin_nontemporal_store(v, &a[j]);
+    else
+      a[j] = v;
+  };
+  if (chunk == 0) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += U * stride) {
+      d2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t j = i + u * stride;
+        v[u] = j < n2 ? ld(j) : d2{0, 0};
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t j = i + u * stride;
+        if (j < n2) st(j, v[u]);
+      }
+    }
+    return;
+  }
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int64_t nwaves = (int64_t)gridDim.x * 4;
+  for (int64_t c0 = wave * chunk; c0 < n2; c0 += nwaves * chunk) {
+    const int64_t cend = c0 + chunk < n2 ? c0 + chunk : n2;
+    for (int64_t i = c0 + lane; i < cend; i += 64 * U) {
+      d2 v[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t j = i + 64 * u;
+        v[u] = j < cend ? ld(j) : d2{0, 0};
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t j = i + 64 * u;
+        if (j < cend) st(j, v[u]);
+      }
+    }
+  }
+}
+
 template <typename F>
 float time_ms(F&& launch, hipStream_t st, int reps) {
   hipEvent_t e0, e1;
@@ -87,9 +140,20 @@ float time_ms(F&& launch, hipStream_t st, int reps) {
   return best;
 }
 
+// With clk set, the first lane of workgroup 0 records the shader-clock
+// counter (s_memtime) and the constant-rate real-time counter (s_memrealtime)
+// at its start (idx 0) and end (idx 1): clk[idx], clk[2 + idx].
+__device__ __forceinline__ void clock_stamp(unsigned long long* clk, int idx) {
+  if (clk && blockIdx.x == 0 && threadIdx.x == 0) {
+    clk[idx] = __builtin_amdgcn_s_memtime();
+    clk[2 + idx] = __builtin_amdgcn_s_memrealtime();
+  }
+}
+
 // FP64 FMA chains: 8 independent chains per lane
 __global__ void __launch_bounds__(256) fma_kernel(double* out, int iters,
-                                                  double x) {
+                                                  double x, unsigned long long* clk) {
+  clock_stamp(clk, 0);
   double v[8];
 #pragma unroll
   for (int j = 0; j < 8; j++) v[j] = x + j + threadIdx.x;
@@ -101,6 +165,7 @@ __global__ void __launch_bounds__(256) fma_kernel(double* out, int iters,
 #pragma unroll
   for (int j = 0; j < 8; j++) s += v[j];
   if (s == 12345.678) out[threadIdx.x] = s;
+  clock_stamp(clk, 1);
 }
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -109,7 +174,8 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 
 // FP64 matrix core: 4 independent accumulator chains per wave
 __global__ void __launch_bounds__(256) mfma_f64_kernel(double* out, int iters,
-                                                       double x) {
+                                                       double x, unsigned long long* clk) {
+  clock_stamp(clk, 0);
   v4d acc[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) acc[j] = v4d{0, 0, 0, 0};
@@ -123,11 +189,13 @@ __global__ void __launch_bounds__(256) mfma_f64_kernel(double* out, int iters,
 #pragma unroll
   for (int j = 0; j < 4; j++) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
   if (s == 12345.678) out[threadIdx.x] = s;
+  clock_stamp(clk, 1);
 }
 
 // INT8 matrix core (32x32x32, int32 accumulate): 2 chains per wave
 __global__ void __launch_bounds__(256) mfma_i8_kernel(int* out, int iters,
-                                                      int x) {
+                                                      int x, unsigned long long* clk) {
+  clock_stamp(clk, 0);
   v16i acc[2];
 #pragma unroll
   for (int j = 0; j < 2; j++)
@@ -146,6 +214,41 @@ __global__ void __launch_bounds__(256) mfma_i8_kernel(int* out, int iters,
 #pragma unroll
     for (int r = 0; r < 16; r++) s += acc[j][r];
   if (s == 12345) out[threadIdx.x] = s;
+  clock_stamp(clk, 1);
+}
+
+// FP64 matrix core, four-block 4x4x4 form: 16 independent chains per wave
+__global__ void __launch_bounds__(256) mfma_f64_q4_kernel(double* out, int iters,
+                                                          double x, unsigned long long* clk) {
+  clock_stamp(clk, 0);
+  double acc[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) acc[j] = 0.0;
+  double a = x + threadIdx.x, b = x - threadIdx.x;
+  for (int it = 0; it < iters; it++) {
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+      acc[j] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, acc[j], 0, 0, 0);
+  }
+  double s = 0;
+#pragma unroll
+  for (int j = 0; j < 16; j++) s += acc[j];
+  if (s == 12345.678) out[threadIdx.x] = s;
+  clock_stamp(clk, 1);
+}
+
+// The idle clock: one wave reads both counters about `spin` real-time ticks
+// apart (the loop ends once the real-time counter has advanced that far).
+__global__ void clock_kernel(unsigned long long* clk, int spin) {
+  if (threadIdx.x != 0) return;
+  const unsigned long long c0 = __builtin_amdgcn_s_memtime();
+  const unsigned long long r0 = __builtin_amdgcn_s_memrealtime();
+  unsigned long long r1 = r0;
+  while (r1 - r0 < (unsigned long long)spin) r1 = __builtin_amdgcn_s_memrealtime();
+  clk[0] = c0;
+  clk[1] = __builtin_amdgcn_s_memtime();
+  clk[2] = r0;
+  clk[3] = r1;
 }
 
 // ---- memory hierarchy: latency and bandwidth by working-set size ---------
@@ -257,7 +360,7 @@ extern "C" {
 
 skinny_status_t skinny_mb_stream(int kind, size_t bytes_per_array, int reps,
                                  double* gbs, skinny_stream_t stream) {
-  if (!gbs || kind < 0 || kind > 4 || bytes_per_array < 4096 || reps < 1)
+  if (!gbs || kind < 0 || kind > 5 || bytes_per_array < 4096 || reps < 1)
     return fail(SKINNY_INVALID_ARG, "skinny_mb_stream: bad arguments");
   skinny_handle_t h = skinny_default_handle();
   if (!h) return no_handle("microbench");
@@ -285,7 +388,47 @@ skinny_status_t skinny_mb_stream(int kind, size_t bytes_per_array, int reps,
   // with few workgroups; reads peak at 2 per CU with the hint, which took a
   // read stream from 6.4 to 7.1 TB/s)
   float ms = 1e30f;
+  if (kind == 5) {
+    // copy variants: nontemporal stores and/or loads (grid-stride, 4 in
+    // flight), and per-wave bursts of 16 KiB, 64 KiB or 1 MiB (8 in flight)
+    d2* A = (d2*)a;
+    const d2* B = (const d2*)b;
+    for (int per : {1, 2, 4, 8}) {
+      const int blocks = per * h->cu_count;
+      for (int v = 0; v < 4; v++) {
+        auto launch = [&]() {
+          if (v == 0)
+            hipLaunchKernelGGL((copy_variant_kernel<4, false, true>), dim3(blocks), dim3(256), 0,
+                               st, A, B, n2, (int64_t)0);
+          else if (v == 1)
+            hipLaunchKernelGGL((copy_variant_kernel<4, true, true>), dim3(blocks), dim3(256), 0,
+                               st, A, B, n2, (int64_t)0);
+          else if (v == 2)
+            hipLaunchKernelGGL((copy_variant_kernel<4, true, false>), dim3(blocks), dim3(256), 0,
+                               st, A, B, n2, (int64_t)0);
+          else
+            hipLaunchKernelGGL((copy_variant_kernel<4, false, false>), dim3(blocks), dim3(256), 0,
+                               st, A, B, n2, (int64_t)0);
+        };
+        ms = std::min(ms, time_ms(launch, st, reps));
+      }
+      for (int64_t chunk : {(int64_t)1024, (int64_t)4096, (int64_t)65536}) {
+        for (int ntl = 0; ntl < 2; ntl++) {
+          auto launch = [&]() {
+            if (ntl)
+              hipLaunchKernelGGL((copy_variant_kernel<8, true, false>), dim3(blocks), dim3(256),
+                                 0, st, A, B, n2, chunk);
+            else
+              hipLaunchKernelGGL((copy_variant_kernel<8, false, false>), dim3(blocks), dim3(256),
+                                 0, st, A, B, n2, chunk);
+          };
+          ms = std::min(ms, time_ms(launch, st, reps));
+        }
+      }
+    }
+  }
   for (int per : {1, 2, 4, 8}) {
+    if (kind == 5) break;
     for (int unroll : {1, 4}) {
      for (int nt : {0, 1}) {
       if (nt && kind == 3) continue;  // no loads
@@ -323,7 +466,7 @@ skinny_status_t skinny_mb_stream(int kind, size_t bytes_per_array, int reps,
     }
   }
   e = hipGetLastError();
-  const double arrays[5] = {1, 2, 3, 1, 3};  // bytes moved per array size
+  const double arrays[6] = {1, 2, 3, 1, 3, 2};  // bytes moved per array size
   *gbs = arrays[kind] * (double)nb / (ms * 1e-3) / 1e9;
   (void)hipFree(a);
   (void)hipFree(b);
@@ -468,11 +611,11 @@ skinny_status_t skinny_mb_barrier(int per_cu, int reps, double* us,
   return SKINNY_OK;
 }
 
-// kind 0: FP64 vector FMA, 1: FP64 MFMA 16x16x4, 2: INT8 MFMA 32x32x32.
-// Result in TFLOP/s (TOP/s for INT8).
+// kind 0: FP64 vector FMA, 1: FP64 MFMA 16x16x4, 2: INT8 MFMA 32x32x32,
+// 3: FP64 MFMA 4x4x4 (four blocks).  Result in TFLOP/s (TOP/s for INT8).
 skinny_status_t skinny_mb_peak(int kind, int reps, double* tflops,
                                skinny_stream_t stream) {
-  if (!tflops || kind < 0 || kind > 2 || reps < 1)
+  if (!tflops || kind < 0 || kind > 3 || reps < 1)
     return fail(SKINNY_INVALID_ARG, "skinny_mb_peak: bad arguments");
   skinny_handle_t h = skinny_default_handle();
   if (!h) return no_handle("microbench");
@@ -485,13 +628,16 @@ skinny_status_t skinny_mb_peak(int kind, int reps, double* tflops,
   auto launch = [&]() {
     if (kind == 0)
       hipLaunchKernelGGL(fma_kernel, dim3(blocks), dim3(256), 0, st,
-                         (double*)out, iters, 1.0);
+                         (double*)out, iters, 1.0, nullptr);
     else if (kind == 1)
       hipLaunchKernelGGL(mfma_f64_kernel, dim3(blocks), dim3(256), 0, st,
-                         (double*)out, iters, 1.0);
-    else
+                         (double*)out, iters, 1.0, nullptr);
+    else if (kind == 2)
       hipLaunchKernelGGL(mfma_i8_kernel, dim3(blocks), dim3(256), 0, st,
-                         (int*)out, iters, 1);
+                         (int*)out, iters, 1, nullptr);
+    else
+      hipLaunchKernelGGL(mfma_f64_q4_kernel, dim3(blocks), dim3(256), 0, st,
+                         (double*)out, iters, 1.0, nullptr);
   };
   float ms = time_ms(launch, st, reps);
   e = hipGetLastError();
@@ -503,9 +649,58 @@ skinny_status_t skinny_mb_peak(int kind, int reps, double* tflops,
     ops = threads * iters * 8 * 2;  // 8 chains, 2 flops per FMA
   else if (kind == 1)
     ops = threads / 64.0 * iters * 4 * (16.0 * 16 * 4 * 2);
-  else
+  else if (kind == 2)
     ops = threads / 64.0 * iters * 2 * (32.0 * 32 * 32 * 2);
+  else
+    ops = threads / 64.0 * iters * 16 * (4.0 * 4 * 4 * 4 * 2);
   *tflops = ops / (ms * 1e-3) / 1e12;
+  return SKINNY_OK;
+}
+
+// Shader clock (MHz) while the peak loop of `kind` (as skinny_mb_peak) runs,
+// from the counters workgroup 0 of that loop reads at its start and end;
+// kind -1: a one-wave spin of about 1 ms with nothing else running.
+skinny_status_t skinny_mb_clock(int kind, double* mhz, skinny_stream_t stream) {
+  if (!mhz || kind < -1 || kind > 3)
+    return fail(SKINNY_INVALID_ARG, "skinny_mb_clock: bad arguments");
+  skinny_handle_t h = skinny_default_handle();
+  if (!h) return no_handle("microbench");
+  int rate_khz = 0;
+  hipError_t e = hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, h->device);
+  if (e != hipSuccess) return hip_fail(e, "wall clock rate");
+  if (rate_khz <= 0) return fail(SKINNY_HIP_ERROR, "skinny_mb_clock: no wall clock rate");
+  hipStream_t st = (hipStream_t)stream;
+  void* out = nullptr;
+  unsigned long long* clk = nullptr;
+  e = hipMalloc(&out, 4096);
+  if (e == hipSuccess) e = hipMalloc(&clk, 32);
+  if (e != hipSuccess) {
+    (void)hipFree(out);
+    return hip_fail(e, "skinny_mb_clock alloc");
+  }
+  const int blocks = 8 * h->cu_count;
+  if (kind == 0)
+    hipLaunchKernelGGL(fma_kernel, dim3(blocks), dim3(256), 0, st, (double*)out, 16384, 1.0, clk);
+  else if (kind == 1)
+    hipLaunchKernelGGL(mfma_f64_kernel, dim3(blocks), dim3(256), 0, st, (double*)out, 4096, 1.0,
+                       clk);
+  else if (kind == 2)
+    hipLaunchKernelGGL(mfma_i8_kernel, dim3(blocks), dim3(256), 0, st, (int*)out, 4096, 1, clk);
+  else if (kind == 3)
+    hipLaunchKernelGGL(mfma_f64_q4_kernel, dim3(blocks), dim3(256), 0, st, (double*)out, 4096,
+                       1.0, clk);
+  else
+    hipLaunchKernelGGL(clock_kernel, dim3(1), dim3(64), 0, st, clk, rate_khz);  // 1 ms
+  e = hipGetLastError();
+  unsigned long long hc[4] = {0, 0, 0, 0};
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = hipMemcpy(hc, clk, 32, hipMemcpyDeviceToHost);
+  (void)hipFree(out);
+  (void)hipFree(clk);
+  if (e != hipSuccess) return hip_fail(e, "skinny_mb_clock");
+  const double ticks = (double)(hc[3] - hc[2]);
+  if (ticks <= 0) return fail(SKINNY_HIP_ERROR, "skinny_mb_clock: no ticks");
+  *mhz = (double)(hc[1] - hc[0]) / (ticks / (rate_khz * 1e3)) / 1e6;
   return SKINNY_OK;
 }
 

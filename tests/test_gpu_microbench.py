@@ -46,3 +46,18 @@ def test_grid_barrier_cost(gpu):
     us = gpu.mb_barrier(1, 1)
     assert us is not None and 0.1 < us < 1000.0, us
     assert gpu.mb_barrier(1000, 1) is None      # cannot be resident: refused, not launched
+
+
+def test_fp64_matrix_core_forms_and_clock(gpu):
+    """The four-block 4x4x4 FP64 form and the shader clock: every peak stays
+    below the datasheet figure scaled to the measured clock (78.6 TFLOP/s at
+    2400 MHz), and the clock readings lie between 100 MHz and the 2400 MHz
+    maximum engine clock."""
+    note("microbench: fp64 4x4x4 peak and clocks")
+    q4 = gpu.mb_peak("fp64_mfma_4x4x4", 1)
+    assert 10.0 < q4 < 78.6 * 1.02, q4
+    for kind in ("idle", "fp64_fma", "fp64_mfma", "fp64_mfma_4x4x4", "int8_mfma"):
+        mhz = gpu.mb_clock(kind)
+        assert 100.0 < mhz < 2400.0 * 1.05, (kind, mhz)
+    mhz = gpu.mb_clock("fp64_mfma_4x4x4")
+    assert q4 < 78.6 * (mhz / 2400.0) * 1.05, (q4, mhz)
