@@ -289,11 +289,14 @@ skinny_status_t skinny_fill_int(double* out, int64_t n, uint64_t seed,
 
 /* ---- autotuner ----------------------------------------------------------- */
 /* Time the unroll variants and grid sizes (0.5 .. 8 workgroups per CU) of
- * one FP64 shape class (op 0: TSMTTSM, 1: TSMM; dtype 0: double; M, N <= 64;
- * K class = floor(log2 K)) on scratch data and remember the fastest in the
- * handle: later calls with blockCount == 0 use it.  Replaces the hand-fitted
- * blends (tsmm/fix_blend.cuh:13-28, varip_blend.cuh:13-26) and perf.cu's
- * per-shape grid sweep (perf.cu:186-204).  Synchronous. */
+ * one shape class (op 0: TSMTTSM, 1: TSMM; dtype 0: double, 1: float,
+ * 2: complex double, 3: complex float, 4: HR (TSMTTSM only); M, N <= 1024,
+ * complex <= 512, complex widths for complex types; K class =
+ * floor(log2 K)) through the public entry points on scratch data, time the
+ * three fastest again interleaved, and remember the fastest in the handle:
+ * later calls of that type and shape with blockCount == 0 use it.  Replaces
+ * the hand-fitted blends (tsmm/fix_blend.cuh:13-28, varip_blend.cuh:13-26)
+ * and perf.cu's per-shape grid sweep (perf.cu:186-204).  Synchronous. */
 skinny_status_t skinny_autotune(skinny_handle_t h, int op, int dtype,
                                 int64_t M, int64_t N, int64_t K, int reps,
                                 double* best_ms);

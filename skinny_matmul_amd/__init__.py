@@ -449,11 +449,16 @@ def fill_int(out, n, seed, stream_id, lo=-2, hi=2, offset=0, stream=None) -> Non
                                  _stream(stream)), False)
 
 
-def autotune(op, M, N, K, reps=3, handle=None) -> float:
-    """Tune one FP64 shape class; returns the best time in ms."""
+TUNE_TYPES = {"DR": 0, "FR": 1, "DC": 2, "FC": 3, "HR": 4}
+
+
+def autotune(op, M, N, K, reps=3, handle=None, dtype="DR") -> float:
+    """Tune one shape class of one type ("DR", "FR", "DC", "FC", "HR"; complex
+    widths for complex types); returns the best time in ms."""
     kind = {"tsmttsm": 0, "tsmm": 1}.get(op, op)
     out = C.c_double()
-    _check(lib().skinny_autotune(_h(handle), kind, 0, M, N, K, reps, C.byref(out)), False)
+    _check(lib().skinny_autotune(_h(handle), kind, TUNE_TYPES.get(dtype, dtype), M, N, K,
+                                 reps, C.byref(out)), False)
     return out.value
 
 

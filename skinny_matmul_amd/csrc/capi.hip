@@ -89,7 +89,7 @@ skinny_status_t tsmttsm_cplx(const char* fn, skinny_handle_t h,
   h = resolve(h);
   if (!h) return no_handle(fn);
   TsmttsmPlan p = plan_tsmttsm(h, blockCount, 2 * M, 2 * N, K, (int)sizeof(T),
-                               (int)sizeof(T));
+                               (int)sizeof(T), sizeof(T) == 8 ? TT_DC : TT_FC, M, N);
   return launch_tsmttsm<T>(h, p, 2 * M, 2 * N, K, (const T*)A, 2 * lda,
                            alpha.x, alpha.y, (const T*)B, 2 * ldb, beta.x,
                            beta.y, (T*)C, ldc, true, (hipStream_t)stream);
@@ -163,7 +163,11 @@ size_t skinny_tsmttsm_workspace(skinny_handle_t h, size_t blockCount,
                                 int elem_bytes) {
   h = resolve(h);
   if (!h || M <= 0 || N <= 0) return 0;
-  TsmttsmPlan p = plan_tsmttsm(h, blockCount, M, N, K, elem_bytes, elem_bytes);
+  // complex double runs on the interleaved real view (2M x 2N, 8-byte
+  // partials), as the call itself plans it
+  TsmttsmPlan p = elem_bytes == 16
+                      ? plan_tsmttsm(h, blockCount, 2 * M, 2 * N, K, 8, 8, TT_DC, M, N)
+                      : plan_tsmttsm(h, blockCount, M, N, K, elem_bytes, elem_bytes);
   return p.scratch_bytes;
 }
 

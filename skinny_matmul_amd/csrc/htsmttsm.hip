@@ -218,6 +218,8 @@ skinny_status_t launch_htsmttsm(skinny_handle_t h, size_t blockCount,
   a.tilesN = (int)((N + a.tileN - 1) / a.tileN);
   const int ntiles = a.tilesM * a.tilesN;
   int64_t G = (int64_t)blockCount;
+  TuneEntry te;
+  if (G <= 0 && tune_lookup(h, 0, TT_HR, M, N, K, &te)) G = te.G;
   if (G <= 0) {
     int64_t target = (4LL * h->cu_count) / ntiles;
     if (target < 1) target = 1;

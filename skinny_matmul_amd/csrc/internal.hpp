@@ -68,6 +68,10 @@ skinny_status_t no_handle(const std::string& fn);
 skinny_status_t acquire_workspace(skinny_handle_t h, hipStream_t stream,
                                   size_t scratch_bytes, Workspace* out);
 
+// Drop the workspace of `stream` (a stream about to be destroyed): waits for
+// the stream, then frees its buffer.
+void release_workspace(skinny_handle_t h, hipStream_t stream);
+
 inline unsigned int* ws_counters(const Workspace& w) {
   return reinterpret_cast<unsigned int*>(w.base);
 }
@@ -77,9 +81,13 @@ inline skinny_handle_t resolve(skinny_handle_t h) {
   return h ? h : skinny_default_handle();
 }
 
-// autotuner table lookup (op 0 TSMTTSM, 1 TSMM; dtype 0 double)
+// autotuner table lookup (op 0 TSMTTSM, 1 TSMM; dtype a TuneType; M, N the
+// widths the caller passed, complex ones for complex types)
+enum TuneType { TT_DR = 0, TT_FR = 1, TT_DC = 2, TT_FC = 3, TT_HR = 4 };
 bool tune_lookup(skinny_handle_t h, int op, int dtype, int64_t M, int64_t N,
                  int64_t K, TuneEntry* out);
+void tune_store(skinny_handle_t h, int op, int dtype, int64_t M, int64_t N,
+                int64_t K, TuneEntry e);
 
 // ---- TSMTTSM ------------------------------------------------------------
 struct TsmttsmPlan {
@@ -97,8 +105,11 @@ struct TsmttsmPlan {
   size_t scratch_bytes = 0;
 };
 
+// tdtype, tM, tN: the autotuner key of the call (TuneType and its widths);
+// tdtype < 0 derives it from a real type of elem_bytes and M, N.
 TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
-                         int64_t N, int64_t K, int elem_bytes, int partial_bytes);
+                         int64_t N, int64_t K, int elem_bytes, int partial_bytes,
+                         int tdtype = -1, int64_t tM = 0, int64_t tN = 0);
 
 // Default grid of a real FP64 TSMTTSM that runs on the paired-row kernel
 // (one operand 5..16 wide, the other narrow), 0 for other shapes.
@@ -129,9 +140,12 @@ skinny_status_t launch_tsmttsm_real(skinny_handle_t h, const TsmttsmPlan& p,
 // Default grid of a TSMM call; real = false for the complex kernels, whose
 // whole-row variant covers only M, N <= 2 of the real view.  beta0 and
 // inplace select between the real kernels as the call does.
+// tdtype, tM, tN as for plan_tsmttsm (complex kernels pass their complex
+// widths; tdtype < 0 with real = false looks nothing up).
 int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
                       int elem_bytes, bool real = true, bool beta0 = true,
-                      bool inplace = false);
+                      bool inplace = false, int tdtype = -1, int64_t tM = 0,
+                      int64_t tN = 0);
 
 template <typename T>
 skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,

@@ -58,6 +58,15 @@ skinny_status_t acquire_workspace(skinny_handle_t h, hipStream_t stream,
   return SKINNY_OK;
 }
 
+void release_workspace(skinny_handle_t h, hipStream_t stream) {
+  std::lock_guard<std::mutex> lock(h->mu);
+  auto it = h->ws.find(stream);
+  if (it == h->ws.end()) return;
+  (void)hipStreamSynchronize(stream);
+  if (it->second.base) (void)hipFree(it->second.base);
+  h->ws.erase(it);
+}
+
 // ---- counter RNG (same function as the host generator) -------------------
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z += 0x9E3779B97F4A7C15ull;

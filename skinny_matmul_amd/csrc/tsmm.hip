@@ -410,9 +410,15 @@ static bool tm_rows_wins(int64_t M, int64_t N, bool inplace, bool beta0) {
 }
 
 int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
-                      int elem_bytes, bool real, bool beta0, bool inplace) {
+                      int elem_bytes, bool real, bool beta0, bool inplace,
+                      int tdtype, int64_t tM, int64_t tN) {
   TuneEntry te;
-  if (elem_bytes == 8 && tune_lookup(h, 1, 0, M, N, K, &te)) return te.G;
+  if (tdtype < 0 && real && (elem_bytes == 8 || elem_bytes == 4)) {
+    tdtype = elem_bytes == 8 ? TT_DR : TT_FR;
+    tM = M;
+    tN = N;
+  }
+  if (tdtype >= 0 && tune_lookup(h, 1, tdtype, tM, tN, K, &te)) return te.G;
   // measured on MI355X at K = 2^26 (fp64): the one-row-per-thread kernel
   // wants four workgroups per CU for M = 1 or beta != 0 (old C loaded too)
   // and two otherwise; the matrix-core kernel wants eight when A is at most
@@ -563,7 +569,9 @@ skinny_status_t launch_tsmm_cview(skinny_handle_t h, size_t blockCount,
   a.fa = T(1);
   a.fb = T(0);
   int blocks = blockCount > 0 ? (int)blockCount
-                              : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T), false);
+                              : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T), false, true,
+                                                  false, sizeof(T) == 8 ? TT_DC : TT_FC, Mc,
+                                                  Nc);
   bool beta0 = (beta == T(0));
   if (Mc == 1) {
     // one complex column: the real view is 2 x 2, whole rows per thread

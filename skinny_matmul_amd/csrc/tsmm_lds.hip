@@ -319,8 +319,11 @@ skinny_status_t launch_tsmm_lds(skinny_handle_t h, size_t blockCount,
     lds = (size_t)M * a.ns * sizeof(T);
   }
   if (lds < 16) lds = 16;
-  int blocks = blockCount > 0 ? (int)blockCount
-                              : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T), !cplx);
+  int blocks = blockCount > 0
+                   ? (int)blockCount
+                   : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T), !cplx, true, false,
+                                       cplx ? (sizeof(T) == 8 ? TT_DC : TT_FC) : -1, M / 2,
+                                       N / 2);
   const bool beta0 = cplx ? (beta == T(0) && beta_i == T(0)) : (beta == T(0));
   // At most 4 complex columns: whole rows per thread.  Wider rows make each
   // lane's loads span several cache lines and the LDS kernel is faster

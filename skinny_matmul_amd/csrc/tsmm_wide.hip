@@ -326,6 +326,11 @@ skinny_status_t launch_tsmm_wide(skinny_handle_t h, size_t blockCount,
   if (per_cu > 4) per_cu = 4;
   if (per_cu < 1) per_cu = 1;
   int64_t blocks = blockCount > 0 ? (int64_t)blockCount : per_cu * (int64_t)h->cu_count;
+  TuneEntry te;
+  if (blockCount == 0 &&
+      tune_lookup(h, 1, cplx ? (sizeof(T) == 8 ? TT_DC : TT_FC) : (sizeof(T) == 8 ? TT_DR : TT_FR),
+                  cplx ? M / 2 : M, cplx ? N / 2 : N, K, &te))
+    blocks = te.G;
   if (blocks > nblk) blocks = nblk;
   if (blocks < 1) blocks = 1;
   const bool beta0 = cplx ? (beta == T(0) && beta_i == T(0)) : (beta == T(0));

@@ -441,7 +441,7 @@ static int isqrt_ceil(int x) {
 
 TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
                          int64_t N, int64_t K, int elem_bytes,
-                         int partial_bytes) {
+                         int partial_bytes, int tdtype, int64_t tM, int64_t tN) {
   TsmttsmPlan p;
   int64_t W = M > N ? M : N;
   if (W <= 8) {
@@ -472,7 +472,9 @@ TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
   if (blockCount > 0) {
     G = (int64_t)blockCount;
     p.fixedG = true;
-  } else if (elem_bytes == 8 && tune_lookup(h, 0, 0, M, N, K, &te)) {
+  } else if ((tdtype >= 0 || elem_bytes == 8 || elem_bytes == 4) &&
+             tune_lookup(h, 0, tdtype >= 0 ? tdtype : (elem_bytes == 8 ? TT_DR : TT_FR),
+                         tdtype >= 0 ? tM : M, tdtype >= 0 ? tN : N, K, &te)) {
     G = te.G;
     p.uvar = te.uvar;
     p.tuned = true;

@@ -76,3 +76,47 @@ def test_default_tsmm_grid(gpu, eb, M, N, beta0, inplace, per_cu):
     # a short K caps the grid at one workgroup per 256 rows
     assert gpu.tuned_blockcount_tsmm(M, N, 1000, eb, handle=h, beta0=beta0,
                                      inplace=inplace) == 4
+
+
+def test_autotune_every_type(gpu, orc):
+    """Every type tunes through its own entry point and keeps its own table
+    entry; calls with blockCount 0 then run the tuned grid and stay exact on
+    integer data (FR, DC, FC) or within the double-double bound (HR)."""
+    import torch
+    h = gpu.Handle()
+    K = 1 << 16
+    M, N = 12, 20
+    for op, dt in (("tsmttsm", "FR"), ("tsmttsm", "DC"), ("tsmttsm", "FC"), ("tsmttsm", "HR"),
+                   ("tsmm", "FR"), ("tsmm", "DC"), ("tsmm", "FC")):
+        note(f"autotune {op} {dt} M={M} N={N} K={K}")
+        assert gpu.autotune(op, M, N, K, reps=1, handle=h, dtype=dt) > 0
+    lines = {tuple(l.split()[:4]) for l in gpu.tune_export(handle=h).strip().splitlines()}
+    for key in (("0", "1"), ("0", "2"), ("0", "3"), ("0", "4"), ("1", "1"), ("1", "2"), ("1", "3")):
+        assert key + (str(M), str(N)) in lines, (key, lines)
+    # tuned calls: complex double TSMTTSM and float TSMM on integer data
+    Ar = orc.integers(2 * K * M, 5, 0)
+    Br = orc.integers(2 * K * N, 5, 1)
+    Ac = Ar.view(np.complex128)
+    Bc = Br.view(np.complex128)
+    want = np.zeros(M * N, dtype=np.complex128)
+    orc.tsmttsm(M, N, K, Ac, M, 1.0, Bc, N, 0.0, want, M)
+    dC = dev(np.zeros(2 * M * N))
+    assert gpu.ztsmttsm(0, M, N, K, dev(Ar), M, 1.0, dev(Br), N, 0.0, dC, M, handle=h)
+    assert np.array_equal(host(dC).view(np.complex128), want)
+    A32 = orc.integers(K * M, 6, 0).astype(np.float32)
+    W32 = orc.integers(M * N, 6, 1).astype(np.float32)
+    want32 = np.zeros(K * N, dtype=np.float32)
+    orc.tsmm(M, N, K, A32, M, 1.0, W32, M, 0.0, want32, N)
+    dC32 = torch.zeros(K * N, dtype=torch.float32, device="cuda")
+    assert gpu.stsmm(0, M, N, K, dev(A32), M, 1.0, dev(W32), M, 0.0, dC32, N, handle=h)
+    assert np.array_equal(host(dC32), want32)
+    # HR with its tuned grid: within the double-double bound of the exact sum
+    A = orc.uniform(K * M, 7, 0)
+    B = orc.uniform(K * N, 7, 1)
+    hi, lo = orc.xtsmttsm(M, N, K, A, M, B, N)
+    Cdd = dev(np.zeros(2 * M * N))
+    assert gpu.htsmttsm_dd(0, M, N, K, dev(A), M, 1.0, dev(B), N, 0.0, Cdd, M, handle=h)
+    q = host(Cdd).reshape(-1, 2)
+    err = np.sqrt(np.sum(((q[:, 0] - hi) + (q[:, 1] - lo)) ** 2)) / np.sqrt(np.sum(hi ** 2))
+    assert err <= 1e-28, err
+    h.close()
