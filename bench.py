@@ -212,6 +212,9 @@ def extra_configs(sk, torch, logK: int) -> dict:
         views.append((Mb, Nb, Kb, a, b, w, c1, c2))
         nbytes += (Mb + Nb) * Kb * 8 + op_bytes("tsmm", Mb, Nb, Kb)
     batch = sk.Batch(ops)
+    ms_auto = _time_ms(torch, batch.run, 5)
+    # the batch's own tuner picks the item size and grid of the plan
+    _, tuned_ib, tuned_grid = batch.tune(reps=2)
     ms = _time_ms(torch, batch.run, 5)
     # per-instance error against an FP64 library product (rocBLAS through
     # torch): the whole M x N result of each TSMTTSM, 64 sampled rows of each
@@ -232,6 +235,8 @@ def extra_configs(sk, torch, logK: int) -> dict:
         bad += (e1 > 8 * Kb * eps) + (e2 > 8 * Mb * eps)
     out["c4"] = {"pairs": len(views), "ops": len(ops), "items": batch.items, "ms": round(ms, 3),
                  "GBs": round(nbytes / (ms * 1e-3) / 1e9, 1), "bytes": nbytes,
+                 "untuned_plan_ms": round(ms_auto, 3), "tuned_item_bytes": tuned_ib,
+                 "tuned_grid": tuned_grid,
                  "max_tsmttsm_err_over_K_eps": round(worst_ts, 3),
                  "max_tsmm_err_over_M_eps": round(worst_tm, 3),
                  "instances_over_8_eps_bound": int(bad)}

@@ -335,6 +335,16 @@ skinny_status_t skinny_dbatch_create(skinny_handle_t h, int nops,
                                      int64_t item_bytes, skinny_batch_t* batch);
 /* One launch; no allocation, memset or host synchronisation (graph-safe). */
 skinny_status_t skinny_dbatch_run(skinny_batch_t batch, skinny_stream_t stream);
+/* Autotune the batch's plan: time it (running it `reps` + 1 times each, on
+ * the null stream; the ops' outputs are overwritten) with items of the
+ * automatic size and of 1, 2, 4, 8 and 16 MiB, on grids of 1 and 2
+ * workgroups per CU, and keep the fastest (replaces the per-width ladder of
+ * multi_dispatch.cuh:3-90 for the choice of work split).  best_ms,
+ * item_bytes (0: automatic) and grid may be NULL.  Synchronous; a captured
+ * graph of an earlier plan must be captured again. */
+skinny_status_t skinny_dbatch_tune(skinny_batch_t batch, int reps,
+                                   double* best_ms, int64_t* item_bytes,
+                                   int* grid);
 skinny_status_t skinny_dbatch_destroy(skinny_batch_t batch);
 int64_t skinny_dbatch_items(skinny_batch_t batch);
 

@@ -119,6 +119,8 @@ _SIGS = {
     "skinny_tune_clear": (None, [vp]),
     "skinny_dbatch_create": (C.c_int, [vp, C.c_int, vp, i64, C.POINTER(vp)]),
     "skinny_dbatch_run": (C.c_int, [vp, vp]),
+    "skinny_dbatch_tune": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_int)]),
     "skinny_dbatch_destroy": (C.c_int, [vp]),
     "skinny_dbatch_items": (i64, [vp]),
     "skinny_dsvqb": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
@@ -501,6 +503,14 @@ class Batch:
 
     def run(self, stream=None) -> None:
         _check(lib().skinny_dbatch_run(self._b, _stream(stream)), False)
+
+    def tune(self, reps=3):
+        """Time plans of the batch (item sizes, grids) and keep the fastest;
+        returns (ms, item_bytes (0: automatic), grid)."""
+        ms, ib, g = C.c_double(), C.c_int64(), C.c_int()
+        _check(lib().skinny_dbatch_tune(self._b, reps, C.byref(ms), C.byref(ib), C.byref(g)),
+               False)
+        return ms.value, ib.value, g.value
 
     def close(self) -> None:
         if self._b:

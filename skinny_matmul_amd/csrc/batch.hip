@@ -99,15 +99,67 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 
 struct skinny_batch_s {
   int device = 0;
+  int cu_count = 256;
   int grid = 0;
   int nitems = 0;
   int nops = 0;
+  int64_t item_bytes = 0;  // as planned (0: the automatic size)
   size_t partial_bytes = 0;
+  std::vector<skinny_batch_op> ops;  // host copy, for re-planning
   skinny::BatchOpDev* d_ops = nullptr;
   skinny::BatchItemDev* d_items = nullptr;
   double* d_part = nullptr;
   unsigned* d_cnt = nullptr;  // nops counters + 2 queue counters
 };
+
+namespace {
+
+void batch_free_device(skinny_batch_t b) {
+  (void)hipFree(b->d_ops);
+  (void)hipFree(b->d_items);
+  (void)hipFree(b->d_part);
+  (void)hipFree(b->d_cnt);
+  b->d_ops = nullptr;
+  b->d_items = nullptr;
+  b->d_part = nullptr;
+  b->d_cnt = nullptr;
+}
+
+// Plan the batch's ops with `item_bytes` per item (0: automatic) for a grid of
+// `grid` workgroups and upload the plan (replacing any previous one).
+skinny_status_t batch_build(skinny_batch_t b, int64_t item_bytes, int grid) {
+  using namespace skinny;
+  std::vector<BatchOpDev> dops;
+  std::vector<BatchItemDev> items;
+  size_t part_doubles = 0;
+  skinny_status_t ps =
+      batch_plan(b->nops, b->ops.data(), item_bytes, grid, dops, items, part_doubles);
+  if (ps != SKINNY_OK) return ps;
+  batch_free_device(b);
+  b->nitems = (int)items.size();
+  b->grid = std::max(1, std::min(b->nitems, grid));
+  b->item_bytes = item_bytes;
+  const int nops = b->nops;
+  hipError_t e = hipSetDevice(b->device);
+  if (e == hipSuccess) e = hipMalloc(&b->d_ops, sizeof(BatchOpDev) * nops);
+  if (e == hipSuccess) e = hipMalloc(&b->d_items, sizeof(BatchItemDev) * std::max<size_t>(1, items.size()));
+  if (e == hipSuccess) e = hipMalloc(&b->d_part, sizeof(double) * std::max<size_t>(1, part_doubles));
+  if (e == hipSuccess) e = hipMalloc(&b->d_cnt, sizeof(unsigned) * (nops + 2));
+  if (e == hipSuccess) e = hipMemcpy(b->d_ops, dops.data(), sizeof(BatchOpDev) * nops, hipMemcpyHostToDevice);
+  if (e == hipSuccess && !items.empty())
+    e = hipMemcpy(b->d_items, items.data(), sizeof(BatchItemDev) * items.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(b->d_cnt, 0, sizeof(unsigned) * (nops + 2));
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) {
+    batch_free_device(b);
+    b->nitems = 0;
+    return hip_fail(e, "batch plan upload");
+  }
+  b->partial_bytes = part_doubles * sizeof(double);
+  return SKINNY_OK;
+}
+
+}  // namespace
 
 using namespace skinny;
 
@@ -116,10 +168,7 @@ extern "C" {
 skinny_status_t skinny_dbatch_destroy(skinny_batch_t b) {
   if (!b) return SKINNY_OK;
   (void)hipDeviceSynchronize();
-  (void)hipFree(b->d_ops);
-  (void)hipFree(b->d_items);
-  (void)hipFree(b->d_part);
-  (void)hipFree(b->d_cnt);
+  batch_free_device(b);
   delete b;
   return SKINNY_OK;
 }
@@ -132,32 +181,66 @@ skinny_status_t skinny_dbatch_create(skinny_handle_t h, int nops,
     return fail(SKINNY_INVALID_ARG, "batch: no ops");
   h = resolve(h);
   if (!h) return no_handle("batch");
-  std::vector<BatchOpDev> dops;
-  std::vector<BatchItemDev> items;
-  size_t part_doubles = 0;
-  skinny_status_t ps = batch_plan(nops, ops, item_bytes, 2 * h->cu_count, dops, items,
-                                  part_doubles);
-  if (ps != SKINNY_OK) return ps;
   skinny_batch_t b = new skinny_batch_s();
   b->device = h->device;
+  b->cu_count = h->cu_count;
   b->nops = nops;
-  b->nitems = (int)items.size();
-  b->grid = std::max(1, std::min(b->nitems, 2 * h->cu_count));
-  hipError_t e = hipMalloc(&b->d_ops, sizeof(BatchOpDev) * nops);
-  if (e == hipSuccess) e = hipMalloc(&b->d_items, sizeof(BatchItemDev) * std::max<size_t>(1, items.size()));
-  if (e == hipSuccess) e = hipMalloc(&b->d_part, sizeof(double) * std::max<size_t>(1, part_doubles));
-  if (e == hipSuccess) e = hipMalloc(&b->d_cnt, sizeof(unsigned) * (nops + 2));
-  if (e == hipSuccess) e = hipMemcpy(b->d_ops, dops.data(), sizeof(BatchOpDev) * nops, hipMemcpyHostToDevice);
-  if (e == hipSuccess && !items.empty())
-    e = hipMemcpy(b->d_items, items.data(), sizeof(BatchItemDev) * items.size(), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(b->d_cnt, 0, sizeof(unsigned) * (nops + 2));
-  if (e == hipSuccess) e = hipDeviceSynchronize();
-  if (e != hipSuccess) {
+  b->ops.assign(ops, ops + nops);
+  skinny_status_t s = batch_build(b, item_bytes, 2 * h->cu_count);
+  if (s != SKINNY_OK) {
     skinny_dbatch_destroy(b);
-    return hip_fail(e, "batch create");
+    return s;
   }
-  b->partial_bytes = part_doubles * sizeof(double);
   *out = b;
+  return SKINNY_OK;
+}
+
+// Times plans of the batch (item sizes: automatic, 1, 2, 4, 8 and 16 MiB;
+// 1 or 2 workgroups per CU, the kernel's residency), keeps the fastest.
+// TSMTTSM results follow the plan's partition: reproducible for a fixed plan.
+skinny_status_t skinny_dbatch_tune(skinny_batch_t b, int reps, double* best_ms,
+                                   int64_t* item_bytes, int* grid) {
+  if (!b) return fail(SKINNY_INVALID_ARG, "batch tune: null batch");
+  if (reps < 1) reps = 3;
+  hipEvent_t e0, e1;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e != hipSuccess) return hip_fail(e, "batch tune events");
+  const int64_t sizes[] = {0, 1 << 20, 2 << 20, 4 << 20, 8 << 20, 16 << 20};
+  float best = 1e30f;
+  int64_t best_ib = 0;
+  int best_g = 2 * b->cu_count;
+  skinny_status_t s = SKINNY_OK;
+  for (int per = 1; per <= 2 && s == SKINNY_OK; per++) {
+    for (int64_t ib : sizes) {
+      s = batch_build(b, ib, per * b->cu_count);
+      if (s != SKINNY_OK) break;
+      std::vector<float> t;
+      for (int r = 0; r <= reps && s == SKINNY_OK; r++) {
+        (void)hipEventRecord(e0, nullptr);
+        s = skinny_dbatch_run(b, nullptr);
+        (void)hipEventRecord(e1, nullptr);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (r > 0) t.push_back(ms);  // the first run warms up
+      }
+      if (s != SKINNY_OK) break;
+      std::sort(t.begin(), t.end());
+      if (t[t.size() / 2] < best) {
+        best = t[t.size() / 2];
+        best_ib = ib;
+        best_g = per * b->cu_count;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (s == SKINNY_OK) s = batch_build(b, best_ib, best_g);
+  if (s != SKINNY_OK) return s;
+  if (best_ms) *best_ms = best;
+  if (item_bytes) *item_bytes = best_ib;
+  if (grid) *grid = b->grid;
   return SKINNY_OK;
 }
 

@@ -147,3 +147,27 @@ def test_batch_graph_replay(gpu, orc):
             assert np.array_equal(host(dC), want), (rep, kind, M, N, K)
         del junk
     b.close()
+
+
+def test_batch_tune_replans_and_stays_exact(gpu, orc):
+    """skinny_dbatch_tune times plans over item sizes and grids and keeps the
+    fastest; the tuned batch (outputs restored after the tuning runs) gives
+    the oracle's results bitwise on integer data, and runs again."""
+    import torch
+    rng = random.Random(7)
+    ops, refs = _make_ops(orc, rng, 40, 200000)
+    b = gpu.Batch(ops)
+    note(f"batch tune: {len(ops)} ops, {b.items} items")
+    ms, ib, g = b.tune(reps=1)
+    cus = gpu.cu_count()
+    assert ms > 0
+    assert ib in (0, 1 << 20, 2 << 20, 4 << 20, 8 << 20, 16 << 20)
+    assert 1 <= g <= 2 * cus
+    for _ in range(2):
+        for (kind, M, N, K, C0, want, dC) in refs:
+            dC.copy_(torch.from_numpy(C0))
+        torch.cuda.synchronize()
+        b.run()
+        for (kind, M, N, K, C0, want, dC) in refs:
+            assert np.array_equal(host(dC), want), (kind, M, N, K, ib, g)
+    b.close()
