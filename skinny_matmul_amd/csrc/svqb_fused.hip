@@ -116,40 +116,48 @@ skinny_status_t skinny_dsvqb_fused(skinny_handle_t h, int nblocks,
     toff[i] = tdoubles;
     tdoubles += (size_t)blocks[i].M * blocks[i].M;
   }
-  double* dT = nullptr;
-  hipError_t e = hipMalloc(&dT, sizeof(double) * tdoubles);
-  if (e != hipSuccess) return hip_fail(e, "svqb fused alloc");
-
+  // the plan needs T's address only for the TSMM op records: plan with
+  // offsets from a null base, then rebase the records on the workspace
   std::vector<skinny_batch_op> ops(2 * nblocks);
   for (int i = 0; i < nblocks; i++) {
     const skinny_svqb_block& b = blocks[i];
     ops[i] = skinny_batch_op{0, b.M, b.M, b.K, b.X, b.ldx, 1.0, b.X, b.ldx, 0.0, b.X, b.ldx};
-    ops[nblocks + i] = skinny_batch_op{1, b.M, b.M, b.K, b.X, b.ldx, 1.0, dT + toff[i],
-                                       b.M, 0.0, b.X, b.ldx};
+    ops[nblocks + i] = skinny_batch_op{1, b.M, b.M, b.K, b.X, b.ldx, 1.0,
+                                       (const double*)(uintptr_t)(8 * (toff[i] + 1)), b.M, 0.0,
+                                       b.X, b.ldx};
   }
   std::vector<BatchOpDev> dops;
   std::vector<BatchItemDev> items, gitems, titems;
   size_t part_doubles = 0;
   skinny_status_t s = batch_plan(2 * nblocks, ops.data(), 0, 2 * h->cu_count, dops, items,
                                  part_doubles);
-  if (s != SKINNY_OK) {
-    (void)hipFree(dT);
-    return s;
-  }
+  if (s != SKINNY_OK) return s;
   for (const BatchItemDev& it : items) (it.op < nblocks ? gitems : titems).push_back(it);
 
-  BatchOpDev* d_ops = nullptr;
-  BatchItemDev* d_items = nullptr;
-  double* d_part = nullptr;
-  unsigned* d_bar = nullptr;
+  // every per-call buffer lives in the stream's workspace (it grows on the
+  // first call of a size and is reused afterwards): transforms, op records,
+  // items, Gram partials and the barrier counters
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t nit = gitems.size() + titems.size();
-  e = hipMalloc(&d_ops, sizeof(BatchOpDev) * dops.size());
-  if (e == hipSuccess) e = hipMalloc(&d_items, sizeof(BatchItemDev) * std::max<size_t>(1, nit));
-  if (e == hipSuccess) e = hipMalloc(&d_part, sizeof(double) * std::max<size_t>(1, part_doubles));
-  if (e == hipSuccess) e = hipMalloc(&d_bar, sizeof(unsigned) * 4);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(d_ops, dops.data(), sizeof(BatchOpDev) * dops.size(),
-                       hipMemcpyHostToDevice, st);
+  const size_t oT = 0;
+  const size_t oOps = oT + up(sizeof(double) * tdoubles);
+  const size_t oItems = oOps + up(sizeof(BatchOpDev) * dops.size());
+  const size_t oPart = oItems + up(sizeof(BatchItemDev) * std::max<size_t>(1, nit));
+  const size_t oBar = oPart + up(sizeof(double) * std::max<size_t>(1, part_doubles));
+  const size_t need = oBar + 256;
+  Workspace w;
+  s = acquire_workspace(h, st, need, &w);
+  if (s != SKINNY_OK) return s;
+  char* base = static_cast<char*>(ws_scratch(w));
+  double* dT = reinterpret_cast<double*>(base + oT);
+  BatchOpDev* d_ops = reinterpret_cast<BatchOpDev*>(base + oOps);
+  BatchItemDev* d_items = reinterpret_cast<BatchItemDev*>(base + oItems);
+  double* d_part = reinterpret_cast<double*>(base + oPart);
+  unsigned* d_bar = reinterpret_cast<unsigned*>(base + oBar);
+  for (int i = 0; i < nblocks; i++) dops[nblocks + i].B = dT + toff[i];
+
+  hipError_t e = hipMemcpyAsync(d_ops, dops.data(), sizeof(BatchOpDev) * dops.size(),
+                                hipMemcpyHostToDevice, st);
   if (e == hipSuccess && !gitems.empty())
     e = hipMemcpyAsync(d_items, gitems.data(), sizeof(BatchItemDev) * gitems.size(),
                        hipMemcpyHostToDevice, st);
@@ -182,15 +190,10 @@ skinny_status_t skinny_dsvqb_fused(skinny_handle_t h, int nblocks,
                                    args, 0, st);
     if (e == hipSuccess)
       e = hipMemcpyAsync(&flag, d_bar + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st);
-    // the per-call buffers must outlive the kernel
+    // the host copies of the plan and the flag must outlive the stream's use
     const hipError_t e2 = hipStreamSynchronize(st);
     if (e == hipSuccess) e = e2;
   }
-  (void)hipFree(d_ops);
-  (void)hipFree(d_items);
-  (void)hipFree(d_part);
-  (void)hipFree(d_bar);
-  (void)hipFree(dT);
   if (e != hipSuccess) return hip_fail(e, "svqb fused launch");
   if (flag) return fail(SKINNY_HIP_ERROR, "svqb fused: a grid barrier timed out");
   return SKINNY_OK;
