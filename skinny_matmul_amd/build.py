@@ -99,7 +99,7 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
 # driver -> (source, extra link libraries); all link libskinny
 DRIVER_TARGETS = {
     "skinny_test": ("test.cpp", ["-lrocblas"]),
-    "skinny_perf": ("perf.cpp", ["-l:libsqlite3.so.0"]),
+    "skinny_perf": ("perf.cpp", ["-l:libsqlite3.so.0", "-lrocblas"]),
     "skinny_numeric_tests": ("numeric_tests.cpp", []),
     "skinny_svqb": ("svqb.cpp", []),
 }

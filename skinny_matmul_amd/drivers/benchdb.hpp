@@ -41,6 +41,8 @@ int sqlite3_bind_double(sqlite3_stmt* stmt, int idx, double value);
 int sqlite3_step(sqlite3_stmt* stmt);
 int sqlite3_finalize(sqlite3_stmt* stmt);
 const unsigned char* sqlite3_column_text(sqlite3_stmt* stmt, int col);
+long long sqlite3_column_int64(sqlite3_stmt* stmt, int col);
+double sqlite3_column_double(sqlite3_stmt* stmt, int col);
 }
 
 namespace drv {
@@ -138,6 +140,49 @@ class BenchDB {
     sqlite3_bind_text(st, 1, device.c_str(), -1, sqlite_transient());
     sqlite3_bind_text(st, 2, text.c_str(), -1, sqlite_transient());
     step_done(st);
+  }
+
+  // A stored measurement: the configuration to replay it.
+  struct Stored {
+    int M = 0, N = 0, inplace = 0, zerobeta = 0;
+    std::string name, branch, usr1_name, usr1_val;
+    long long blockCount = 0, K = 0;
+    double time = 0.0;
+  };
+  // Rows of one operation, device and type set, in a fixed order.
+  std::vector<Stored> rows(const std::string& multype, const std::string& device,
+                           const std::string& types) {
+    std::vector<Stored> out;
+    if (!db_) return out;
+    sqlite3_stmt* st = prepare(
+        "SELECT M, N, name, inplace, zerobeta, blockCount, K, time, branch, usr1_name, "
+        "usr1_val FROM benchmarks WHERE multype = ?1 AND device = ?2 AND types = ?3 "
+        "ORDER BY M, N, name, inplace, zerobeta, branch, usr1_name, usr1_val");
+    if (!st) return out;
+    sqlite3_bind_text(st, 1, multype.c_str(), -1, sqlite_transient());
+    sqlite3_bind_text(st, 2, device.c_str(), -1, sqlite_transient());
+    sqlite3_bind_text(st, 3, types.c_str(), -1, sqlite_transient());
+    while (sqlite3_step(st) == kSqliteRow) {
+      Stored r;
+      auto text = [&](int c) {
+        const unsigned char* t = sqlite3_column_text(st, c);
+        return t ? std::string(reinterpret_cast<const char*>(t)) : std::string();
+      };
+      r.M = (int)sqlite3_column_int64(st, 0);
+      r.N = (int)sqlite3_column_int64(st, 1);
+      r.name = text(2);
+      r.inplace = (int)sqlite3_column_int64(st, 3);
+      r.zerobeta = (int)sqlite3_column_int64(st, 4);
+      r.blockCount = sqlite3_column_int64(st, 5);
+      r.K = sqlite3_column_int64(st, 6);
+      r.time = sqlite3_column_double(st, 7);
+      r.branch = text(8);
+      r.usr1_name = text(9);
+      r.usr1_val = text(10);
+      out.push_back(r);
+    }
+    sqlite3_finalize(st);
+    return out;
   }
 
   std::string load_tuning(const std::string& device) {

@@ -19,15 +19,31 @@
 //              beta = 1: (M K + 2 N K + M N) s / t (in place: one N K),
 //                        (M N K + N K) fpc / t.
 //   The reference's CUPTI columns (l2bw, sharedbw, occupancy, drambw) are
-//   stored as 0; HBM traffic is measured with rocprofv3 counters instead.
-// * --tune runs the library autotuner for each FP64 shape (M, N <= 64)
+//   stored as 0 by the sweep; --counters fills drambw and occupancy later.
+// * Variants: the library's registry (SKINNY, OZAKI3, SKINNY_IP) and, for
+//   every type but HR, ROCBLAS: the vendor library's gemm with the same
+//   operand mapping, timed like the others (the CUBLAS entries of the
+//   reference registries, versions.hpp:70-71, 105-106); it takes no grid.
+// * --tune runs the library autotuner for each shape of the type
 //   before timing and keeps the tuned table in BenchDB's `tuning` table; the
 //   stored table is loaded at start.
+// * --counters MARKS replays the rows already stored for this operation,
+//   device and type set, once each, for a profiler collecting hardware
+//   counters per dispatch (rocprofv3 --pmc): first one HBM copy
+//   microbenchmark over 1 GiB arrays (the byte-counter calibration), then per
+//   row a one-element fill_uniform (start marker), the row's call with its
+//   stored K, blockCount, beta and aliasing, and a one-element fill_int (end
+//   marker).  MARKS gets one tab-separated line per replayed row (the row's
+//   key columns and stored time).  skinny_matmul_amd.benchdb_counters turns
+//   the counters into the drambw and occupancy columns.
 #include <iomanip>
 #include <iostream>
 
+#include <fstream>
+
 #include "benchdb.hpp"
 #include "common.hpp"
+#include "rocblas_ref.hpp"
 
 #ifndef SKINNY_BRANCH
 #define SKINNY_BRANCH "main"
@@ -39,9 +55,18 @@ struct Opts {
   std::string db = "benchmarks.db";
   size_t mib = 4096;
   bool tune = false;
+  std::string counters;  // marks file of a --counters replay
   std::string usr1_name, usr1_val;
   drv::Range range;
 };
+
+constexpr int kTuneDtype(const char* name) {
+  return name[0] == 'D' && name[1] == 'R' ? 0
+         : name[0] == 'F' && name[1] == 'R' ? 1
+         : name[0] == 'D' && name[1] == 'C' ? 2
+         : name[0] == 'F' && name[1] == 'C' ? 3
+                                            : 4;
+}
 
 struct Perf {
   Opts o;
@@ -67,8 +92,17 @@ struct Perf {
 
     auto versions = is_tsmm ? skinny::tsmm_versions<Types>()
                             : skinny::tsmttsm_versions<Types>();
+    const std::string tname = Types::name();
+    if (tname != "HR") {
+      if (is_tsmm)
+        versions.push_back({drv::tsmm_rocblas<T>, "ROCBLAS"});
+      else
+        versions.push_back({drv::tsmttsm_rocblas<T>, "ROCBLAS"});
+    }
     skinny::set_abort_on_error(true);
     const double s = sizeof(T), fpc = Types::flopsPerCell;
+    if (!o.counters.empty())
+      return replay(db, versions, is_tsmm, multype, device, tname, A, B, C, total);
 
     for (int M = o.range.m1; M <= o.range.m2; M++) {
       const int n1 = o.range.n1 ? o.range.n1 : M, n2 = o.range.n2 ? o.range.n2 : M;
@@ -102,16 +136,18 @@ struct Perf {
             t = measure(K, 0, false, 1.0);
           }
           if (t < 0) continue;  // variant does not support this width
-          if (o.tune && std::is_same<T, double>::value && M <= 64 && N <= 64) {
+          if (o.tune && ver.second == "SKINNY") {
             double best = 0;
-            SKINNY_CHECK(skinny_autotune(nullptr, is_tsmm ? 1 : 0, 0, M, N, K, 3, &best));
+            SKINNY_CHECK(skinny_autotune(nullptr, is_tsmm ? 1 : 0, kTuneDtype(Types::name()),
+                                         M, N, K, 3, &best));
           }
           const int selfs = (M == N || is_tsmm) ? 1 : 0;
           for (int self = 0; self <= selfs; self++) {
             if (self && is_tsmm) ldc = lda = std::max(M, N);
             for (double beta = is_tsmm ? 0.0 : 1.0; beta <= 1.0; beta += 1.0) {
               std::vector<size_t> counts = {0};
-              for (int k = 1; k <= 8; k++) counts.push_back((size_t)k * cus);
+              if (ver.second != "ROCBLAS")  // the vendor library takes no grid
+                for (int k = 1; k <= 8; k++) counts.push_back((size_t)k * cus);
               double bestT = -1;
               size_t bestBC = 0;
               for (size_t bc : counts) {
@@ -172,13 +208,67 @@ struct Perf {
     db.flush();
     return 0;
   }
+
+  // --counters: replay the stored rows between marker kernels (see above).
+  template <class VL, class T>
+  int replay(drv::BenchDB& db, const VL& versions, bool is_tsmm, const std::string& multype,
+             const std::string& device, const std::string& tname, drv::DevBuf<T>& A,
+             drv::DevBuf<T>& B, drv::DevBuf<T>& C, size_t total) {
+    std::ofstream marks(o.counters);
+    if (!marks) {
+      std::fprintf(stderr, "cannot write %s\n", o.counters.c_str());
+      return 3;
+    }
+    double gbs = 0.0;
+    SKINNY_CHECK(skinny_mb_stream(1, (size_t)1 << 30, 1, &gbs, nullptr));
+    HIP_CHECK(hipDeviceSynchronize());
+    drv::DevBuf<double> mark(1);
+    const T two = drv::make<T>(2.0);
+    int n = 0;
+    for (const auto& r : db.rows(multype, device, tname)) {
+      if (r.M < o.range.m1 || r.M > o.range.m2) continue;
+      const auto* ver = &versions[0];
+      bool found = false;
+      for (const auto& v : versions)
+        if (v.second == r.name) {
+          ver = &v;
+          found = true;
+        }
+      if (!found) continue;
+      const int M = r.M, N = r.N;
+      int64_t lda = M, ldb = is_tsmm ? M : N, ldc = is_tsmm ? N : M;
+      if (r.inplace && is_tsmm) ldc = lda = std::max(M, N);
+      if ((size_t)r.K * std::max(lda, is_tsmm ? ldc : ldb) > total) continue;
+      const T bt = drv::make<T>(r.zerobeta ? 0.0 : 1.0);
+      SKINNY_CHECK(skinny_fill_uniform(mark.p, 1, 1, 0, 0, nullptr));
+      bool ok;
+      if (!r.inplace)
+        ok = ver->first((size_t)r.blockCount, M, N, (int)r.K, A.p, (int)lda, two, B.p,
+                        (int)ldb, bt, C.p, (int)ldc);
+      else if (is_tsmm)
+        ok = ver->first((size_t)r.blockCount, M, N, (int)r.K, C.p, (int)ldc, two, B.p,
+                        (int)ldb, bt, C.p, (int)ldc);
+      else
+        ok = ver->first((size_t)r.blockCount, M, N, (int)r.K, A.p, (int)lda, two, A.p,
+                        (int)lda, bt, C.p, (int)ldc);
+      SKINNY_CHECK(skinny_fill_int(mark.p, 1, 1, 0, 0, 0, 1, nullptr));
+      HIP_CHECK(hipDeviceSynchronize());
+      if (!ok) continue;
+      marks << multype << '\t' << device << '\t' << tname << '\t' << M << '\t' << N << '\t'
+            << r.name << '\t' << r.inplace << '\t' << r.zerobeta << '\t' << r.branch << '\t'
+            << r.usr1_name << '\t' << r.usr1_val << '\t' << r.time << '\n';
+      n++;
+    }
+    std::cout << "replayed " << n << " rows; marks in " << o.counters << "\n";
+    return 0;
+  }
 };
 
 int usage() {
   std::fprintf(stderr,
                "usage: skinny_perf <tsmttsm|tsmm> <FR|DR|FC|DC|HR> "
                "[m2 | m1 m2 | m1 m2 n1 n2] [--db FILE] [--mib MB] [--tune] "
-               "[--usr1 NAME=VALUE]\n");
+               "[--usr1 NAME=VALUE] [--counters MARKS]\n");
   return 2;
 }
 
@@ -195,6 +285,7 @@ int main(int argc, char** argv) {
     if (a == "--db" && i + 1 < argc) p.o.db = argv[++i];
     else if (a == "--mib" && i + 1 < argc) p.o.mib = std::stoul(argv[++i]);
     else if (a == "--tune") p.o.tune = true;
+    else if (a == "--counters" && i + 1 < argc) p.o.counters = argv[++i];
     else if (a == "--usr1" && i + 1 < argc) {
       const std::string kv = argv[++i];
       const size_t eq = kv.find('=');
