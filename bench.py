@@ -547,6 +547,10 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": (f"profiles/traffic.json[{kernel_key}] (rocprofv3 FETCH_SIZE and "
+                               f"WRITE_SIZE, calibrated on a copy)" if traffic is not None else
+                               f"none: profiles/traffic.json holds no counter summary for "
+                               f"{kernel_key}"),
         },
         "cpu_baseline": cpu,
         "configs": extras,

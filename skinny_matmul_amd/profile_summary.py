@@ -14,7 +14,7 @@ TSMM M=N=64.  The last sweeps are taken from the end of the trace: every
 product kernel dispatch is one call, and a TSMTTSM call may be followed by
 its partial-sum kernel (tsmttsm_reduce_kernel).  Prints a markdown table of
 average durations per call and GB/s of algorithmic bytes (perf.cu:209-231
-accounting: (M + N) K 8 bytes for either product).
+accounting: (M + N) K 8 bytes for either product, plus M N 8 for TSMM).
 
 With --svqb the summary covers the SVQB measurement instead: the hipGraph
 replays of two sweeps over 64 blocks (K = 2^22, M = 8, 16, 32, 64 in turn),
@@ -141,7 +141,8 @@ def main() -> int:
         names = sorted({c[0] for c in mine})
         prod = sum(c[1] for c in mine) / len(mine)
         red = sum(c[2] for c in mine) / len(mine)
-        nbytes = (M + M) * K * 8
+        # perf.cu:209-231: TSMM also writes C once per call (M N 8 bytes)
+        nbytes = (M + M) * K * 8 + (M * M * 8 if op == "tsmm" else 0)
         total_ns += (prod + red) * len(mine)
         total_bytes += nbytes * len(mine)
         print(f"| {op} M=N={M} | `{'`, `'.join(names)}` | {len(mine)} | {prod / 1e6:.3f} | "

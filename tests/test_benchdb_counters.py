@@ -99,3 +99,40 @@ def test_merge_refuses_misaligned_runs(tmp_path):
     _write_counters(w, _run("WRITE_SIZE", {"WRITE_SIZE": gib_kib}, [[{"WRITE_SIZE": 1.0}]]))
     with pytest.raises(SystemExit, match="segment counts differ"):
         bc.merge(str(tmp_path / "x.db"), str(marks), str(f), str(w), dry_run=True)
+
+
+def test_profile_traffic_per_marked_call(tmp_path):
+    """profile_traffic: calibrated bytes per marked call, kernels listed,
+    entries merged into an existing traffic file."""
+    import json
+    from skinny_matmul_amd import profile_traffic as pt
+    marks = tmp_path / "m.tsv"
+    marks.write_text("tsmttsm_f64_M8_K2^20\ttsmttsm\t8\t134217728\n"
+                     "tsmm_f64_M8_K2^20\ttsmm\t8\t134218240\n")
+    gib_kib = (1 << 30) / 1024
+    f = tmp_path / "f.csv"
+    _write_counters(f, _run("FETCH_SIZE", {"FETCH_SIZE": gib_kib / 2},
+                            [[{"FETCH_SIZE": 65536.0}, {"FETCH_SIZE": 8.0}],
+                             [{"FETCH_SIZE": 32768.0}]]))
+    w = tmp_path / "w.csv"
+    _write_counters(w, _run("WRITE_SIZE", {"WRITE_SIZE": gib_kib},
+                            [[{"WRITE_SIZE": 4.0}, {"WRITE_SIZE": 1.0}],
+                             [{"WRITE_SIZE": 65536.0}]]))
+    ff, wf, rows = pt.summarize(str(marks), str(f), str(w))
+    assert ff == pytest.approx(2.0) and wf == pytest.approx(1.0)
+    assert rows[0]["hbm"] == pytest.approx((2 * (65536 + 8) + 5) * 1024)
+    assert rows[1]["hbm"] == pytest.approx((2 * 32768 + 65536) * 1024)
+    assert rows[0]["kernels"] == ["tsmttsm_kernel<...>"] * 2
+    js = tmp_path / "t.json"
+    js.write_text(json.dumps({"other": {"hbm_bytes_per_launch": 1.0}}))
+    import sys
+    argv = sys.argv
+    sys.argv = ["x", "--marks", str(marks), "--fetch", str(f), "--write", str(w),
+                "--json", str(js)]
+    try:
+        assert pt.main() == 0
+    finally:
+        sys.argv = argv
+    d = json.loads(js.read_text())
+    assert set(d) == {"other", "tsmttsm_f64_M8_K2^20", "tsmm_f64_M8_K2^20"}
+    assert d["tsmm_f64_M8_K2^20"]["hbm_bytes_per_launch"] == pytest.approx(rows[1]["hbm"])
