@@ -745,11 +745,14 @@ void orc_gram_error(int64_t M, int64_t K, const double* X, int64_t ldx,
 }
 
 /* ------------------------------------------------------------------------ */
-/* Ozaki split TSMTTSM, restated for blockCount = 1 (one workgroup per       */
-/* 128 x 128 tile walking all row chunks in order): the bit-exact twin of    */
-/* the device path.  Chunks are 32 rows, or 64 rows for the diagonal tiles  */
-/* of a Gram product with 3 limbs (A == B, lda == ldb, M == N,              */
-/* lda <= 2^21), where both operands share one scale per column.  Algorithm: Ozaki, Ogita, Oishi, Rump, "Error-free       */
+/* Ozaki split TSMTTSM, restated for blockCount = 1 (one workgroup per tile */
+/* walking all row chunks in order): the bit-exact twin of the device path.  */
+/* Tiles are 128 columns of A by 128 columns of B for 3 limbs and 128 by 64  */
+/* for 4 and 5 (a flush is decided over the tile's columns, so the width    */
+/* shapes the result).  Chunks are 32 rows, or 64 rows for the diagonal     */
+/* tiles of a Gram product with 3 limbs (A == B, lda == ldb, M == N,        */
+/* lda <= 2^21), where both operands share one scale per column.            */
+/* Algorithm: Ozaki, Ogita, Oishi, Rump, "Error-free                         */
 /* transformations of matrix multiplication by using fast routines of matrix */
 /* multiplication and its applications", Numer. Algorithms 59 (2012), with   */
 /* 7-bit limbs of the rounded integer q = rint(|x| 2^(7S - E)), levels up to  */
@@ -761,10 +764,11 @@ void orc_gram_error(int64_t M, int64_t K, const double* X, int64_t ldx,
 #define OZ_FLUSH 16384
 #define OZ_EMIN (-4000)
 
-/* L = S + 1 level accumulators per entry, level g = s + t at index g - 2 */
-static void oz_flush_tile(int L, int64_t* acc, double* cacc, const int* ea,
-                          const int* eb) {
-  for (int n = 0; n < OZ_TILE; n++)
+/* L = S + 1 level accumulators per entry, level g = s + t at index g - 2;
+ * TN columns of B in the tile */
+static void oz_flush_tile(int L, int TN, int64_t* acc, double* cacc,
+                          const int* ea, const int* eb) {
+  for (int n = 0; n < TN; n++)
     for (int m = 0; m < OZ_TILE; m++) {
       int64_t* a = acc + ((int64_t)n * OZ_TILE + m) * L;
       if (ea[m] != OZ_EMIN && eb[n] != OZ_EMIN) {
@@ -798,8 +802,9 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
   signed char* da = (signed char*)malloc((size_t)S * OZ_ROWS_SELF * OZ_TILE);
   signed char* db = (signed char*)malloc((size_t)S * OZ_ROWS_SELF * OZ_TILE);
   const int gram = A == B && lda == ldb && M == N && lda <= ((int64_t)1 << 21);
+  const int TN = S == 3 ? OZ_TILE : 64; /* the tile's columns of B */
   for (int64_t m0 = 0; m0 < M; m0 += OZ_TILE)
-    for (int64_t n0 = 0; n0 < N; n0 += OZ_TILE) {
+    for (int64_t n0 = 0; n0 < N; n0 += TN) {
       const int R = (gram && m0 == n0 && S == 3) ? OZ_ROWS_SELF : OZ_ROWS;
       int erun[2][OZ_TILE], ecur[2][OZ_TILE];
       for (int c = 0; c < OZ_TILE; c++) erun[0][c] = erun[1][c] = OZ_EMIN;
@@ -809,7 +814,7 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
       for (int64_t r0 = 0; r0 < K; r0 += R) {
         int flush = (since + R > OZ_FLUSH);
         for (int mat = 0; mat < 2; mat++)
-          for (int c = 0; c < OZ_TILE; c++) {
+          for (int c = 0; c < (mat == 0 ? OZ_TILE : TN); c++) {
             int64_t gc = (mat == 0 ? m0 : n0) + c;
             int valid = gc < (mat == 0 ? M : N);
             double mx = 0.0;
@@ -836,15 +841,15 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
           /* the accumulators are empty after a flush, so every column's
            * scale restarts from this chunk: an outlier only coarsens the
            * window it falls in */
-          oz_flush_tile(L, acc, cacc, erun[0], erun[1]);
+          oz_flush_tile(L, TN, acc, cacc, erun[0], erun[1]);
           since = 0;
           for (int mat = 0; mat < 2; mat++)
-            for (int c = 0; c < OZ_TILE; c++) erun[mat][c] = ecur[mat][c];
+            for (int c = 0; c < (mat == 0 ? OZ_TILE : TN); c++) erun[mat][c] = ecur[mat][c];
         }
         /* limbs */
         for (int mat = 0; mat < 2; mat++) {
           signed char* d = mat == 0 ? da : db;
-          for (int c = 0; c < OZ_TILE; c++) {
+          for (int c = 0; c < (mat == 0 ? OZ_TILE : TN); c++) {
             int64_t gc = (mat == 0 ? m0 : n0) + c;
             int valid = gc < (mat == 0 ? M : N);
             int E = erun[mat][c];
@@ -872,7 +877,7 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
             if (t1 < 1 || t1 > S) continue;
             const signed char* pa = da + (size_t)(s1 - 1) * OZ_TILE * R;
             const signed char* pb = db + (size_t)(t1 - 1) * OZ_TILE * R;
-            for (int n = 0; n < OZ_TILE; n++)
+            for (int n = 0; n < TN; n++)
               for (int m = 0; m < OZ_TILE; m++) {
                 int64_t sum = 0;
                 const signed char* x = pa + (size_t)m * R;
@@ -883,8 +888,8 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
           }
         since += R;
       }
-      oz_flush_tile(L, acc, cacc, erun[0], erun[1]);
-      for (int64_t n = 0; n < OZ_TILE && n0 + n < N; n++)
+      oz_flush_tile(L, TN, acc, cacc, erun[0], erun[1]);
+      for (int64_t n = 0; n < TN && n0 + n < N; n++)
         for (int64_t m = 0; m < OZ_TILE && m0 + m < M; m++) {
           double sres = cacc[n * OZ_TILE + m];
           double* c = &C[(n0 + n) * ldc + (m0 + m)];

@@ -27,24 +27,28 @@
 //      C_acc(m, n) += V 2^(E_m + E_n).
 //    For data of steady magnitude the conversion cost is amortised over
 //    thousands of rows, so the INT8 units and HBM set the pace.
-//  * Workgroups (4 waves, one 64 x 64 region of C each, the level
-//    accumulators in the matrix-core registers) own a 128 x 128 tile of C
-//    and a grid-stride set of chunks; their FP64 partials are summed by the
-//    same two-level last-arrival reduction as the FP64 kernel.  With
-//    blockCount == 1 the whole computation is one fixed sequence, restated
-//    bit for bit on the host by the oracle.
+//  * Workgroups (4 waves, the level accumulators in the matrix-core
+//    registers) own a tile of C and a grid-stride set of chunks; their FP64
+//    partials are summed by the same two-level last-arrival reduction as the
+//    FP64 kernel.  With 3 limbs a tile is 128 x 128 (a 64 x 64 region per
+//    wave, 4 levels x 64 registers); with 4 and 5 limbs it is 128 x 64 (a
+//    64 x 32 region per wave, so 5 or 6 levels take 160 or 192 registers
+//    instead of spilling 320 or 384), which reads A once per 64 columns of
+//    B.  With blockCount == 1 the whole computation is one fixed sequence,
+//    restated bit for bit on the host by the oracle.
 //  * Per chunk the loop is software-pipelined: the next chunks' loads are in
 //    flight while this chunk is split (VALU, packed 4 rows to a word) into
 //    one LDS buffer and the previous chunk's limbs in the other buffer are
-//    multiplied, the two interleaved instruction by instruction.  With 3
-//    limbs two chunks are in flight (prefetch distance 2), with 4 or 5 one.
-//  * Gram products (A == B, lda == ldb, M == N) compute only the tiles on
-//    and above the diagonal, writing each off-diagonal tile to C twice
-//    (C(m, n) and C(n, m)).  With 3 limbs they also read A once: the
-//    diagonal tiles run with 64-row chunks whose two 32-row halves take the
-//    places of A and B in the staging (one scale per column, shared by both
-//    operands).  The loads in flight per workgroup stay the same, so such a
-//    tile streams half the bytes of a general one in about half the time.
+//    multiplied, the two interleaved instruction by instruction.  Two chunks
+//    are in flight (prefetch distance 2).
+//  * Gram products with 3 limbs (A == B, lda == ldb, M == N) compute only
+//    the tiles on and above the diagonal, writing each off-diagonal tile to
+//    C twice (C(m, n) and C(n, m)), and read A once: the diagonal tiles run
+//    with 64-row chunks whose two 32-row halves take the places of A and B
+//    in the staging (one scale per column, shared by both operands).  The
+//    loads in flight per workgroup stay the same, so such a tile streams
+//    half the bytes of a general one in about half the time.  With 4 and 5
+//    limbs a Gram product runs as a general one.
 #include <cmath>
 #include <utility>
 
@@ -113,7 +117,7 @@ struct OzArgs {
   double alpha, beta;
   double* part1;
   double* part2;
-  int* stage;  // per wave: (S + 1) * 4096 INT32 accumulator staging
+  int* stage;  // per wave: (S + 1) * 1024 * TT INT32 accumulator staging
   unsigned* cnt;
   int G, GS, ngroups, tilesM, tilesN;
   int ntiles;  // tiles of this launch
@@ -146,57 +150,60 @@ __device__ __forceinline__ void oz_tile_of(const OzArgs& a, int tile, int& tm,
 // registers: placed at the top of the chunk loop it keeps the register
 // allocator from carrying them around the loop as VGPR copies (which the
 // flush path otherwise provokes, spilling them every iteration).
-template <int L>
-__device__ __forceinline__ void oz_pin(v16i (&acc)[L][4]) {
+template <int L, int TT>
+__device__ __forceinline__ void oz_pin(v16i (&acc)[L][TT]) {
 #pragma unroll
   for (int g = 0; g < L; g++)
 #pragma unroll
-    for (int tt = 0; tt < 4; tt++) asm volatile("" : "+a"(acc[g][tt]));
+    for (int tt = 0; tt < TT; tt++) asm volatile("" : "+a"(acc[g][tt]));
 }
 
 // Convert the INT32 level accumulators to FP64 and add them to this wave's
-// 64 x 64 region of the workgroup's partial tile in global memory (entry
-// (m, n) at n * 128 + m, owned by exactly one lane), then clear them.  The
+// 64 x (32 NSUB) region of the workgroup's partial tile in global memory
+// (entry (m, n) at n * 128 + m, owned by exactly one lane), then clear them.
+// Accumulator tt holds the 32 x 32 block (tt / NSUB, tt % NSUB) of the
+// region.  The
 // accumulators go straight from the matrix-core registers to the wave's INT32
 // staging area and are converted from there one entry at a time, so the
 // conversion holds no copy of them.  `part` is the resource of the region's
 // first entry, `lofs` the lane's byte offset in it.
-template <int L>
-__device__ __forceinline__ void oz_flush(v16i (&acc)[L][4],
+template <int L, int NSUB>
+__device__ __forceinline__ void oz_flush(v16i (&acc)[L][2 * NSUB],
                                          __amdgpu_buffer_rsrc_t stage,
                                          __amdgpu_buffer_rsrc_t part, int lofs,
                                          const int* ea_w, const int* eb_w,
                                          int lane) {
+  constexpr int TT = 2 * NSUB;
 #pragma unroll
   for (int g = 0; g < L; g++)
 #pragma unroll
-    for (int tt = 0; tt < 4; tt++) {
+    for (int tt = 0; tt < TT; tt++) {
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const v4i w = {acc[g][tt][4 * q], acc[g][tt][4 * q + 1],
                        acc[g][tt][4 * q + 2], acc[g][tt][4 * q + 3]};
         __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4u, w), stage,
-                                               16 * lane, 1024 * ((g * 4 + tt) * 4 + q), 0);
+                                               16 * lane, 1024 * ((g * TT + tt) * 4 + q), 0);
       }
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[g][tt][r] = 0;
     }
 #pragma unroll 1
-  for (int e = 0; e < 64; e++) {
+  for (int e = 0; e < 16 * TT; e++) {
     const int tt = e >> 4, r = e & 15;
-    const int dm = 32 * (tt >> 1) + i8_drow(lane, r);
-    const int dn = 32 * (tt & 1) + (lane & 31);
+    const int dm = 32 * (tt / NSUB) + i8_drow(lane, r);
+    const int dn = 32 * (tt % NSUB) + (lane & 31);
     const int ea = ea_w[dm], eb = eb_w[dn];
     if (ea != kOzEmin && eb != kOzEmin) {
       const int lo = 16 * lane + 4 * (r & 3);
       auto level = [&](int g) {
         return (int)__builtin_amdgcn_raw_buffer_load_b32(
-            stage, lo, 1024 * ((g * 4 + tt) * 4 + (r >> 2)), 0);
+            stage, lo, 1024 * ((g * TT + tt) * 4 + (r >> 2)), 0);
       };
       double v = ldexp((double)level(L - 1), -7 * (L + 1));
 #pragma unroll
       for (int g = L; g >= 2; g--) v = v + ldexp((double)level(g - 2), -7 * g);
-      const int so = 8 * (32 * (tt & 1) * kOzTile + 32 * (tt >> 1) +
+      const int so = 8 * (32 * (tt % NSUB) * kOzTile + 32 * (tt / NSUB) +
                           8 * (r >> 2) + (r & 3));
       oz_st(oz_ld(part, lofs, so) + ldexp(v, ea + eb), part, lofs, so);
     }
@@ -260,8 +267,9 @@ __device__ __forceinline__ void oz_static_for(F&& f, std::integer_sequence<int, 
   (f(std::integral_constant<int, I>{}), ...);
 }
 
-// The k-th INT8 product of a chunk (tile tt, level g, limb s of A, limb
-// g - s of B), tile-major over the pairs with 2 <= s + t <= S + 2.
+// The k-th INT8 product of a chunk (accumulator block tt of TT, level g,
+// limb s of A, limb g - s of B), block-major over the pairs with
+// 2 <= s + t <= S + 2.
 struct OzMf {
   int tt, g, s;
 };
@@ -269,18 +277,18 @@ template <int S>
 constexpr int oz_levels() {
   return S + 1;
 }
-template <int S>
+template <int S, int TT>
 constexpr int oz_nmfma() {
   int n = 0;
   for (int g = 2; g <= S + 2; g++)
     for (int s = 1; s <= S; s++)
       if (g - s >= 1 && g - s <= S) n++;
-  return 4 * n;
+  return TT * n;
 }
-template <int S>
+template <int S, int TT>
 constexpr OzMf oz_mfma_of(int k) {
   int n = 0;
-  for (int tt = 0; tt < 4; tt++)
+  for (int tt = 0; tt < TT; tt++)
     for (int g = 2; g <= S + 2; g++)
       for (int s = 1; s <= S; s++) {
         const int t = g - s;
@@ -293,9 +301,15 @@ constexpr OzMf oz_mfma_of(int k) {
 
 // SELF: a diagonal tile of a Gram product (64-row chunks, the second index of
 // the limb staging is the chunk's 32-row half instead of A/B).
-template <int S, bool SELF>
+// TN: the tile's width in columns of B (128, or 64 for 4 and 5 limbs, whose
+// S + 1 accumulator levels for a 64 x 64 region per wave would not fit the
+// 256 matrix-core registers: a 64 x 32 region per wave halves them).
+template <int S, bool SELF, int TN>
 __global__ void __launch_bounds__(kOzThreads)
     __attribute__((amdgpu_waves_per_eu(1, 1))) ozaki_kernel(OzArgs a) {
+  static_assert(TN == 128 ? S == 3 : (TN == 64 && !SELF), "tile width");
+  constexpr int NSUB = TN / 64;  // 32-column blocks per wave
+  constexpr int TT = 2 * NSUB;   // 32 x 32 accumulator blocks per wave
   // limbs of two chunks (double buffer): [buffer][A/B or half][limb][column][row]
   __shared__ __attribute__((aligned(16))) signed char sl[2][2][S][kOzTile][kOzRows];
   __shared__ int erun[2][kOzTile];
@@ -305,17 +319,19 @@ __global__ void __launch_bounds__(kOzThreads)
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wm = wave >> 1, wn = wave & 1;  // this wave's 64 x 64 region
+  // this wave's 64 x (TN / 2) region: rows 64 wm.., columns (TN / 2) wn..
+  const int wm = wave >> 1, wn = wave & 1;
   const int ntiles = a.ntiles;
   const int tile = blockIdx.x % ntiles;
   const int kp = blockIdx.x / ntiles;
   int tm, tn;
   oz_tile_of(a, tile, tm, tn);
   const int m0 = tm * kOzTile;
-  const int n0 = tn * kOzTile;
+  const int n0 = tn * TN;
 
   // General tiles: waves 0-1 stream the 128 columns of A's tile, waves 2-3
-  // those of B; thread `col` holds its column's 32 rows of the current chunk.
+  // the TN columns of B (TN = 64: wave 3 loads nothing); thread `col` holds
+  // its column's 32 rows of the current chunk.
   // SELF: wave w streams columns 32 w .. 32 w + 31, lanes 0-31 the chunk's
   // first 32 rows and lanes 32-63 its last 32 (mat = the half), so the two
   // halves of a column meet in one wave.
@@ -325,7 +341,10 @@ __global__ void __launch_bounds__(kOzThreads)
   const bool isA = SELF || mat == 0;
   const int width = isA ? a.M : a.N;
   const int gcol = (isA ? m0 : n0) + col;
-  const bool cvalid = gcol < width;
+  // a column of the tile (for TN = 64, waves 0-2; stated through the wave
+  // index so that the compiler sees a wave-uniform load resource)
+  const bool cload = TN == 128 || isA || (wave & 1) == 0;
+  const bool cvalid = cload && gcol < width;
   const double* mbase = isA ? a.A : a.B;
   const int64_t ld = isA ? a.lda : a.ldb;
   const int ld8 = (int)(8 * ld);
@@ -333,48 +352,50 @@ __global__ void __launch_bounds__(kOzThreads)
 
   erun[mat][col] = kOzEmin;
   constexpr int L = oz_levels<S>();  // accumulator levels g = 2 .. S + 2
-  v16i acc[L][4];
+  v16i acc[L][TT];
 #pragma unroll
   for (int g = 0; g < L; g++)
 #pragma unroll
-    for (int tt = 0; tt < 4; tt++)
+    for (int tt = 0; tt < TT; tt++)
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[g][tt][r] = 0;
   // this workgroup's FP64 partial tile, accumulated by the (rare) flushes
-  constexpr int PS = kOzTile * kOzTile;
+  constexpr int PS = kOzTile * TN;
+  constexpr int WN = TN / 2;  // the wave region's width
   double* mine = a.part1 + ((int64_t)tile * a.G + kp) * PS;
   const __amdgpu_buffer_rsrc_t part =
-      oz_rsrc(mine + 64 * wn * kOzTile + 64 * wm,
-              8 * (int64_t)(PS - 64 * wn * kOzTile - 64 * wm));
+      oz_rsrc(mine + WN * wn * kOzTile + 64 * wm,
+              8 * (int64_t)(PS - WN * wn * kOzTile - 64 * wm));
   const int lofs = 8 * ((lane & 31) * kOzTile + 4 * (lane >> 5));
 #pragma unroll
-  for (int tt = 0; tt < 4; tt++)
+  for (int tt = 0; tt < TT; tt++)
 #pragma unroll
     for (int r = 0; r < 16; r++)
       oz_st(0.0, part, lofs,
-            8 * (32 * (tt & 1) * kOzTile + 32 * (tt >> 1) + 8 * (r >> 2) + (r & 3)));
+            8 * (32 * (tt % NSUB) * kOzTile + 32 * (tt / NSUB) + 8 * (r >> 2) + (r & 3)));
   const __amdgpu_buffer_rsrc_t stage = oz_rsrc(
-      a.stage + ((int64_t)blockIdx.x * (kOzThreads / 64) + wave) * (L * 4096),
-      (int64_t)L * 4096 * 4);
+      a.stage + ((int64_t)blockIdx.x * (kOzThreads / 64) + wave) * (L * TT * 1024),
+      (int64_t)L * TT * 1024 * 4);
   const int* ea_w = &erun[0][64 * wm];
-  const int* eb_w = &erun[1][64 * wn];
+  const int* eb_w = &erun[1][WN * wn];
   __syncthreads();
 
   const int64_t nch = (a.K + CR - 1) / CR;
   // D chunks of this thread's rows in flight while one is split: a chunk's
   // registers are refilled row group by row group, as it is split, with the
-  // chunk D steps ahead.  3 limbs keep two chunks (prefetch distance 2);
-  // 4 and 5 limbs need the registers for their accumulators and keep one.
-  constexpr int D = S == 3 ? 2 : 1;
+  // chunk D = 2 steps ahead (prefetch distance 2)
+  constexpr int D = 2;
   double xa[kOzRows], xb[kOzRows];
   // this thread's 32 rows of chunk c in its column: the chunk's resource
   // ends after its last row below K, so rows at or past K read 0 and every
   // chunk issues the same 32 loads back to back.  Padding columns read
-  // column 0 and are zeroed through their scale.
+  // column 0 and are zeroed through their scale; threads without a column
+  // (TN = 64) get an empty resource, so their loads return 0 and touch no
+  // memory.
   auto chunk_rsrc = [&](int64_t c) {
     const int64_t r0 = c * CR;
     const int rows = (int)min<int64_t>(CR, a.K - r0);
-    return oz_rsrc(mbase + r0 * ld, 8 * ((int64_t)(rows - 1) * ld + width));
+    return oz_rsrc(mbase + r0 * ld, cload ? 8 * ((int64_t)(rows - 1) * ld + width) : 0);
   };
   auto load_chunk = [&](int64_t c, double (&dst)[kOzRows]) {
     const __amdgpu_buffer_rsrc_t rs = chunk_rsrc(c);
@@ -428,9 +449,9 @@ __global__ void __launch_bounds__(kOzThreads)
 #pragma unroll
     for (int h = 0; h < (SELF ? 2 : 1); h++)  // SELF: both halves of the chunk
 #pragma unroll
-      for (int tt = 0; tt < 4; tt++) {
-        const int mcol = 64 * wm + 32 * (tt >> 1) + (lane & 31);
-        const int ncol = 64 * wn + 32 * (tt & 1) + (lane & 31);
+      for (int tt = 0; tt < TT; tt++) {
+        const int mcol = 64 * wm + 32 * (tt / NSUB) + (lane & 31);
+        const int ncol = WN * wn + 32 * (tt % NSUB) + (lane & 31);
 #pragma unroll
         for (int g = 2; g <= S + 2; g++) {
 #pragma unroll
@@ -456,15 +477,17 @@ __global__ void __launch_bounds__(kOzThreads)
   // second half's operands are read at the end of unit 3 and multiplied in
   // units 4-7.
   auto split_multiply = [&](int bm, int bs, double (&x)[kOzRows], int64_t cnext) {
-    v4i av[2][S], bv[2][S];
+    v4i av[2][S], bv[NSUB][S];
     auto operands = [&](int h2) {
 #pragma unroll
-      for (int h = 0; h < 2; h++)
+      for (int s2 = 0; s2 < S; s2++) {
 #pragma unroll
-        for (int s2 = 0; s2 < S; s2++) {
+        for (int h = 0; h < 2; h++)
           av[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][SELF ? h2 : 0][s2][64 * wm + 32 * h + (lane & 31)][kh]);
-          bv[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][SELF ? h2 : 1][s2][64 * wn + 32 * h + (lane & 31)][kh]);
-        }
+#pragma unroll
+        for (int h = 0; h < NSUB; h++)
+          bv[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][SELF ? h2 : 1][s2][WN * wn + 32 * h + (lane & 31)][kh]);
+      }
     };
     operands(0);
     const __amdgpu_buffer_rsrc_t rn = chunk_rsrc(cnext);
@@ -487,14 +510,14 @@ __global__ void __launch_bounds__(kOzThreads)
           // these 4 rows are split: their registers take chunk cnext
 #pragma unroll
           for (int j = 0; j < 4; j++) x[4 * U + j] = oz_ld_stream(rn, voff, (4 * U + j) * ld8);
-          constexpr int NM = oz_nmfma<S>();
+          constexpr int NM = oz_nmfma<S, TT>();
           constexpr int NT = SELF ? 2 * NM : NM;  // products per chunk
           constexpr int k0 = U * NT / 8, k1 = (U + 1) * NT / 8;
           oz_static_for(
               [&](auto K_) {
-                constexpr OzMf m = oz_mfma_of<S>((k0 + decltype(K_)::value) % NM);
+                constexpr OzMf m = oz_mfma_of<S, TT>((k0 + decltype(K_)::value) % NM);
                 acc[m.g - 2][m.tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                    av[m.tt >> 1][m.s - 1], bv[m.tt & 1][m.g - m.s - 1],
+                    av[m.tt / NSUB][m.s - 1], bv[m.tt % NSUB][m.g - m.s - 1],
                     acc[m.g - 2][m.tt], 0, 0, 0);
               },
               std::make_integer_sequence<int, k1 - k0>{});
@@ -532,7 +555,7 @@ __global__ void __launch_bounds__(kOzThreads)
     load_chunk(ch, xa);
     erun[mat][col] = chunk_exp(xa);
     __syncthreads();
-    if (D == 2) load_chunk(min(ch + a.G, nch - 1), xb);
+    load_chunk(min(ch + a.G, nch - 1), xb);
     limbs(pb, xa);
     load_chunk(min(ch + D * a.G, nch - 1), xa);
     int since = CR;
@@ -543,7 +566,7 @@ __global__ void __launch_bounds__(kOzThreads)
     // multiplied, chunk ch split again after the flush)
     auto step = [&](double (&x)[kOzRows]) -> int {
       if (ch >= nch) return 1;
-      oz_pin<L>(acc);
+      oz_pin<L, TT>(acc);
       if (since + CR > kOzFlushRows) return 1;  // uniform
       const bool rescale = split_multiply(pb, pb ^ 1, x, min(ch + D * a.G, nch - 1));
       if (__syncthreads_or(rescale)) return 2;
@@ -552,16 +575,10 @@ __global__ void __launch_bounds__(kOzThreads)
       ch += a.G;
       return 0;
     };
-    // D == 2: xb holds chunk ch and xa chunk ch + G, the two alternate;
-    // D == 1: xa holds chunk ch
+    // xb holds chunk ch and xa chunk ch + G, the two alternate
     for (;;) {
-      int r;
-      if constexpr (D == 2) {
-        r = step(xb);
-        if (r == 0) r = step(xa);
-      } else {
-        r = step(xa);
-      }
+      int r = step(xb);
+      if (r == 0) r = step(xa);
       if (r != 0) {
         staged = r == 1;
         break;
@@ -569,7 +586,7 @@ __global__ void __launch_bounds__(kOzThreads)
     }
     // drain: multiply the staged chunk, convert the accumulators
     if (staged) multiply(pb);
-    oz_flush<L>(acc, stage, part, lofs, ea_w, eb_w, lane);
+    oz_flush<L, NSUB>(acc, stage, part, lofs, ea_w, eb_w, lane);
     __syncthreads();
   }
 
@@ -587,7 +604,7 @@ __global__ void __launch_bounds__(kOzThreads)
   __syncthreads();
   if (!s_last) return;
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  const int Mloc = min(a.M - m0, kOzTile), Nloc = min(a.N - n0, kOzTile);
+  const int Mloc = min(a.M - m0, kOzTile), Nloc = min(a.N - n0, TN);
   const bool mirror = a.mode == OZ_UPPER;
   auto apply = [&](int e, double s) {
     const int ml = e % kOzTile, nl = e / kOzTile;
@@ -682,12 +699,13 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
     return fail(SKINNY_UNSUPPORTED, "ozaki: lda, ldb <= 2^22");
   h = resolve(h);
   if (!h) return no_handle("ozaki");
-  // Gram products skip the tiles below the diagonal; with 3 limbs they also
-  // read A once (64-row chunks on the diagonal tiles, whose offsets need
-  // lda <= 2^21)
-  const bool gram = A == B && lda == ldb && M == N && lda <= (1 << 21);
+  // 3 limbs: 128 x 128 tiles.  Gram products skip the tiles below the
+  // diagonal and read A once (64-row chunks on the diagonal tiles, whose
+  // offsets need lda <= 2^21).  4 and 5 limbs: 128 x 64 tiles, all of them.
+  const int TN = slices == 3 ? kOzTile : 64;
+  const bool gram = slices == 3 && A == B && lda == ldb && M == N && lda <= (1 << 21);
   const int tilesM = (int)((M + kOzTile - 1) / kOzTile);
-  const int tilesN = (int)((N + kOzTile - 1) / kOzTile);
+  const int tilesN = (int)((N + TN - 1) / TN);
   struct Set {
     int mode, ntiles;
     bool self;
@@ -696,14 +714,13 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
   Set sets[2];
   int nsets = 0;
   if (gram) {
-    // 64-row chunks for 3 limbs; 4 and 5 limbs keep 32-row chunks (their
-    // accumulators leave no registers for the second half's operands)
-    sets[nsets++] = Set{OZ_DIAG, tilesM, slices == 3, {}};
+    sets[nsets++] = Set{OZ_DIAG, tilesM, true, {}};
     if (tilesM > 1) sets[nsets++] = Set{OZ_UPPER, tilesM * (tilesM - 1) / 2, false, {}};
   } else {
     sets[nsets++] = Set{OZ_ALL, tilesM * tilesN, false, {}};
   }
-  const size_t PS = (size_t)kOzTile * kOzTile;
+  const size_t PS = (size_t)kOzTile * TN;
+  const int TT = 2 * (TN / 64);  // accumulator blocks per wave
   size_t bytes = 0;  // the launches run one after the other: the larger one
   for (int i = 0; i < nsets; i++) {
     OzArgs& a = sets[i].a;
@@ -727,7 +744,7 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
       return fail(SKINNY_UNSUPPORTED, "ozaki: too many partial groups");
     size_t b = (size_t)ntiles * a.G * PS * sizeof(double);
     if (a.ngroups > 1) b += (size_t)ntiles * a.ngroups * PS * sizeof(double);
-    b += (size_t)ntiles * a.G * (kOzThreads / 64) * (slices + 1) * 4096 * sizeof(int);
+    b += (size_t)ntiles * a.G * (kOzThreads / 64) * (slices + 1) * 1024 * TT * sizeof(int);
     bytes = std::max(bytes, b);
   }
   Workspace w;
@@ -760,16 +777,12 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
     a.cnt = ws_counters(w);
     const int blocks = a.G * ntiles;
     if (sets[i].self) {
-      switch (slices) {
-        case 3: hipLaunchKernelGGL((ozaki_kernel<3, true>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
-        case 4: hipLaunchKernelGGL((ozaki_kernel<4, true>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
-        default: hipLaunchKernelGGL((ozaki_kernel<5, true>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
-      }
+      hipLaunchKernelGGL((ozaki_kernel<3, true, 128>), dim3(blocks), dim3(kOzThreads), 0, s, a);
     } else {
       switch (slices) {
-        case 3: hipLaunchKernelGGL((ozaki_kernel<3, false>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
-        case 4: hipLaunchKernelGGL((ozaki_kernel<4, false>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
-        default: hipLaunchKernelGGL((ozaki_kernel<5, false>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((ozaki_kernel<3, false, 128>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((ozaki_kernel<4, false, 64>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+        default: hipLaunchKernelGGL((ozaki_kernel<5, false, 64>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
       }
     }
     hipError_t e = hipGetLastError();
