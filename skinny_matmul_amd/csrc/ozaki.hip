@@ -310,8 +310,10 @@ __global__ void __launch_bounds__(kOzThreads)
   static_assert(TN == 128 ? S == 3 : (TN == 64 && !SELF), "tile width");
   constexpr int NSUB = TN / 64;  // 32-column blocks per wave
   constexpr int TT = 2 * NSUB;   // 32 x 32 accumulator blocks per wave
-  // limbs of two chunks (double buffer): [buffer][A/B or half][limb][column][row]
-  __shared__ __attribute__((aligned(16))) signed char sl[2][2][S][kOzTile][kOzRows];
+  // limbs of two chunks (double buffer): [buffer][A/B or half][limb][column][row];
+  // a column's 32 bytes sit in a 48-byte row, so the 16-byte accesses of 16
+  // consecutive lanes (32-byte strides otherwise) fall on distinct banks
+  __shared__ __attribute__((aligned(16))) signed char sl[2][2][S][kOzTile][kOzRows + 16];
   __shared__ int erun[2][kOzTile];
   __shared__ int s_last;
   __shared__ double tree[2 * kOzThreads];
