@@ -38,6 +38,7 @@ SOURCES = [
     "htsmttsm.hip",
     "tsmm.hip",
     "tsmm_lds.hip",
+    "tsmm_q4.hip",
     "tsmm_wide.hip",
     "capi.hip",
     "microbench.hip",

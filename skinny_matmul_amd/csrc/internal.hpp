@@ -180,6 +180,15 @@ skinny_status_t launch_tsmm_lds(skinny_handle_t h, size_t blockCount,
                                 int64_t ldb, T beta, T beta_i, T* C,
                                 int64_t ldc, bool cplx, hipStream_t stream);
 
+// FP64 on the 4x4x4 matrix-core instruction with B staged in LDS (real-view
+// M in 29..128, N <= 128; same arguments as launch_tsmm_lds).  Returns
+// SKINNY_UNSUPPORTED, having launched nothing, outside that range.
+skinny_status_t launch_tsmm_q4(skinny_handle_t h, size_t blockCount, int64_t M, int64_t N,
+                               int64_t K, const double* A, int64_t lda, double alpha,
+                               double alpha_i, const double* B, int64_t ldb, double beta,
+                               double beta_i, double* C, int64_t ldc, bool cplx,
+                               hipStream_t stream);
+
 // Wide blocks (real-view M or N above 128, up to kTsmmWideMax): the row band
 // of A in LDS, B packed into the stream's workspace; same arguments as
 // launch_tsmm_lds.

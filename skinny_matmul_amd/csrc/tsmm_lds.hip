@@ -291,6 +291,12 @@ skinny_status_t launch_tsmm_lds(skinny_handle_t h, size_t blockCount,
   if (M > 128 || N > 128)
     return launch_tsmm_wide<T>(h, blockCount, M, N, K, A, lda, alpha, alpha_i,
                                B, ldb, beta, beta_i, C, ldc, cplx, stream);
+  if constexpr (sizeof(T) == 8) {
+    // FP64 with A at least 29 real columns wide: the 4x4x4 matrix-core form
+    const skinny_status_t q = launch_tsmm_q4(h, blockCount, M, N, K, A, lda, alpha, alpha_i,
+                                             B, ldb, beta, beta_i, C, ldc, cplx, stream);
+    if (q != SKINNY_UNSUPPORTED) return q;
+  }
   TlArgs<T> a;
   a.A = A;
   a.B = B;
