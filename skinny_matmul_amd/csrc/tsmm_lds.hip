@@ -5,13 +5,17 @@
 // i), but B no longer fits in registers, so each workgroup copies it into LDS
 // once (up to 160 KB per workgroup on gfx950) and every MFMA reads its B
 // operand from there.  N is a runtime loop over 16-column subtiles taken two
-// at a time (two independent accumulator chains).
+// at a time (two independent accumulator chains).  LDS holds the operand
+// zero-padded to 4 S rows and an even number of 16-column subtiles, so an
+// operand read is one LDS access with no range checks.
 //
 // Complex TSMM runs on the interleaved real view: with A_r the K x 2M real
 // matrix, C_r = A_r * B^ where B^ (2M x 2N) holds, for each complex B(m, n),
 //     [ Re  Im ]
 //     [-Im  Re ]   at rows 2m, 2m + 1 and columns 2n, 2n + 1,
-// built on the fly from LDS planes Re(B) and Im(B).  Lanes 2j and 2j + 1 hold
+// written out in LDS when the workgroup stages B (it was formed per operand
+// read from planes Re(B) and Im(B): FC M = N = 64 ran at 1.7 TB/s, bound by
+// that VALU work).  Lanes 2j and 2j + 1 hold
 // the real and imaginary parts of one C entry; the complex alpha and beta
 // are applied after exchanging them with a lane swap (tsmm/fix3.cuh:31-35 and
 // cu_complex.h axpby: beta * C + alpha * sum).
@@ -29,7 +33,8 @@ struct TlArgs {
   int M, N;                  // real-view widths
   T alpha, beta, alpha_i, beta_i;
   int vec;  // 16-byte A slice loads allowed
-  int ns;   // LDS row stride (elements)
+  int ns;   // LDS row stride (elements), at least np
+  int np;   // padded columns staged (16 x an even subtile count)
 };
 
 template <int S>
@@ -53,33 +58,28 @@ __global__ void __launch_bounds__(256) tsmm_lds_kernel(TlArgs<T> a) {
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int li = lane & 15, lg = lane >> 4;
-  const int M = a.M, N = a.N, NS = a.ns;
-  const int Mc = M / 2, Nc = N / 2;
+  const int M = a.M, N = a.N, NS = a.ns, NP = a.np;
 
-  // ---- stage B in LDS: [m][n] (real) or planes Re[m][n], Im[m][n] -------
-  if (CPLX) {
-    for (int idx = tid; idx < Mc * Nc; idx += 256) {
-      const int m = idx % Mc, n = idx / Mc;
-      const T* b = a.B + 2 * ((int64_t)n * a.ldb + m);
-      bs[m * NS + n] = b[0];
-      bs[(Mc + m) * NS + n] = b[1];
+  // ---- stage B (or B^) in LDS, [m][n], zero outside M x N ----------------
+  for (int idx = tid; idx < 4 * S * NP; idx += 256) {
+    const int m = idx / NP, n = idx - m * NP;
+    T v = T(0);
+    if (m < M && n < N) {
+      if (CPLX) {
+        const int pa = m & 1, pb = n & 1;
+        const T* b = a.B + 2 * ((int64_t)(n >> 1) * a.ldb + (m >> 1));
+        v = b[pa ^ pb];  // Re on the diagonal of the 2 x 2 block, Im off it
+        if (pa && !pb) v = -v;
+      } else {
+        v = a.B[(int64_t)n * a.ldb + m];
+      }
     }
-  } else {
-    for (int idx = tid; idx < M * N; idx += 256) {
-      const int m = idx % M, n = idx / M;
-      bs[m * NS + n] = a.B[(int64_t)n * a.ldb + m];
-    }
+    bs[m * NS + n] = v;
   }
   __syncthreads();
 
-  auto bval = [&](int s, int u) -> T {
-    const int mr = lg * S + s, nr = 16 * u + li;
-    if (mr >= M || nr >= N) return T(0);
-    if (!CPLX) return bs[mr * NS + nr];
-    const int pa = mr & 1, pb = nr & 1;
-    const T v = bs[((pa ^ pb) * Mc + (mr >> 1)) * NS + (nr >> 1)];
-    return (pa && !pb) ? -v : v;
-  };
+  const T* bl0 = bs + (lg * S) * NS + li;
+  auto bval = [&](int s, int u) -> T { return bl0[s * NS + 16 * u]; };
 
   const int64_t K = a.K;
   const int64_t gw = (int64_t)blockIdx.x * 4 + (tid >> 6);
@@ -316,15 +316,24 @@ skinny_status_t launch_tsmm_lds(skinny_handle_t h, size_t blockCount,
               : S0 <= 16 ? 16 : 32;
   a.vec = (sizeof(T) == 8 && S % 2 == 0 && M % 2 == 0 && lda % 2 == 0 &&
            (uintptr_t)A % 16 == 0) ? 1 : 0;
-  size_t lds;
-  if (cplx) {
-    a.ns = (int)(N / 2) + 1;
-    lds = (size_t)2 * (M / 2) * a.ns * sizeof(T);
-  } else {
-    a.ns = (int)N + 1;
-    lds = (size_t)M * a.ns * sizeof(T);
+  // operand rows 4 S, columns an even number of 16-column subtiles; the row
+  // stride puts the four k-lane groups of an operand read (S rows apart) on
+  // different banks where a small pad can (float: 16 banks apart)
+  const int nt = (int)((N + 15) / 16);
+  a.np = 16 * (nt + (nt & 1));
+  a.ns = a.np;
+  for (int pad = 0; pad <= 16; pad++) {
+    const int64_t words = (int64_t)S * (a.np + pad) * (int64_t)sizeof(T) / 4;
+    if (sizeof(T) == 4 ? (words % 64 == 16 || words % 64 == 48) : (words % 64 == 32)) {
+      a.ns = a.np + pad;
+      break;
+    }
   }
-  if (lds < 16) lds = 16;
+  size_t lds = (size_t)4 * S * a.ns * sizeof(T);
+  if (lds > 160 * 1024) {
+    a.ns = a.np;
+    lds = (size_t)4 * S * a.ns * sizeof(T);
+  }
   int blocks = blockCount > 0
                    ? (int)blockCount
                    : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T), !cplx, true, false,
