@@ -213,8 +213,11 @@ def extra_configs(sk, torch, logK: int) -> dict:
         nbytes += (Mb + Nb) * Kb * 8 + op_bytes("tsmm", Mb, Nb, Kb)
     batch = sk.Batch(ops)
     ms_auto = _time_ms(torch, batch.run, 5)
-    # the batch's own tuner picks the item size and grid of the plan
+    default_cls = batch.classes()
+    # the batch's own tuner picks the tile class of each op shape, then the
+    # item size and grid of the plan
     _, tuned_ib, tuned_grid = batch.tune(reps=2)
+    tuned_cls = batch.classes()
     ms = _time_ms(torch, batch.run, 5)
     # per-instance error against an FP64 library product (rocBLAS through
     # torch): the whole M x N result of each TSMTTSM, 64 sampled rows of each
@@ -237,6 +240,8 @@ def extra_configs(sk, torch, logK: int) -> dict:
                  "GBs": round(nbytes / (ms * 1e-3) / 1e9, 1), "bytes": nbytes,
                  "untuned_plan_ms": round(ms_auto, 3), "tuned_item_bytes": tuned_ib,
                  "tuned_grid": tuned_grid,
+                 "ops_tuned_to_another_tile_class": sum(
+                     1 for c0, c1 in zip(default_cls, tuned_cls) if c0 != c1),
                  "max_tsmttsm_err_over_K_eps": round(worst_ts, 3),
                  "max_tsmm_err_over_M_eps": round(worst_tm, 3),
                  "instances_over_8_eps_bound": int(bad)}

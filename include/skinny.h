@@ -335,16 +335,37 @@ skinny_status_t skinny_dbatch_create(skinny_handle_t h, int nops,
                                      int64_t item_bytes, skinny_batch_t* batch);
 /* One launch; no allocation, memset or host synchronisation (graph-safe). */
 skinny_status_t skinny_dbatch_run(skinny_batch_t batch, skinny_stream_t stream);
-/* Autotune the batch's plan: time it (running it `reps` + 1 times each, on
- * the null stream; the ops' outputs are overwritten) with items of the
- * automatic size and of 1, 2, 4, 8 and 16 MiB, on grids of 1 and 2
- * workgroups per CU, and keep the fastest (replaces the per-width ladder of
- * multi_dispatch.cuh:3-90 for the choice of work split).  best_ms,
- * item_bytes (0: automatic) and grid may be NULL.  Synchronous; a captured
- * graph of an earlier plan must be captured again. */
+/* Autotune the batch (replaces the per-width ladder of multi_dispatch.cuh:3-90,
+ * which fixed one kernel instance per (M, N) at compile time):
+ *  1. tile class per op shape: the ops of one (op kind, M, N, Gram or not)
+ *     whose class was not set with skinny_dbatch_set_class are run alone
+ *     with each candidate tile class (for TSMTTSM with max(M, N) <= 8 the
+ *     two densest row packings and the unpacked 16 x 16 tile, for Gram
+ *     products the Gram and the general tile, for TSMM with M, N <= 2 the
+ *     row-per-thread and the matrix-core form) and take the fastest;
+ *  2. work split: the plan is timed with items of the automatic size and of
+ *     1, 2, 4, 8 and 16 MiB, on grids of 1 and 2 workgroups per CU, and the
+ *     fastest kept.
+ * Every timing runs `reps` + 1 times on the null stream (the ops' outputs are
+ * overwritten, in-place TSMM ops updated each time).  best_ms, item_bytes
+ * (0: automatic) and grid may be NULL.  Synchronous; a captured graph of an
+ * earlier plan must be captured again. */
 skinny_status_t skinny_dbatch_tune(skinny_batch_t batch, int reps,
                                    double* best_ms, int64_t* item_bytes,
                                    int* grid);
+/* Tile class of op `op` (the kernel form the persistent kernel runs it with;
+ * the per-width template instances of multi_dispatch.cuh:3-90 made this
+ * choice at compile time).  TSMTTSM: 0..5 rows packed P = 16, 8, 5, 4, 3, 2
+ * to the matrix-core lanes (P max(M, N) <= 16); 6..14 subtiles
+ * (MT, NT) in {1, 2, 4}^2 of 16 x 16 (16 MT >= M, 16 NT >= N); 15..17 Gram
+ * products (B == A, ldb == lda, M == N) on 16, 32, 64 square tiles.  TSMM:
+ * 0..14 slices S in {1, 2, 4, 8, 16} x NT in {1, 2, 4} column tiles
+ * (class 3 iS + iN; 4 S >= M, 16 NT >= N); 15, 16 one thread per row for
+ * M <= 1, 2 and N <= 2.  cls = -1 returns to the default rule.  Re-plans the
+ * batch; SKINNY_UNSUPPORTED if the class cannot compute the op. */
+skinny_status_t skinny_dbatch_set_class(skinny_batch_t batch, int op, int cls);
+/* The tile class of every op in the current plan (nops ints). */
+skinny_status_t skinny_dbatch_classes(skinny_batch_t batch, int* cls);
 skinny_status_t skinny_dbatch_destroy(skinny_batch_t batch);
 int64_t skinny_dbatch_items(skinny_batch_t batch);
 

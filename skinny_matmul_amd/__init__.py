@@ -121,6 +121,8 @@ _SIGS = {
     "skinny_dbatch_run": (C.c_int, [vp, vp]),
     "skinny_dbatch_tune": (C.c_int, [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int64),
                                      C.POINTER(C.c_int)]),
+    "skinny_dbatch_set_class": (C.c_int, [vp, C.c_int, C.c_int]),
+    "skinny_dbatch_classes": (C.c_int, [vp, vp]),
     "skinny_dbatch_destroy": (C.c_int, [vp]),
     "skinny_dbatch_items": (i64, [vp]),
     "skinny_dsvqb": (C.c_int, [vp, C.c_int, vp, C.c_int, vp]),
@@ -496,6 +498,7 @@ class Batch:
         _check(lib().skinny_dbatch_create(_h(handle), len(ops), C.cast(arr, vp), item_bytes,
                                           C.byref(b)), False)
         self._b = b.value
+        self._n = len(ops)
 
     @property
     def items(self) -> int:
@@ -505,12 +508,24 @@ class Batch:
         _check(lib().skinny_dbatch_run(self._b, _stream(stream)), False)
 
     def tune(self, reps=3):
-        """Time plans of the batch (item sizes, grids) and keep the fastest;
-        returns (ms, item_bytes (0: automatic), grid)."""
+        """Tune the tile class of each op shape, then time plans of the batch
+        (item sizes, grids) and keep the fastest; returns (ms, item_bytes
+        (0: automatic), grid)."""
         ms, ib, g = C.c_double(), C.c_int64(), C.c_int()
         _check(lib().skinny_dbatch_tune(self._b, reps, C.byref(ms), C.byref(ib), C.byref(g)),
                False)
         return ms.value, ib.value, g.value
+
+    def set_class(self, op: int, cls: int) -> bool:
+        """Run op `op` with tile class `cls` (-1: the default rule); False if
+        that class cannot compute the op (the plan is then unchanged)."""
+        return _check(lib().skinny_dbatch_set_class(self._b, op, cls))
+
+    def classes(self):
+        """The tile class of every op in the current plan."""
+        arr = (C.c_int * self._n)()
+        _check(lib().skinny_dbatch_classes(self._b, C.cast(arr, vp)), False)
+        return list(arr)
 
     def close(self) -> None:
         if self._b:

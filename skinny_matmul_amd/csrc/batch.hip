@@ -18,6 +18,8 @@
 //    op's partials in item order and applies alpha / beta, so the result does
 //    not depend on the schedule.  A TSMM item owns its rows (in place allowed).
 #include <algorithm>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "batch_items.hpp"
@@ -105,7 +107,11 @@ struct skinny_batch_s {
   int nops = 0;
   int64_t item_bytes = 0;  // as planned (0: the automatic size)
   size_t partial_bytes = 0;
+  int grid_req = 0;        // grid asked for (grid is at most the item count)
   std::vector<skinny_batch_op> ops;  // host copy, for re-planning
+  std::vector<int> cls_force;        // per op: tile class, or -1 (default rule)
+  std::vector<char> cls_user;        // per op: the class was set by the caller
+  std::vector<int> cls_plan;         // per op: the class of the current plan
   skinny::BatchOpDev* d_ops = nullptr;
   skinny::BatchItemDev* d_items = nullptr;
   double* d_part = nullptr;
@@ -132,9 +138,12 @@ skinny_status_t batch_build(skinny_batch_t b, int64_t item_bytes, int grid) {
   std::vector<BatchOpDev> dops;
   std::vector<BatchItemDev> items;
   size_t part_doubles = 0;
-  skinny_status_t ps =
-      batch_plan(b->nops, b->ops.data(), item_bytes, grid, dops, items, part_doubles);
+  skinny_status_t ps = batch_plan(b->nops, b->ops.data(), item_bytes, grid, dops, items,
+                                  part_doubles, b->cls_force.data());
   if (ps != SKINNY_OK) return ps;
+  b->cls_plan.resize(b->nops);
+  for (int i = 0; i < b->nops; i++) b->cls_plan[i] = dops[i].cls;
+  b->grid_req = grid;
   batch_free_device(b);
   b->nitems = (int)items.size();
   b->grid = std::max(1, std::min(b->nitems, grid));
@@ -186,6 +195,8 @@ skinny_status_t skinny_dbatch_create(skinny_handle_t h, int nops,
   b->cu_count = h->cu_count;
   b->nops = nops;
   b->ops.assign(ops, ops + nops);
+  b->cls_force.assign(nops, -1);
+  b->cls_user.assign(nops, 0);
   skinny_status_t s = batch_build(b, item_bytes, 2 * h->cu_count);
   if (s != SKINNY_OK) {
     skinny_dbatch_destroy(b);
@@ -195,9 +206,80 @@ skinny_status_t skinny_dbatch_create(skinny_handle_t h, int nops,
   return SKINNY_OK;
 }
 
-// Times plans of the batch (item sizes: automatic, 1, 2, 4, 8 and 16 MiB;
-// 1 or 2 workgroups per CU, the kernel's residency), keeps the fastest.
-// TSMTTSM results follow the plan's partition: reproducible for a fixed plan.
+// Median of `reps` timed runs of the batch's current plan after one warm-up
+// run (events on the null stream).
+static skinny_status_t batch_time(skinny_batch_t b, int reps, hipEvent_t e0, hipEvent_t e1,
+                                  float* ms_out) {
+  std::vector<float> t;
+  for (int r = 0; r <= reps; r++) {
+    (void)hipEventRecord(e0, nullptr);
+    skinny_status_t s = skinny_dbatch_run(b, nullptr);
+    (void)hipEventRecord(e1, nullptr);
+    if (s != SKINNY_OK) return s;
+    const hipError_t e = hipEventSynchronize(e1);
+    if (e != hipSuccess) return hip_fail(e, "batch tune run");
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (r > 0) t.push_back(ms);  // the first run warms up
+  }
+  std::sort(t.begin(), t.end());
+  *ms_out = t[t.size() / 2];
+  return SKINNY_OK;
+}
+
+// Tile classes: the ops of one shape (op kind, M, N, Gram or not) whose class
+// the caller did not set are timed alone, as a batch of their own, with each
+// candidate class (ts_class_candidates / tm_class_candidates), and take the
+// fastest.
+static skinny_status_t batch_tune_classes(skinny_batch_t b, int reps, hipEvent_t e0,
+                                          hipEvent_t e1) {
+  std::map<std::tuple<int, int64_t, int64_t, bool>, std::vector<int>> shapes;
+  for (int i = 0; i < b->nops; i++) {
+    if (b->cls_user[i]) continue;
+    const skinny_batch_op& o = b->ops[i];
+    shapes[std::make_tuple(o.op, o.M, o.N, batch_op_gram(o))].push_back(i);
+  }
+  std::vector<int> cand;
+  for (const auto& kv : shapes) {
+    const int kind = std::get<0>(kv.first);
+    const int64_t M = std::get<1>(kv.first), N = std::get<2>(kv.first);
+    if (kind == 0)
+      ts_class_candidates(M, N, std::get<3>(kv.first), cand);
+    else
+      tm_class_candidates(M, N, cand);
+    if (cand.size() < 2) continue;
+    const std::vector<int>& idx = kv.second;
+    skinny_batch_s sub;
+    sub.device = b->device;
+    sub.cu_count = b->cu_count;
+    sub.nops = (int)idx.size();
+    for (int i : idx) sub.ops.push_back(b->ops[i]);
+    sub.cls_user.assign(sub.nops, 0);
+    float best = 1e30f;
+    int best_c = -1;
+    skinny_status_t s = SKINNY_OK;
+    for (int c : cand) {
+      sub.cls_force.assign(sub.nops, c);
+      s = batch_build(&sub, 0, 2 * b->cu_count);
+      float ms = 0.0f;
+      if (s == SKINNY_OK) s = batch_time(&sub, reps, e0, e1, &ms);
+      if (s != SKINNY_OK) break;
+      if (ms < best) {
+        best = ms;
+        best_c = c;
+      }
+    }
+    batch_free_device(&sub);
+    if (s != SKINNY_OK) return s;
+    for (int i : idx) b->cls_force[i] = best_c;
+  }
+  return SKINNY_OK;
+}
+
+// Tunes the tile class of every op shape (above), then times plans of the
+// batch (item sizes: automatic, 1, 2, 4, 8 and 16 MiB; 1 or 2 workgroups per
+// CU, the kernel's residency) and keeps the fastest.  TSMTTSM results follow
+// the plan's partition: reproducible for a fixed plan.
 skinny_status_t skinny_dbatch_tune(skinny_batch_t b, int reps, double* best_ms,
                                    int64_t* item_bytes, int* grid) {
   if (!b) return fail(SKINNY_INVALID_ARG, "batch tune: null batch");
@@ -206,29 +288,19 @@ skinny_status_t skinny_dbatch_tune(skinny_batch_t b, int reps, double* best_ms,
   hipError_t e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   if (e != hipSuccess) return hip_fail(e, "batch tune events");
+  skinny_status_t s = batch_tune_classes(b, reps, e0, e1);
   const int64_t sizes[] = {0, 1 << 20, 2 << 20, 4 << 20, 8 << 20, 16 << 20};
   float best = 1e30f;
   int64_t best_ib = 0;
   int best_g = 2 * b->cu_count;
-  skinny_status_t s = SKINNY_OK;
   for (int per = 1; per <= 2 && s == SKINNY_OK; per++) {
     for (int64_t ib : sizes) {
       s = batch_build(b, ib, per * b->cu_count);
+      float ms = 0.0f;
+      if (s == SKINNY_OK) s = batch_time(b, reps, e0, e1, &ms);
       if (s != SKINNY_OK) break;
-      std::vector<float> t;
-      for (int r = 0; r <= reps && s == SKINNY_OK; r++) {
-        (void)hipEventRecord(e0, nullptr);
-        s = skinny_dbatch_run(b, nullptr);
-        (void)hipEventRecord(e1, nullptr);
-        (void)hipEventSynchronize(e1);
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (r > 0) t.push_back(ms);  // the first run warms up
-      }
-      if (s != SKINNY_OK) break;
-      std::sort(t.begin(), t.end());
-      if (t[t.size() / 2] < best) {
-        best = t[t.size() / 2];
+      if (ms < best) {
+        best = ms;
         best_ib = ib;
         best_g = per * b->cu_count;
       }
@@ -241,6 +313,31 @@ skinny_status_t skinny_dbatch_tune(skinny_batch_t b, int reps, double* best_ms,
   if (best_ms) *best_ms = best;
   if (item_bytes) *item_bytes = best_ib;
   if (grid) *grid = b->grid;
+  return SKINNY_OK;
+}
+
+skinny_status_t skinny_dbatch_set_class(skinny_batch_t b, int op, int cls) {
+  if (!b) return fail(SKINNY_INVALID_ARG, "batch: null batch");
+  if (op < 0 || op >= b->nops) return fail(SKINNY_INVALID_ARG, "batch: op index out of range");
+  const skinny_batch_op& o = b->ops[op];
+  if (cls >= 0 && !(o.op == 0 ? ts_class_valid(cls, o.M, o.N, batch_op_gram(o))
+                              : tm_class_valid(cls, o.M, o.N)))
+    return fail(SKINNY_UNSUPPORTED, "batch: tile class cannot compute the op");
+  const int prev = b->cls_force[op];
+  const char prev_user = b->cls_user[op];
+  b->cls_force[op] = cls < 0 ? -1 : cls;
+  b->cls_user[op] = cls < 0 ? 0 : 1;
+  skinny_status_t s = batch_build(b, b->item_bytes, b->grid_req);
+  if (s != SKINNY_OK) {  // keep the previous plan's choice
+    b->cls_force[op] = prev;
+    b->cls_user[op] = prev_user;
+  }
+  return s;
+}
+
+skinny_status_t skinny_dbatch_classes(skinny_batch_t b, int* cls) {
+  if (!b || !cls) return fail(SKINNY_INVALID_ARG, "batch classes: null argument");
+  for (int i = 0; i < b->nops; i++) cls[i] = b->cls_plan[i];
   return SKINNY_OK;
 }
 

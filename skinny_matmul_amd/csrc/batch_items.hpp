@@ -129,6 +129,71 @@ static int tm_class(int64_t M, int64_t N) {
   return 3 * iS + iN;
 }
 
+// S of a TSMM matrix-core class (columns of A per lane group), 0 for the
+// row classes 15 and 16
+static int tm_class_S(int cls) {
+  static const int S[5] = {1, 2, 4, 8, 16};
+  return cls >= 15 ? 0 : S[cls / 3];
+}
+
+// ---- which tile classes can compute an op --------------------------------
+// Any of them gives the op's result (on integer data the same bits); they
+// differ in loads per row, padding and registers, which is what the batch
+// tuner times.  TSMTTSM: packed class P needs P max(M, N) <= 16 (P rows of
+// both operands in 16 lanes); subtile class (MT, NT) needs 16 MT >= M and
+// 16 NT >= N; the Gram classes a Gram op (B is A, lda == ldb, M == N)
+// within their square tile.  TSMM: class (S, NT) needs 4 S >= M and
+// 16 NT >= N; the row classes M <= 1 (15) or M <= 2 (16), and N <= 2.
+static bool ts_class_valid(int cls, int64_t M, int64_t N, bool gram) {
+  if (cls < 0 || cls > 17) return false;
+  if (cls < 6) {
+    static const int P[6] = {16, 8, 5, 4, 3, 2};
+    return (int64_t)P[cls] * std::max(M, N) <= 16;
+  }
+  if (cls >= 15) {
+    const int side = cls == 15 ? 16 : cls == 16 ? 32 : 64;
+    return gram && M <= side;
+  }
+  static const int sz[3] = {1, 2, 4};
+  return 16 * sz[(cls - 6) / 3] >= M && 16 * sz[(cls - 6) % 3] >= N;
+}
+static bool tm_class_valid(int cls, int64_t M, int64_t N) {
+  if (cls < 0 || cls > 16) return false;
+  if (cls >= 15) return M <= (cls == 15 ? 1 : 2) && N <= 2;
+  static const int nt[3] = {1, 2, 4};
+  return 4 * tm_class_S(cls) >= M && 16 * nt[cls % 3] >= N;
+}
+
+// The classes the batch tuner times for an op: the default rule's class and
+// the alternatives that pad nothing or little.  TSMTTSM with max(M, N) <= 8:
+// the packed classes of the largest two P that fit and the unpacked 16 x 16
+// tile; a Gram op: its Gram class and the general class of the same tile.
+// TSMM with M, N <= 2: the row class and the matrix-core class.
+static void ts_class_candidates(int64_t M, int64_t N, bool gram, std::vector<int>& out) {
+  out.clear();
+  const int64_t W = std::max(M, N);
+  if (W <= 8) {
+    int npacked = 0;
+    for (int c = 0; c < 6 && npacked < 2; c++)
+      if (ts_class_valid(c, M, N, gram)) {
+        out.push_back(c);
+        npacked++;
+      }
+    out.push_back(6);
+    if (gram) out.push_back(15);
+    return;
+  }
+  const int base = ts_class(M, N);
+  out.push_back(base);
+  if (gram && ts_gram_class(base) != base) out.push_back(ts_gram_class(base));
+}
+static void tm_class_candidates(int64_t M, int64_t N, std::vector<int>& out) {
+  out.clear();
+  const int base = tm_class(M, N);
+  out.push_back(base);
+  if (base >= 15) out.push_back(0);  // S = 1, NT = 1: 4 >= M, 16 >= N
+}
+
 // BETA0 is a template parameter: with a run-time choice the compiler may
 // load C before every store (the store makes the load safe to speculate),
 // which adds a read of C to an op that should only write it.
@@ -309,11 +374,17 @@ __device__ void run_tm_item(const BatchOpDev& o, const BatchItemDev& it) {
 // TSMTTSM op).  item_bytes <= 0 picks about 8 items per workgroup of a grid
 // of `grid` workgroups, within 1..8 MiB per item: a batch of few large ops
 // still gives every workgroup work, one of many ops keeps large items.
+// cls_force (may be null): per op a tile class to use instead of the default
+// rule, or -1; a class that cannot compute the op is refused.
+inline bool batch_op_gram(const skinny_batch_op& o) {
+  return o.op == 0 && o.A == o.B && o.lda == o.ldb && o.M == o.N;
+}
 inline skinny_status_t batch_plan(int nops, const skinny_batch_op* ops,
                                   int64_t item_bytes, int grid,
                                   std::vector<BatchOpDev>& dops,
                                   std::vector<BatchItemDev>& items,
-                                  size_t& part_doubles) {
+                                  size_t& part_doubles,
+                                  const int* cls_force = nullptr) {
   if (item_bytes <= 0) {
     double total = 0.0;
     for (int i = 0; i < nops; i++)
@@ -350,7 +421,12 @@ inline skinny_status_t batch_plan(int nops, const skinny_batch_op* ops,
         return fail(SKINNY_INVALID_ARG, "batch: leading dimension too small");
       int rows_min;
       d.cls = ts_class(o.M, o.N);
-      if (o.A == o.B && o.lda == o.ldb && o.M == o.N) d.cls = ts_gram_class(d.cls);
+      if (batch_op_gram(o)) d.cls = ts_gram_class(d.cls);
+      if (cls_force && cls_force[i] >= 0) {
+        if (!ts_class_valid(cls_force[i], o.M, o.N, batch_op_gram(o)))
+          return fail(SKINNY_UNSUPPORTED, "batch: tile class cannot compute the op");
+        d.cls = cls_force[i];
+      }
       ts_class_shape(d.cls, o.M, o.N, &d.outM, &d.outN, &rows_min);
       int64_t rows = std::max<int64_t>(item_bytes / row_bytes, rows_min);
       rows = std::max<int64_t>(rows, (o.K + 63) / 64);  // at most 64 partials
@@ -367,9 +443,16 @@ inline skinny_status_t batch_plan(int nops, const skinny_batch_op* ops,
       if (o.A == o.C && o.lda != o.ldc)
         return fail(SKINNY_UNSUPPORTED, "batch: in place needs lda == ldc");
       d.cls = tm_class(o.M, o.N);
+      if (cls_force && cls_force[i] >= 0) {
+        if (!tm_class_valid(cls_force[i], o.M, o.N))
+          return fail(SKINNY_UNSUPPORTED, "batch: tile class cannot compute the op");
+        d.cls = cls_force[i];
+      }
       // 16-byte row slices: the class's slice of S columns per lane group is
-      // exactly M / 4 and even (M = 8, 16, 32 or 64), even lda, aligned A
-      d.vec = ((o.M == 8 || o.M == 16 || o.M == 32 || o.M == 64) && o.lda % 2 == 0 &&
+      // exactly M / 4 and even (M = 8, 16, 32 or 64 on its own class), even
+      // lda, aligned A
+      const int S = tm_class_S(d.cls);
+      d.vec = (S % 2 == 0 && o.M == 4 * S && o.lda % 2 == 0 &&
                (reinterpret_cast<uintptr_t>(o.A) % 16) == 0) ? 1 : 0;
       // P interleaved items of about item_bytes each
       int64_t rows = std::max<int64_t>(item_bytes / row_bytes, 64);

@@ -171,3 +171,133 @@ def test_batch_tune_replans_and_stays_exact(gpu, orc):
         for (kind, M, N, K, C0, want, dC) in refs:
             assert np.array_equal(host(dC), want), (kind, M, N, K, ib, g)
     b.close()
+
+
+# Which tile classes can compute an op, stated from the kernels' shapes: a
+# packed TSMTTSM class puts P rows of both operands into the 16 matrix-core
+# lanes, a subtile class covers 16 MT x 16 NT entries, a Gram class one
+# square tile of a Gram op; a TSMM class covers 4 S columns of A and 16 NT of
+# C, a row class M <= 1 or 2 with N <= 2.
+_PACK = (16, 8, 5, 4, 3, 2)
+_SUB = (1, 2, 4)
+
+
+def _ts_valid(cls, M, N, gram):
+    if cls < 6:
+        return _PACK[cls] * max(M, N) <= 16
+    if cls >= 15:
+        return gram and M <= (16, 32, 64)[cls - 15]
+    return 16 * _SUB[(cls - 6) // 3] >= M and 16 * _SUB[(cls - 6) % 3] >= N
+
+
+def _tm_valid(cls, M, N):
+    if cls >= 15:
+        return M <= cls - 14 and N <= 2
+    return 4 * (1, 2, 4, 8, 16)[cls // 3] >= M and 16 * _SUB[cls % 3] >= N
+
+
+def _one_op(orc, rng, kind, M, N, K, gram=False, inplace=False, beta=0.0):
+    seed = rng.randint(0, 1 << 30)
+    if kind == "tsmttsm":
+        lda = M + rng.randint(0, 2)
+        ldb = lda if gram else N + rng.randint(0, 2)
+        A = orc.integers(K * lda, seed, 0)
+        B = A if gram else orc.integers(K * ldb, seed, 1)
+        C = orc.integers(N * M, seed, 2)
+        want = C.copy()
+        orc.tsmttsm(M, N, K, A, lda, 1.0, B, ldb, beta, want, M)
+        dA = dev(A)
+        dB = dA if gram else dev(B)
+        dC = dev(C)
+        return (kind, M, N, K, dA, lda, 1.0, dB, ldb, beta, dC, M), C, want, dC
+    ldb = M + rng.randint(0, 2)
+    if inplace:
+        lda = ldc = max(M, N) + rng.randint(0, 1)
+    else:
+        lda, ldc = M + rng.randint(0, 2), N + rng.randint(0, 2)
+    B = orc.integers(N * ldb, seed, 1)
+    C = orc.integers(K * ldc, seed, 2)
+    A = C.copy() if inplace else orc.integers(K * lda, seed, 0)
+    want = C.copy()
+    orc.tsmm(M, N, K, want if inplace else A, lda, 1.0, B, ldb, beta, want, ldc)
+    dC = dev(C)
+    dA = dC if inplace else dev(A)
+    return (kind, M, N, K, dA, lda, 1.0, dev(B), ldb, beta, dC, ldc), C, want, dC
+
+
+TS_SHAPES = [(1, 1, False), (2, 3, False), (4, 4, False), (5, 2, False), (8, 8, False),
+             (3, 3, True), (8, 8, True), (12, 3, False), (16, 16, True), (20, 40, False),
+             (24, 24, True), (33, 64, False), (64, 64, True)]
+TM_SHAPES = [(1, 1, False, 0.0), (1, 2, True, 1.0), (2, 2, False, 1.0), (2, 1, True, 0.0),
+             (3, 5, False, 0.0), (8, 8, False, 0.0), (8, 8, True, 1.0), (12, 20, False, 1.0),
+             (33, 17, False, 0.0), (64, 64, True, 0.0)]
+
+
+def test_batch_every_tile_class_exact(gpu, orc):
+    """skinny_dbatch_set_class accepts exactly the tile classes that can
+    compute an op, and every accepted class gives the oracle's result bitwise
+    on integer data (TSMTTSM general and Gram, TSMM out of place and in place,
+    beta 0 and 1); a refused class leaves the plan as it was."""
+    import torch
+    rng = random.Random(11)
+    cases = [("tsmttsm", M, N, g, False, 0.0) for (M, N, g) in TS_SHAPES]
+    cases += [("tsmm", M, N, False, ip, beta) for (M, N, ip, beta) in TM_SHAPES]
+    for kind, M, N, gram, inplace, beta in cases:
+        K = rng.randint(3000, 9000)
+        op, C0, want, dC = _one_op(orc, rng, kind, M, N, K, gram, inplace, beta)
+        b = gpu.Batch([op], item_bytes=1 << 16)
+        default = b.classes()[0]
+        ncls = 18 if kind == "tsmttsm" else 17
+        tried = 0
+        for cls in list(range(ncls)) + [-1]:
+            valid = cls < 0 or (_ts_valid(cls, M, N, gram) if kind == "tsmttsm"
+                                else _tm_valid(cls, M, N))
+            note(f"batch class: {kind} M={M} N={N} gram={gram} inplace={inplace} "
+                 f"beta={beta} class={cls}")
+            before = b.classes()[0]
+            ok = b.set_class(0, cls)
+            assert ok == valid, (kind, M, N, cls)
+            if not ok:
+                assert b.classes()[0] == before
+                continue
+            assert b.classes()[0] == (default if cls < 0 else cls)
+            dC.copy_(torch.from_numpy(C0))
+            torch.cuda.synchronize()
+            b.run()
+            assert np.array_equal(host(dC), want), (kind, M, N, gram, inplace, beta, cls)
+            tried += 1
+        assert tried >= 2
+        b.close()
+
+
+def test_batch_tune_classes_valid_and_kept(gpu, orc):
+    """After skinny_dbatch_tune every op runs a class that can compute it, a
+    class the caller set is kept, and the results stay exact."""
+    import torch
+    rng = random.Random(21)
+    ops, refs = [], []
+    for (M, N, g) in TS_SHAPES[:8]:
+        op, C0, want, dC = _one_op(orc, rng, "tsmttsm", M, N, 40000, g)
+        ops.append(op)
+        refs.append((C0, want, dC))
+    for (M, N, ip, beta) in TM_SHAPES[:5]:
+        op, C0, want, dC = _one_op(orc, rng, "tsmm", M, N, 40000, False, ip, beta)
+        ops.append(op)
+        refs.append((C0, want, dC))
+    b = gpu.Batch(ops)
+    assert b.set_class(4, 6)  # TSMTTSM 8 x 8 on the unpacked 16 x 16 tile
+    b.tune(reps=1)
+    cls = b.classes()
+    assert cls[4] == 6
+    for i, o in enumerate(ops):
+        kind, M, N = o[0], o[1], o[2]
+        gram = kind == "tsmttsm" and o[4] is o[7]
+        assert (_ts_valid(cls[i], M, N, gram) if kind == "tsmttsm"
+                else _tm_valid(cls[i], M, N)), (i, kind, M, N, cls[i])
+    for C0, want, dC in refs:
+        dC.copy_(torch.from_numpy(C0))
+    torch.cuda.synchronize()
+    b.run()
+    for (C0, want, dC), o in zip(refs, ops):
+        assert np.array_equal(host(dC), want), o[:4]
+    b.close()
