@@ -9,7 +9,9 @@
   mirrored tiles above the diagonal, bitwise against the oracle at
   blockCount = 1 (ragged K, beta != 0, several tiles, an outlier);
 * default blockCount: error against the exact product within the scheme's
-  bound and within 8 K eps at K = 2^24 (1.5e-8) from 3 limbs on.
+  bound and within 8 K eps at K = 2^24 (1.5e-8) from 3 limbs on;
+* operands above 2 GiB (4 GiB each, every limb count, Gram too): exact on
+  small-integer data, against the FP64 kernel and host int64 dot products.
 """
 import numpy as np
 import pytest
@@ -119,3 +121,38 @@ def test_c3_error(gpu, orc, S):
     err = np.sqrt(np.sum((got[ms, ns] - hi - lo) ** 2)) / np.sqrt(np.sum(hi ** 2))
     assert err <= 8 * (S + 1) * 2.0 ** (-7 * S)
     assert err <= 8 * 2 ** 24 * np.finfo(float).eps
+
+
+@pytest.mark.parametrize("M,N,S,gram", [(128, 128, 3, False), (128, 128, 3, True),
+                                        (130, 70, 4, False), (64, 96, 5, False)])
+def test_operands_above_2GiB_exact(gpu, M, N, S, gram):
+    """Operands of more than 2 GiB (K = 2^22 + 37 rows, default blockCount):
+    the per-chunk buffer resources address them in full.  On integer data in
+    [-2, 2] every limb, INT32 level and flush is exact, so the result is the
+    exact product: bitwise equal to the FP64 kernel's (also exact), and three
+    entries recomputed on the host from the device's columns in int64."""
+    import torch
+    K = (1 << 22) + 37
+    note(f"ozaki above 2 GiB: M={M} N={N} S={S} gram={gram} K={K}")
+    A = torch.empty(K * M, dtype=torch.float64, device="cuda")
+    gpu.fill_int(A, A.numel(), 77, 0, -2, 2)
+    if gram:
+        B, N = A, M
+    else:
+        B = torch.empty(K * N, dtype=torch.float64, device="cuda")
+        gpu.fill_int(B, B.numel(), 77, 1, -2, 2)
+    assert A.numel() * 8 > (2 << 30)
+    Co = torch.zeros(M * N, dtype=torch.float64, device="cuda")
+    Cf = torch.zeros(M * N, dtype=torch.float64, device="cuda")
+    assert gpu.dtsmttsm_ozaki(0, M, N, K, A, M, 1.0, B, N, 0.0, Co, M, slices=S)
+    assert gpu.dtsmttsm(0, M, N, K, A, M, 1.0, B, N, 0.0, Cf, M)
+    torch.cuda.synchronize()
+    got, ref = host(Co), host(Cf)
+    assert np.array_equal(got, ref)
+    A2, B2 = A.view(K, M), B.view(K, N)
+    for m, n in ((0, 0), (M - 1, N - 1), (5, 17)):
+        a = host(A2[:, m].contiguous()).astype(np.int64)
+        b = host(B2[:, n].contiguous()).astype(np.int64)
+        assert got[n * M + m] == float(np.dot(a, b)), (m, n)
+    del A, B
+    torch.cuda.empty_cache()
