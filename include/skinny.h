@@ -422,7 +422,9 @@ skinny_status_t skinny_dorth_error(skinny_handle_t h, int64_t M, int64_t K,
 skinny_status_t skinny_mb_stream(int kind, size_t bytes_per_array, int reps,
                                  double* gbs, skinny_stream_t stream);
 /* kind 0: FP64 vector FMA, 1: FP64 matrix core (16x16x4), 2: INT8 matrix
- * core, 3: FP64 matrix core (4x4x4, four blocks); TFLOP/s (TOP/s).
+ * core, 3: FP64 matrix core (4x4x4, four blocks), 4: FP32 matrix core
+ * (16x16x4), 5: FP32 matrix core (32x32x2), 6: FP32 matrix core (4x4x1,
+ * 16 blocks); TFLOP/s (TOP/s).
  * Replaces microbenches/roofline.cu:19-66. */
 skinny_status_t skinny_mb_peak(int kind, int reps, double* tflops,
                                skinny_stream_t stream);

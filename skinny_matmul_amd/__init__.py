@@ -596,7 +596,8 @@ def orth_error(M, K, X, ldx, handle=None, stream=None) -> float:
 
 
 STREAM_KINDS = {"read": 0, "copy": 1, "triad": 2, "write": 3, "update": 4, "copy_variants": 5}
-PEAK_KINDS = {"fp64_fma": 0, "fp64_mfma": 1, "int8_mfma": 2, "fp64_mfma_4x4x4": 3}
+PEAK_KINDS = {"fp64_fma": 0, "fp64_mfma": 1, "int8_mfma": 2, "fp64_mfma_4x4x4": 3,
+              "fp32_mfma": 4, "fp32_mfma_32x32x2": 5, "fp32_mfma_4x4x1": 6}
 
 
 def mb_stream(kind: str, bytes_per_array: int = 4 << 30, reps: int = 5, stream=None) -> float:

@@ -11,9 +11,10 @@ The analogue of the reference's membw and roofline microbenchmarks
   Cache): read, copy, triad, write and update (a += s b), GB/s of array
   traffic, best of --reps; and the best of 40 other copy patterns
   (nontemporal stores, per-wave bursts of 16 KiB to 1 MiB);
-* FP64 vector FMA, FP64 matrix-core (16x16x4 and four-block 4x4x4) and INT8
-  matrix-core (32x32x32) peaks in TFLOP/s (TOP/s), each with the shader
-  clock measured while its loop runs.
+* FP64 vector FMA, FP64 matrix-core (16x16x4 and four-block 4x4x4), INT8
+  matrix-core (32x32x32) and FP32 matrix-core (16x16x4, 32x32x2 and the
+  16-block 4x4x1) peaks in TFLOP/s (TOP/s), each with the shader clock
+  measured while its loop runs.
 
 * with --hierarchy: ns per dependent load over working sets from 4 KiB to
   1 GiB, GB/s re-reading buffers from 4 MiB to 8 GiB, LDS GB/s and latency
@@ -32,7 +33,8 @@ import sys
 from . import mb_clock, mb_peak, mb_stream
 
 STREAMS = ("read", "copy", "copy_variants", "triad", "write", "update")
-PEAKS = ("fp64_fma", "fp64_mfma", "fp64_mfma_4x4x4", "int8_mfma")
+PEAKS = ("fp64_fma", "fp64_mfma", "fp64_mfma_4x4x4", "int8_mfma", "fp32_mfma",
+         "fp32_mfma_32x32x2", "fp32_mfma_4x4x1")
 
 
 def measure(gib: float = 4.0, reps: int = 5) -> dict:

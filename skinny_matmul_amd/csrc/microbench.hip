@@ -237,6 +237,50 @@ __global__ void __launch_bounds__(256) mfma_f64_q4_kernel(double* out, int iters
   clock_stamp(clk, 1);
 }
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+// FP32 matrix core, three instruction shapes with independent chains per wave:
+// F = 0: 16x16x4 (4 chains), F = 1: 32x32x2 (2 chains), F = 2: 4x4x1 with 16
+// blocks (4 chains)
+template <int F>
+__global__ void __launch_bounds__(256) mfma_f32_kernel(float* out, int iters, float x,
+                                                       unsigned long long* clk) {
+  clock_stamp(clk, 0);
+  const float a = x + threadIdx.x, b = x - threadIdx.x;
+  float s = 0.0f;
+  if constexpr (F == 1) {
+    v16f acc[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[j][r] = 0.0f;
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+        acc[j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) s += acc[j][r];
+  } else {
+    v4f acc[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[j] = v4f{0, 0, 0, 0};
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        acc[j] = F == 0 ? __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[j], 0, 0, 0)
+                        : __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, acc[j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+  }
+  if (s == 12345.678f) out[threadIdx.x] = s;
+  clock_stamp(clk, 1);
+}
+
 // The idle clock: one wave reads both counters about `spin` real-time ticks
 // apart (the loop ends once the real-time counter has advanced that far).
 __global__ void clock_kernel(unsigned long long* clk, int spin) {
@@ -611,11 +655,35 @@ skinny_status_t skinny_mb_barrier(int per_cu, int reps, double* us,
   return SKINNY_OK;
 }
 
+// The peak loop of `kind` (as skinny_mb_peak) on `blocks` workgroups, with
+// the clock stamps written to clk when it is not null.
+static void launch_peak(int kind, int blocks, int iters, void* out, unsigned long long* clk,
+                        hipStream_t st) {
+  switch (kind) {
+    case 0: hipLaunchKernelGGL(fma_kernel, dim3(blocks), dim3(256), 0, st, (double*)out,
+                               iters, 1.0, clk); break;
+    case 1: hipLaunchKernelGGL(mfma_f64_kernel, dim3(blocks), dim3(256), 0, st, (double*)out,
+                               iters, 1.0, clk); break;
+    case 2: hipLaunchKernelGGL(mfma_i8_kernel, dim3(blocks), dim3(256), 0, st, (int*)out,
+                               iters, 1, clk); break;
+    case 3: hipLaunchKernelGGL(mfma_f64_q4_kernel, dim3(blocks), dim3(256), 0, st,
+                               (double*)out, iters, 1.0, clk); break;
+    case 4: hipLaunchKernelGGL(mfma_f32_kernel<0>, dim3(blocks), dim3(256), 0, st, (float*)out,
+                               iters, 1.0f, clk); break;
+    case 5: hipLaunchKernelGGL(mfma_f32_kernel<1>, dim3(blocks), dim3(256), 0, st, (float*)out,
+                               iters, 1.0f, clk); break;
+    default: hipLaunchKernelGGL(mfma_f32_kernel<2>, dim3(blocks), dim3(256), 0, st,
+                                (float*)out, iters, 1.0f, clk); break;
+  }
+}
+
 // kind 0: FP64 vector FMA, 1: FP64 MFMA 16x16x4, 2: INT8 MFMA 32x32x32,
-// 3: FP64 MFMA 4x4x4 (four blocks).  Result in TFLOP/s (TOP/s for INT8).
+// 3: FP64 MFMA 4x4x4 (four blocks), 4: FP32 MFMA 16x16x4, 5: FP32 MFMA
+// 32x32x2, 6: FP32 MFMA 4x4x1 (16 blocks).  Result in TFLOP/s (TOP/s for
+// INT8).
 skinny_status_t skinny_mb_peak(int kind, int reps, double* tflops,
                                skinny_stream_t stream) {
-  if (!tflops || kind < 0 || kind > 3 || reps < 1)
+  if (!tflops || kind < 0 || kind > 6 || reps < 1)
     return fail(SKINNY_INVALID_ARG, "skinny_mb_peak: bad arguments");
   skinny_handle_t h = skinny_default_handle();
   if (!h) return no_handle("microbench");
@@ -625,20 +693,7 @@ skinny_status_t skinny_mb_peak(int kind, int reps, double* tflops,
   if (e != hipSuccess) return hip_fail(e, "skinny_mb_peak alloc");
   const int blocks = 8 * h->cu_count;
   const int iters = kind == 0 ? 4096 : 1024;
-  auto launch = [&]() {
-    if (kind == 0)
-      hipLaunchKernelGGL(fma_kernel, dim3(blocks), dim3(256), 0, st,
-                         (double*)out, iters, 1.0, nullptr);
-    else if (kind == 1)
-      hipLaunchKernelGGL(mfma_f64_kernel, dim3(blocks), dim3(256), 0, st,
-                         (double*)out, iters, 1.0, nullptr);
-    else if (kind == 2)
-      hipLaunchKernelGGL(mfma_i8_kernel, dim3(blocks), dim3(256), 0, st,
-                         (int*)out, iters, 1, nullptr);
-    else
-      hipLaunchKernelGGL(mfma_f64_q4_kernel, dim3(blocks), dim3(256), 0, st,
-                         (double*)out, iters, 1.0, nullptr);
-  };
+  auto launch = [&]() { launch_peak(kind, blocks, iters, out, nullptr, st); };
   float ms = time_ms(launch, st, reps);
   e = hipGetLastError();
   (void)hipFree(out);
@@ -651,8 +706,14 @@ skinny_status_t skinny_mb_peak(int kind, int reps, double* tflops,
     ops = threads / 64.0 * iters * 4 * (16.0 * 16 * 4 * 2);
   else if (kind == 2)
     ops = threads / 64.0 * iters * 2 * (32.0 * 32 * 32 * 2);
-  else
+  else if (kind == 3)
     ops = threads / 64.0 * iters * 16 * (4.0 * 4 * 4 * 4 * 2);
+  else if (kind == 4)
+    ops = threads / 64.0 * iters * 4 * (16.0 * 16 * 4 * 2);
+  else if (kind == 5)
+    ops = threads / 64.0 * iters * 2 * (32.0 * 32 * 2 * 2);
+  else
+    ops = threads / 64.0 * iters * 4 * (16.0 * 4 * 4 * 1 * 2);
   *tflops = ops / (ms * 1e-3) / 1e12;
   return SKINNY_OK;
 }
@@ -661,7 +722,7 @@ skinny_status_t skinny_mb_peak(int kind, int reps, double* tflops,
 // from the counters workgroup 0 of that loop reads at its start and end;
 // kind -1: a one-wave spin of about 1 ms with nothing else running.
 skinny_status_t skinny_mb_clock(int kind, double* mhz, skinny_stream_t stream) {
-  if (!mhz || kind < -1 || kind > 3)
+  if (!mhz || kind < -1 || kind > 6)
     return fail(SKINNY_INVALID_ARG, "skinny_mb_clock: bad arguments");
   skinny_handle_t h = skinny_default_handle();
   if (!h) return no_handle("microbench");
@@ -679,16 +740,8 @@ skinny_status_t skinny_mb_clock(int kind, double* mhz, skinny_stream_t stream) {
     return hip_fail(e, "skinny_mb_clock alloc");
   }
   const int blocks = 8 * h->cu_count;
-  if (kind == 0)
-    hipLaunchKernelGGL(fma_kernel, dim3(blocks), dim3(256), 0, st, (double*)out, 16384, 1.0, clk);
-  else if (kind == 1)
-    hipLaunchKernelGGL(mfma_f64_kernel, dim3(blocks), dim3(256), 0, st, (double*)out, 4096, 1.0,
-                       clk);
-  else if (kind == 2)
-    hipLaunchKernelGGL(mfma_i8_kernel, dim3(blocks), dim3(256), 0, st, (int*)out, 4096, 1, clk);
-  else if (kind == 3)
-    hipLaunchKernelGGL(mfma_f64_q4_kernel, dim3(blocks), dim3(256), 0, st, (double*)out, 4096,
-                       1.0, clk);
+  if (kind >= 0)
+    launch_peak(kind, blocks, kind == 0 ? 16384 : 4096, out, clk, st);
   else
     hipLaunchKernelGGL(clock_kernel, dim3(1), dim3(64), 0, st, clk, rate_khz);  // 1 ms
   e = hipGetLastError();

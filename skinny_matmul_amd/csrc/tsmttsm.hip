@@ -53,11 +53,26 @@ struct TsArgs {
 
 constexpr int kWaves = 4;
 
-template <int MT, int NT, int P>
+// FP32 tiles of 3 x 3 subtiles and more run the chunk loop over two register
+// sets (TsCore PIPE); measured on MI355X at K = 2^26 against the single set
+// (GB/s, general / Gram): M = N = 64 3869 -> 4411 / 3540 -> 3792, 48 4360 ->
+// 4708 / 3801 -> 4154.
+template <int MT, int NT, int P, int ES>
+struct Pipe {
+  static constexpr bool value = ES == 4 && P == 1 && MT * NT >= 9;
+};
+
+template <int MT, int NT, int P, int ES = 8>
 struct Unroll {
   // independent loads kept in flight per lane: about 16
   static constexpr int L = MT + NT;
-  static constexpr int value = P > 1 ? 8 : (L >= 8 ? 2 : (L >= 4 ? 4 : 8));
+  static constexpr int base = P > 1 ? 8 : (L >= 8 ? 2 : (L >= 4 ? 4 : 8));
+  // an FP32 load carries half the bytes: twice the unroll, except for 16
+  // packed rows (M = N = 1) and the pipelined tiles (same measurement, GB/s:
+  // M = N = 8 6580 -> 6935, 16 6600 -> 7012, 24 4352 -> 4671; Gram 16 5705 ->
+  // 6327, 32 5301 -> 6105; Gram M = N = 1 3762 -> 2942, so not there)
+  static constexpr int value =
+      (ES == 4 && P != 16 && !Pipe<MT, NT, P, ES>::value) ? 2 * base : base;
 };
 
 template <typename T>
@@ -177,7 +192,7 @@ template <typename T, int P, int MT, int NT, int U, bool SELF>
 __global__ void __launch_bounds__(256)
     __attribute__((amdgpu_waves_per_eu((MT * NT >= 9) ? 2 : 1)))
     tsmttsm_kernel(TsArgs<T> a) {
-  using Core = TsCore<T, P, MT, NT, U, SELF>;
+  using Core = TsCore<T, P, MT, NT, U, SELF, Pipe<MT, NT, P, (int)sizeof(T)>::value>;
   using V4 = typename Core::V4;
   constexpr int DM = Core::DM, DN = Core::DN;
   const int wave = threadIdx.x >> 6;
@@ -515,7 +530,7 @@ TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
 template <typename T, int P, int MT, int NT, bool SELF>
 static void launch_one(const TsArgs<T>& a, int blocks, int uvar,
                        hipStream_t s) {
-  constexpr int U0 = Unroll<MT, NT, P>::value;
+  constexpr int U0 = Unroll<MT, NT, P, (int)sizeof(T)>::value;
   if (uvar == 0)
     hipLaunchKernelGGL((tsmttsm_kernel<T, P, MT, NT, U0, SELF>), dim3(blocks),
                        dim3(256), 0, s, a);

@@ -23,7 +23,9 @@ namespace skinny {
 // same order as entry (j, i) of subtile (t, u), so the mirror is bitwise
 // what the full computation gives, with MT (MT + 1) / 2 instead of MT^2
 // matrix-core steps per row group (10 of 16 at 64 columns).
-template <typename T, int P, int MT, int NT, int U, bool SELF>
+// PIPE: the chunk loop runs over two register sets (loads of the next chunk
+// in flight during this chunk's matrix-core steps).
+template <typename T, int P, int MT, int NT, int U, bool SELF, bool PIPE = false>
 struct TsCore {
   using V4 = typename Mfma<T>::v4;
   // the dispatch instantiates every (MT, NT) for SELF; only square ones run
@@ -87,10 +89,9 @@ struct TsCore {
     const int64_t stepB = (int64_t)RS * ldb;
     const int64_t nfull = K / CH;
 
-    for (int64_t c = w0; c < nfull; c += wstride) {
+    auto load = [&](T (&av)[U][MT], T (&bv)[U][NT], int64_t c) {
       const T* pa = Ab + c * CH * lda;
       const T* pb = Bb + c * CH * ldb;
-      T av[U][MT], bv[U][NT];
 #pragma unroll
       for (int s = 0; s < U; s++) {
 #pragma unroll
@@ -102,6 +103,8 @@ struct TsCore {
             bv[s][u] = vb[u] ? ld<STREAM>(&pb[s * stepB + 16 * u]) : T(0);
         }
       }
+    };
+    auto mac = [&](const T (&av)[U][MT], const T (&bv)[U][NT]) {
 #pragma unroll
       for (int s = 0; s < U; s++)
 #pragma unroll
@@ -111,6 +114,28 @@ struct TsCore {
             if (live(t, u))
               acc[t][u] = Mfma<T>::mac(av[s][t], SELF ? av[s][u] : bv[s][u],
                                        acc[t][u]);
+    };
+    if constexpr (PIPE) {
+      // two register sets: the loads of the wave's next chunk (index clamped
+      // to the last full chunk, so they are issued unconditionally) are in
+      // flight while the matrix cores take the current one
+      T a0[U][MT], b0[U][NT], a1[U][MT], b1[U][NT];
+      if (w0 < nfull) {
+        load(a0, b0, w0);
+        for (int64_t c = w0; c < nfull; c += 2 * wstride) {
+          load(a1, b1, c + wstride < nfull ? c + wstride : nfull - 1);
+          mac(a0, b0);
+          if (c + wstride >= nfull) break;
+          load(a0, b0, c + 2 * wstride < nfull ? c + 2 * wstride : nfull - 1);
+          mac(a1, b1);
+        }
+      }
+    } else {
+      for (int64_t c = w0; c < nfull; c += wstride) {
+        T av[U][MT], bv[U][NT];
+        load(av, bv, c);
+        mac(av, bv);
+      }
     }
 
     // ragged tail: the one partial chunk, taken by the wave whose turn it is

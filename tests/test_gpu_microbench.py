@@ -61,3 +61,19 @@ def test_fp64_matrix_core_forms_and_clock(gpu):
         assert 100.0 < mhz < 2400.0 * 1.05, (kind, mhz)
     mhz = gpu.mb_clock("fp64_mfma_4x4x4")
     assert q4 < 78.6 * (mhz / 2400.0) * 1.05, (q4, mhz)
+
+
+def test_fp32_matrix_core_forms(gpu):
+    """FP32 matrix-core peaks (16x16x4, 32x32x2, 16-block 4x4x1): each below
+    the 157.3 TFLOP/s dense datasheet figure and above the FP64 vector rate;
+    the clock read during each loop lies between 100 and 2400 MHz.  (The
+    clock is read by one workgroup at its start and end, which for the
+    32x32x2 loop does not cover the whole run, so the peaks are not scaled
+    by it.)"""
+    fma = gpu.mb_peak("fp64_fma", 1)
+    for kind in ("fp32_mfma", "fp32_mfma_32x32x2", "fp32_mfma_4x4x1"):
+        note(f"microbench: {kind}")
+        tf = gpu.mb_peak(kind, 1)
+        mhz = gpu.mb_clock(kind)
+        assert 100.0 < mhz < 2400.0 * 1.05, (kind, mhz)
+        assert fma < tf < 157.3 * 1.02, (kind, tf, mhz)
