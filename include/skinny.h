@@ -288,7 +288,7 @@ skinny_status_t skinny_fill_int(double* out, int64_t n, uint64_t seed,
                                 int hi, skinny_stream_t stream);
 
 /* ---- autotuner ----------------------------------------------------------- */
-/* Time the unroll variants and grid sizes (0.5 .. 8 workgroups per CU) of
+/* Time the kernel variants and grid sizes (0.5 .. 8 workgroups per CU) of
  * one shape class (op 0: TSMTTSM, 1: TSMM; dtype 0: double, 1: float,
  * 2: complex double, 3: complex float, 4: HR (TSMTTSM only); M, N <= 1024,
  * complex <= 512, complex widths for complex types; K class =
@@ -300,8 +300,10 @@ skinny_status_t skinny_fill_int(double* out, int64_t n, uint64_t seed,
 skinny_status_t skinny_autotune(skinny_handle_t h, int op, int dtype,
                                 int64_t M, int64_t N, int64_t K, int reps,
                                 double* best_ms);
-/* The table as text lines "op dtype M N log2K unroll_variant blockCount";
- * returns the size needed including the terminating NUL. */
+/* The table as text lines "op dtype M N log2K variant blockCount" (TSMTTSM
+ * variant bit 0: doubled unroll, bit 1: partials summed by the last arrivals
+ * of the same launch instead of a second kernel; TSMM: 0); returns the size
+ * needed including the terminating NUL. */
 size_t skinny_tune_export(skinny_handle_t h, char* buf, size_t len);
 skinny_status_t skinny_tune_import(skinny_handle_t h, const char* text);
 void skinny_tune_clear(skinny_handle_t h);

@@ -58,8 +58,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     const int PS = o.outM * o.outN;
     double* part = a.partials + o.part_off;
     run_ts_item(o, it, red, part + (int64_t)it.part * PS);
-    // completion: the item that finishes the op sums its partials in order
+    // completion: the item that finishes the op sums its partials in order.
+    // The partial was written with agent-scope (write-through) stores, so
+    // the arrival waits for them to complete instead of writing the XCD's
+    // L2 back (published(), mfma.hpp)
+#ifdef BATCH_WBL2
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+#else
+    published();
+#endif
     __syncthreads();
     if (tid == 0) {
       unsigned t = __hip_atomic_fetch_add(&a.cnt[o.cnt], 1u, __ATOMIC_RELAXED,

@@ -22,7 +22,7 @@ struct TuneKey {
   }
 };
 struct TuneEntry {
-  int uvar;  // unroll variant (TSMTTSM)
+  int uvar;  // TSMTTSM variant: bit 0 doubled unroll, bit 1 one launch
   int G;     // workgroups along K
 };
 // Device workspace of one stream: [counter region (zeroed at allocation,
@@ -100,6 +100,7 @@ struct TsmttsmPlan {
   int ngroups = 1;
   int outM = 1, outN = 1;  // entries of one partial (outM x outN)
   int uvar = 0;            // unroll variant: 0 base, 1 doubled
+  bool one_pass = false;   // G > 1 partials summed in the same launch
   bool tuned = false;      // G and uvar come from the autotuner table
   bool fixedG = false;     // G given by the caller or the autotuner table
   size_t scratch_bytes = 0;

@@ -22,6 +22,22 @@ __device__ __forceinline__ __attribute__((address_space(3))) T* lmem(T* p) {
   return (__attribute__((address_space(3))) T*)p;
 }
 
+// Hand-over of workgroup partials to the workgroup that sums them, within one
+// launch.  Each XCD has its own L2, and a release at agent scope writes the
+// XCD's L2 back (buffer_wbl2), which costs microseconds per workgroup when
+// hundreds of them finish together.  Partials are instead written with
+// agent-scope stores (global_store ... sc1), which write through the L2, so
+// the only thing an arrival must ensure is that its stores have completed:
+// published() waits for them (s_waitcnt vmcnt(0), which is also the second
+// half of the release sequence) and keeps the compiler from moving memory
+// accesses across it.  The summing workgroup still takes the acquire side
+// (an L2 invalidate) before it reads.
+template <typename T>
+__device__ __forceinline__ void st_agent(T* p, T v) {
+  __hip_atomic_store(gmem(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void published() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 // Streaming load with the nontemporal hint (global_load ... nt): inputs that
 // are read once do not displace cached data.  On MI355X a read stream with
 // the hint runs at about 7.1 TB/s against 6.4 TB/s without it.

@@ -11,7 +11,8 @@ namespace skinny {
 // takes partials j, j + TPE, ...; the team is then combined by a fixed LDS
 // tree), so a small output pays one round trip instead of `count`.  The order
 // depends only on (PS, count): results are reproducible.  The sums go to
-// dst (group level) or to `fin` in LDS (final level, PS <= ldsn / 2).  The
+// dst (group level; agent-scope stores, read by another workgroup of the
+// launch) or to `fin` in LDS (final level, PS <= ldsn / 2).  The
 // LDS area holds ldsn elements: the first blockDim.x are the tree.
 template <typename T>
 __device__ inline void team_sum(const T* src, int count, int PS, T* dst, T* lds,
@@ -57,7 +58,8 @@ __device__ inline void team_sum(const T* src, int count, int PS, T* dst, T* lds,
       if (to_lds)
         fin[e] = tree[tid];
       else
-        dst[e] = tree[tid];
+        __hip_atomic_store((__attribute__((address_space(1))) T*)(dst + e), tree[tid],
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
   }

@@ -17,14 +17,17 @@
 //    larger C is split into 64 x 64 tiles along the grid.
 //  * The four waves of a workgroup stream disjoint rows and are summed
 //    through LDS in a fixed pairwise order; each workgroup writes one partial.
-//  * With G > 1 workgroups per tile a second kernel sums the G partials of
-//    every entry and applies the alpha/beta epilogue (the two stages of
-//    genv3.cuh:9-27 and 31-80).  Handing the partials over at the kernel
-//    boundary costs one cache write-back for the whole grid; an in-kernel
-//    hand-over (release at device scope, then an arrival counter) writes
-//    back the XCD's L2 once per workgroup, which measured 40-60 us per call
-//    at 256-512 workgroups on MI355X.  Summation order is fixed, so the
-//    result is bitwise reproducible for a fixed blockCount.
+//  * With G > 1 workgroups per tile the G partials of every entry are summed
+//    and the alpha/beta epilogue applied (the two stages of genv3.cuh:9-27
+//    and 31-80) either by a second kernel (default) or, in the same launch,
+//    by the last arrival of each group and then the last group (grid_finish;
+//    an autotuner choice).  The in-launch hand-over writes the partials with
+//    agent-scope stores and waits for them before arriving, with no L2
+//    write-back (a release at agent scope per workgroup had cost 40-60 us per
+//    call at 256-512 workgroups).  Measured without it (FP64, MI355X): equal
+//    to the two launches within 1.5% at K = 2^26, 2-3 us slower at K = 2^22.
+//    Summation order is fixed, so the result is bitwise reproducible for a
+//    fixed blockCount and form.
 //  * With G == 1 the one workgroup of a tile applies the epilogue itself.
 #include "internal.hpp"
 #include "mfma.hpp"
@@ -133,21 +136,22 @@ __device__ void final_epilogue(const TsArgs<T>& a, const T* fin, int m0,
 }
 
 // Grid reduction after a workgroup wrote its partial (single-launch form,
-// used with G == 1 and by callers that keep two_pass off): the last arrival
-// of each group sums the group, then the last group sums the group sums and
-// applies the epilogue.  `red` is the kernel's LDS of LDSN elements.
+// used with G == 1 and when the autotuner chose one launch): the last
+// arrival of each group sums the group, then the last group sums the group
+// sums and applies the epilogue.  `red` is the kernel's LDS of LDSN
+// elements.
 template <typename T, int LDSN>
 __device__ __forceinline__ void grid_finish(const TsArgs<T>& a, T* red, int tile, int kp,
                                             int ntiles, int m0, int n0, int Mloc,
                                             int Nloc) {
   const int PS = a.outM * a.outN;
   // ---- grid reduction: last arrival per group, then last group ----------
-  // Arrival publishes this workgroup's partial with a release fence at agent
-  // scope (L2 write-back, no invalidate); only the last arrival takes the
-  // acquire side (L2 invalidate) before it reads the other partials.  A full
-  // __threadfence() would invalidate the XCD's L2 in every workgroup.
+  // The partials were written with agent-scope stores (write-through); an
+  // arrival waits for its stores to complete (published(), no L2 write-back)
+  // before it counts itself, and only the last arrival takes the acquire
+  // side (L2 invalidate) before it reads the other partials.
   __shared__ int s_last;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  published();
   __syncthreads();
   const int grp = kp / a.GS;
   const int gsz = min(a.GS, a.G - grp * a.GS);
@@ -172,7 +176,7 @@ __device__ __forceinline__ void grid_finish(const TsArgs<T>& a, T* red, int tile
   T* gdst = a.part2 + ((int64_t)tile * a.ngroups + grp) * PS;
   team_sum<T>(gsrc, gsz, PS, gdst, red, LDSN, false);
   if (threadIdx.x == 0) *cnt1 = 0u;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  published();
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned ticket = __hip_atomic_fetch_add(cnt2, 1u, __ATOMIC_RELAXED,
@@ -491,7 +495,8 @@ TsmttsmPlan plan_tsmttsm(skinny_handle_t h, size_t blockCount, int64_t M,
              tune_lookup(h, 0, tdtype >= 0 ? tdtype : (elem_bytes == 8 ? TT_DR : TT_FR),
                          tdtype >= 0 ? tM : M, tdtype >= 0 ? tN : N, K, &te)) {
     G = te.G;
-    p.uvar = te.uvar;
+    p.uvar = te.uvar & 1;
+    p.one_pass = (te.uvar & 2) != 0;
     p.tuned = true;
     p.fixedG = true;
   } else {
@@ -757,7 +762,9 @@ skinny_status_t launch_tsmttsm(skinny_handle_t h, const TsmttsmPlan& p,
   // on the diagonal; off-diagonal tiles of a large Gram read two column blocks
   const bool self = (A == B) && (lda == ldb) && (M == N) && ntiles == 1;
   const int blocks = p.G * ntiles;
-  a.two_pass = p.G > 1 ? 1 : 0;
+  // G > 1: a second kernel sums the partials, or (one_pass, an autotuner
+  // choice) the last arrivals of the same launch do (grid_finish)
+  a.two_pass = (p.G > 1 && !p.one_pass) ? 1 : 0;
   // a Gram product loads one operand per step, half the loads in flight of
   // the general product at the same unroll: untuned, it takes the doubled
   // unroll (M = 24 at K = 2^22: 3.4 TB/s with the base unroll)

@@ -166,7 +166,8 @@ struct TsCore {
   }
 
   // Workgroup sum of the four waves' accumulators, written as the outM x outN
-  // partial `out` (entry e = (e % outM, e / outM)).  `red` is LDS of
+  // partial `out` (entry e = (e % outM, e / outM)) with agent-scope stores
+  // (st_agent: another workgroup of the same launch may sum it).  `red` is LDS of
   // RED_ELEMS elements.  Ends with a workgroup barrier after the writes.
   static __device__ __forceinline__ void wg_partial(V4 (&acc)[MT][NT], T* red,
                                                     T* out, int M, int N,
@@ -249,7 +250,7 @@ struct TsCore {
       } else {
         s = Dm[ml * DN + nl];
       }
-      *gmem(out + e) = s;
+      st_agent(out + e, s);
     }
     __syncthreads();
   }

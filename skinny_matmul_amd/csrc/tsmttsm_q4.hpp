@@ -217,7 +217,7 @@ struct TsQ4 {
     const int PS = outM * outN;
     for (int e = threadIdx.x; e < PS; e += blockDim.x) {
       const int ml = e % outM, nl = e / outM;
-      *gmem(out + e) = Dm[ml * 64 + nl];
+      st_agent(out + e, Dm[ml * 64 + nl]);
     }
     __syncthreads();
   }

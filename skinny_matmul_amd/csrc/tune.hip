@@ -165,8 +165,9 @@ skinny_status_t skinny_autotune(skinny_handle_t h, int op, int dtype,
   std::vector<Cand> cands;
   if (s == SKINNY_OK) {
     const double mults[] = {0.5, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0, 8.0};
-    // unroll variants exist for the FP64 / FP32 / complex TSMTTSM kernels
-    const int nuvar = (op == 0 && dtype != TT_HR) ? 2 : 1;
+    // variants of the FP64 / FP32 / complex TSMTTSM kernels: bit 0 the
+    // doubled unroll, bit 1 the partials summed in the same launch
+    const int nuvar = (op == 0 && dtype != TT_HR) ? 4 : 1;
     for (int uvar = 0; uvar < nuvar && s == SKINNY_OK; uvar++) {
       for (double mu : mults) {
         const TuneEntry cand{uvar, std::max(1, (int)(mu * h->cu_count))};
@@ -252,7 +253,7 @@ skinny_status_t skinny_tune_import(skinny_handle_t h, const char* text) {
   int op, dtype, M, N, kc, uvar, G;
   std::lock_guard<std::mutex> lock(h->tune_mu);
   while (is >> op >> dtype >> M >> N >> kc >> uvar >> G) {
-    if (G < 1 || uvar < 0 || uvar > 1)
+    if (G < 1 || uvar < 0 || uvar > 3)
       return fail(SKINNY_INVALID_ARG, "tune import: bad entry");
     h->tune[TuneKey{op, dtype, M, N, kc}] = TuneEntry{uvar, G};
   }

@@ -120,3 +120,49 @@ def test_autotune_every_type(gpu, orc):
     err = np.sqrt(np.sum(((q[:, 0] - hi) + (q[:, 1] - lo)) ** 2)) / np.sqrt(np.sum(hi ** 2))
     assert err <= 1e-28, err
     h.close()
+
+
+@pytest.mark.parametrize("dt,M,N,gram", [("d", 1, 1, False), ("d", 4, 4, True), ("d", 8, 3, False),
+                                         ("d", 48, 48, False), ("d", 64, 64, False),
+                                         ("d", 64, 64, True), ("d", 100, 70, False),
+                                         ("s", 16, 16, False), ("s", 64, 64, True),
+                                         ("z", 8, 8, False)])
+def test_one_launch_variant_exact(gpu, orc, dt, M, N, gram):
+    """Variant bit 1 (the partials of G > 1 workgroups summed by the last
+    arrivals of the same launch: one group, or groups and then the last
+    group) gives the oracle's result bitwise on integer data, as the
+    two-kernel form does, for several grids."""
+    import torch
+    K = 123457
+    log2K = K.bit_length() - 1
+    dtype = {"d": 0, "s": 1, "z": 2}[dt]
+    seed = M * 131 + N
+    A = orc.integers(K * M * (2 if dt == "z" else 1), seed, 0)
+    B = A if gram else orc.integers(K * N * (2 if dt == "z" else 1), seed, 1)
+    if dt == "z":
+        Ac, Bc = A.view(np.complex128), B.view(np.complex128)
+        want = np.zeros(M * N, dtype=np.complex128)
+        orc.tsmttsm(M, N, K, Ac, M, 1.0, Bc, N, 0.0, want, M)
+    else:
+        want = np.zeros(M * N)
+        orc.tsmttsm(M, N, K, A, M, 1.0, B, N, 0.0, want, M)
+    tdev = torch.float32 if dt == "s" else torch.float64
+    dA = torch.from_numpy(A).to(tdev).cuda()
+    dB = dA if gram else torch.from_numpy(B).to(tdev).cuda()
+    h = gpu.Handle()
+    for variant in (0, 2, 3):
+        for G in (7, 100, 600):
+            note(f"one launch: {dt} M={M} N={N} gram={gram} variant={variant} G={G}")
+            gpu.tune_clear(handle=h)
+            gpu.tune_import(f"0 {dtype} {M} {N} {log2K} {variant} {G}\n", handle=h)
+            dC = torch.zeros(M * N * (2 if dt == "z" else 1), dtype=tdev, device="cuda")
+            if dt == "d":
+                assert gpu.dtsmttsm(0, M, N, K, dA, M, 1.0, dB, N, 0.0, dC, M, handle=h)
+            elif dt == "s":
+                assert gpu.stsmttsm(0, M, N, K, dA, M, 1.0, dB, N, 0.0, dC, M, handle=h)
+            else:
+                assert gpu.ztsmttsm(0, M, N, K, dA, M, 1.0, dB, N, 0.0, dC, M, handle=h)
+            got = host(dC).astype(np.float64)
+            if dt == "z":
+                got = got.view(np.complex128)
+            assert np.array_equal(got, want), (variant, G)
