@@ -40,7 +40,8 @@ def test_headline_table_from_trace(tmp_path, capsys, monkeypatch):
         for op, M in ops:
             if op == "tsmttsm":
                 seq.append((f"tsmttsm_kernel<double, {M}>", 1000 * M + sweep))
-                seq.append(("tsmttsm_reduce_kernel<double>", 7))
+                if M != 8:  # M = 8: partials summed in the same launch
+                    seq.append(("tsmttsm_reduce_kernel<double>", 7))
             else:
                 seq.append((f"tsmm_mfma_kernel<double, {M}>", 2000 * M))
     rows = _rows(seq)
@@ -57,5 +58,8 @@ def test_headline_table_from_trace(tmp_path, capsys, monkeypatch):
     line = next(l for l in out.splitlines() if l.startswith("| tsmttsm M=N=64 "))
     assert "| 0.064 | 0.0000 | 0.064 |" in line
     assert line.split("|")[7].strip() == str(2 * 64 * 1024 * 8)
+    # a one-launch call: no partial-sum kernel follows it
+    line = next(l for l in out.splitlines() if l.startswith("| tsmttsm M=N=8 "))
+    assert "| 0.008 | 0.0000 | 0.008 |" in line
     line = next(l for l in out.splitlines() if l.startswith("| tsmm M=N=4 "))
     assert line.split("|")[7].strip() == str((2 * 4 * 1024 + 16) * 8)
