@@ -193,6 +193,146 @@ __global__ void __launch_bounds__(64 * NW) tsmm_q4_kernel(TqArgs a) {
   }
 }
 
+// ---- complex double with the three-product form ---------------------------
+// C = alpha A B + beta C for complex A (K x Mc), B (Mc x Nc) as three real
+// products (the 3M form): T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi),
+// then Re = T1 - T2, Im = T3 - T1 - T2.  Against the 2Mc x 2Nc real view
+// (four real products per complex one) this is 3/4 of the matrix-core steps
+// and of the LDS operand reads, which bound the real view from about 40
+// complex columns on.  The sums Ar + Ai and Br + Bi are rounded, so the
+// imaginary part's error is bounded by eps (|Ar| + |Ai|)(|Br| + |Bi|) terms
+// rather than the four-product form's; on integer data both are exact.
+//
+// Mapping as tsmm_q4_kernel, with complex slices: lane (X, b, y) supplies
+// A_j(row0 + 4 b + y, X S2 + s) at step (j, s) (j = 0: Ar, 1: Ai, 2: Ar + Ai;
+// the lane owns complex columns X S2 .. X S2 + S2 - 1 of one row, loaded
+// 16 bytes at a time), and B_j(X S2 + s, 4 q + y) from LDS (B_0 = Br, B_1 =
+// Bi, B_2 = Br + Bi, each laid out as tsmm_q4_kernel's B^ with QT + 2
+// doubles per row).  Lane (X, b, y) holds T_j(row0 + 4 b + X, 4 q + y) for
+// the three j and combines them itself: one 16-byte store of the complex
+// entry per column group.
+template <int S2, int QC, int NW, bool BETA0>
+__global__ void __launch_bounds__(64 * NW) tsmm_q4c3_kernel(TqArgs a) {
+  extern __shared__ char smem_raw[];
+  double* bl = reinterpret_cast<double*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int X = lane >> 4, blk = (lane >> 2) & 3, y = lane & 3;
+  const int Mc = a.M, Nc = a.N, QT = a.QT;  // complex widths here
+  const int R = QT + 2;                     // LDS row stride (doubles)
+  const int MR = 16 * S2;                   // LDS rows of one B_j
+
+  // ---- stage B_0, B_1, B_2 in LDS -----------------------------------------
+  for (int idx = tid; idx < MR * QT; idx += 64 * NW) {
+    const int r = idx / QT, q = idx - r * QT;
+    const int s = r >> 4, xx = (r >> 2) & 3, yy = r & 3;
+    const int m = xx * S2 + s, n = 4 * q + yy;
+    double br = 0.0, bi = 0.0;
+    if (m < Mc && n < Nc) {
+      const double* b = a.B + 2 * ((int64_t)n * a.ldb + m);
+      br = b[0];
+      bi = b[1];
+    }
+    bl[r * R + q] = br;
+    bl[(MR + r) * R + q] = bi;
+    bl[(2 * MR + r) * R + q] = br + bi;
+  }
+  for (int idx = tid; idx < 3 * MR; idx += 64 * NW) {
+    bl[idx * R + QT] = 0.0;
+    bl[idx * R + QT + 1] = 0.0;
+  }
+  __syncthreads();
+
+  const int64_t K = a.K;
+  const int64_t gw = (int64_t)blockIdx.x * NW + (tid >> 6);
+  const int64_t nw = (int64_t)gridDim.x * NW;
+  const int64_t nblk = (K + 15) >> 4;
+  const int mbeg = X * S2;  // first complex column of the lane's slice
+  const double* brow = bl + (X * 4 + y) * R;  // row of step 0 in B_0
+  constexpr int RS = 16;                      // LDS rows between consecutive steps
+
+  for (int64_t g = gw; g < nblk; g += nw) {
+    const int64_t row = g * 16 + 4 * blk + y;
+    const bool ok = row < K;
+    const double* ar = a.A + 2 * (row * a.lda + mbeg);
+    double xr[S2], xi[S2];
+#pragma unroll
+    for (int s = 0; s < S2; s++) {
+      if (a.vec) {
+        const tq2 v = ok ? *gmem(reinterpret_cast<const tq2*>(ar + 2 * s)) : tq2{0.0, 0.0};
+        xr[s] = v.x;
+        xi[s] = v.y;
+      } else {
+        const bool v = ok && mbeg + s < Mc;
+        xr[s] = v ? *gmem(ar + 2 * s) : 0.0;
+        xi[s] = v ? *gmem(ar + 2 * s + 1) : 0.0;
+      }
+    }
+    const int64_t orow = g * 16 + 4 * blk + X;  // this lane's output row
+    const bool rok = orow < K;
+    double* crow = a.C + 2 * (orow * a.ldc);
+    for (int q0 = 0; q0 < QT; q0 += QC) {
+      tq2 cold[BETA0 ? 1 : QC];
+      if (!BETA0) {
+#pragma unroll
+        for (int q = 0; q < QC; q++) {
+          const int n = 4 * (q0 + q) + y;
+          cold[BETA0 ? 0 : q] =
+              (rok && n < Nc) ? *gmem(reinterpret_cast<const tq2*>(crow + 2 * n)) : tq2{0.0, 0.0};
+        }
+      }
+      double t1[QC], t2[QC], t3[QC];
+#pragma unroll
+      for (int q = 0; q < QC; q++) t1[q] = t2[q] = t3[q] = 0.0;
+#pragma unroll
+      for (int s = 0; s < S2; s++) {
+        const double* bp = brow + s * RS * R + q0;
+        const double xs = xr[s] + xi[s];
+#pragma unroll
+        for (int p = 0; p < QC; p += 2) {
+          const tq2 b0 = *lmem(reinterpret_cast<const tq2*>(bp + p));
+          const tq2 b1 = *lmem(reinterpret_cast<const tq2*>(bp + MR * R + p));
+          const tq2 b2 = *lmem(reinterpret_cast<const tq2*>(bp + 2 * MR * R + p));
+          t1[p] = q4mac(xr[s], b0.x, t1[p]);
+          t1[p + 1] = q4mac(xr[s], b0.y, t1[p + 1]);
+          t2[p] = q4mac(xi[s], b1.x, t2[p]);
+          t2[p + 1] = q4mac(xi[s], b1.y, t2[p + 1]);
+          t3[p] = q4mac(xs, b2.x, t3[p]);
+          t3[p + 1] = q4mac(xs, b2.y, t3[p + 1]);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < QC; q++) {
+        const int n = 4 * (q0 + q) + y;
+        if (rok && n < Nc) {
+#pragma clang fp contract(off)
+          const double re = t1[q] - t2[q];
+          const double im = (t3[q] - t1[q]) - t2[q];
+          double o_re = a.alpha * re - a.alpha_i * im;
+          double o_im = a.alpha * im + a.alpha_i * re;
+          if (!BETA0) {
+            const tq2 c = cold[BETA0 ? 0 : q];
+            o_re = (a.beta * c.x - a.beta_i * c.y) + o_re;
+            o_im = (a.beta * c.y + a.beta_i * c.x) + o_im;
+          }
+          *gmem(reinterpret_cast<tq2*>(crow + 2 * n)) = tq2{o_re, o_im};
+        }
+      }
+    }
+  }
+}
+
+template <int S2, int QC, int NW>
+static hipError_t launch_q4c3_s(const TqArgs& a, int blocks, size_t lds, bool beta0,
+                                hipStream_t st) {
+  auto k = beta0 ? tsmm_q4c3_kernel<S2, QC, NW, true> : tsmm_q4c3_kernel<S2, QC, NW, false>;
+  hipError_t e = hipFuncSetAttribute((const void*)k,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * NW), lds, st, a);
+  return hipGetLastError();
+}
+
 template <int S, int QC, int RB, int NW, bool CPLX>
 static hipError_t launch_q4_s(const TqArgs& a, int blocks, size_t lds, bool beta0,
                               hipStream_t st) {
@@ -232,6 +372,58 @@ skinny_status_t launch_tsmm_q4(skinny_handle_t h, size_t blockCount, int64_t M, 
                                double beta_i, double* C, int64_t ldc, bool cplx,
                                hipStream_t stream) {
   if (M < 29 || M > 128 || N < 1 || N > 128) return SKINNY_UNSUPPORTED;
+  // complex with 16-byte aligned A and C: the three-product form (same box,
+  // K = 2^26, against the real view: M = N = 64 45.5 -> 35.9 ms, 48 28.3 ->
+  // 26.9, 40 27.2 -> 25.2; M = 64, N = 8 20.9 -> 17.7; M = 33, N = 64 33.5 ->
+  // 27.8)
+  if (cplx && (uintptr_t)A % 16 == 0 && (uintptr_t)C % 16 == 0) {
+    const int Mc = (int)(M / 2), Nc = (int)(N / 2);
+    const int S2 = Mc <= 32 ? 8 : Mc <= 48 ? 12 : 16;
+    constexpr int QC3 = 8;
+    const int QT3 = ((Nc + 3) / 4 + QC3 - 1) / QC3 * QC3;
+    TqArgs c;
+    c.A = A;
+    c.B = B;
+    c.C = C;
+    c.K = K;
+    c.lda = lda / 2;  // complex units
+    c.ldb = ldb;
+    c.ldc = ldc / 2;
+    c.M = Mc;
+    c.N = Nc;
+    c.QT = QT3;
+    c.vec = 1;
+    c.alpha = alpha;
+    c.beta = beta;
+    c.alpha_i = alpha_i;
+    c.beta_i = beta_i;
+    const size_t lds = (size_t)3 * 16 * S2 * (QT3 + 2) * sizeof(double);
+    const bool big = lds > 80 * 1024;
+    int blocks;
+    TuneEntry te;
+    if (blockCount > 0) {
+      blocks = (int)blockCount;
+    } else if (tune_lookup(h, 1, TT_DC, Mc, Nc, K, &te)) {
+      blocks = te.G;
+    } else {
+      const int per_cu =
+          big ? 1 : (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / lds));
+      const int64_t want = (K + 16 * 4 - 1) / (16 * 4);
+      blocks = (int)std::max<int64_t>(1, std::min(want, (int64_t)per_cu * h->cu_count));
+    }
+    const bool beta0 = beta == 0.0 && beta_i == 0.0;
+    hipError_t e;
+    switch (S2) {
+      case 8: e = big ? launch_q4c3_s<8, QC3, 8>(c, blocks, lds, beta0, stream)
+                      : launch_q4c3_s<8, QC3, 4>(c, blocks, lds, beta0, stream); break;
+      case 12: e = big ? launch_q4c3_s<12, QC3, 8>(c, blocks, lds, beta0, stream)
+                       : launch_q4c3_s<12, QC3, 4>(c, blocks, lds, beta0, stream); break;
+      default: e = big ? launch_q4c3_s<16, QC3, 8>(c, blocks, lds, beta0, stream)
+                       : launch_q4c3_s<16, QC3, 4>(c, blocks, lds, beta0, stream); break;
+    }
+    if (e != hipSuccess) return hip_fail(e, "tsmm (complex, 3M) launch");
+    return SKINNY_OK;
+  }
   const int S0 = (int)((M + 3) / 4);
   const int S = S0 <= 8 ? 8 : S0 <= 12 ? 12 : S0 <= 16 ? 16 : S0 <= 24 ? 24 : 32;
   const int QC = (cplx && S == 32) ? 16 : 8;

@@ -184,3 +184,43 @@ def test_real_tsmm_wide(gpu, orc, M, N):
         dA = dC if inplace else dev(A)
         assert gpu.dtsmm(0, M, N, K, dA, lda, 1.0, dev(B), ldb, beta, dC, ldc)
         assert np.array_equal(host(dC), want)
+
+
+@pytest.mark.parametrize("offA,offC", [(0, 0), (8, 0), (0, 8), (8, 8)])
+def test_ztsmm_wide_alignment(gpu, orc, offA, offC):
+    """Complex TSMM above 32 columns: the three-product kernel with A and C
+    16-byte aligned, the real-view kernel when either sits at an 8-byte
+    offset; both give the oracle's result bitwise on integer data, out of
+    place and in place, beta 0 and complex."""
+    import torch
+    rng = random.Random(900 + offA + 2 * offC)
+    for M, N, inplace, beta in ((48, 48, False, 0.0), (64, 33, False, complex(1, -1)),
+                                (40, 40, True, complex(-1, 1)), (64, 64, True, 0.0)):
+        K = rng.randint(2000, 9000)
+        seed = rng.randint(0, 1 << 30)
+        ldb = M
+        ldc = max(M, N) if inplace else N
+        lda = ldc if inplace else M
+        note(f"ztsmm alignment offA={offA} offC={offC} M={M} N={N} K={K} inplace={inplace}")
+        B = cint(orc, N * ldb, seed, 1)
+        C = cint(orc, K * ldc, seed, 2)
+        A = C.copy() if inplace else cint(orc, K * lda, seed, 0)
+        want = C.copy()
+        orc.tsmm(M, N, K, want if inplace else A, lda, complex(2, 1), B, ldb, beta, want, ldc)
+        # complex buffers at byte offsets inside float64 device buffers
+        bufC = torch.zeros(2 * C.size + 2, dtype=torch.float64, device="cuda")
+        oc = offC // 8
+        bufC[oc:oc + 2 * C.size] = torch.from_numpy(C.view(np.float64)).cuda()
+        if inplace:
+            if offA != offC:
+                continue
+            addrA = addrC = bufC.data_ptr() + offC
+        else:
+            bufA = torch.zeros(2 * A.size + 2, dtype=torch.float64, device="cuda")
+            oa = offA // 8
+            bufA[oa:oa + 2 * A.size] = torch.from_numpy(A.view(np.float64)).cuda()
+            addrA, addrC = bufA.data_ptr() + offA, bufC.data_ptr() + offC
+        dB = dev(B)
+        assert gpu.ztsmm(0, M, N, K, addrA, lda, complex(2, 1), dB, ldb, beta, addrC, ldc)
+        got = host(bufC[oc:oc + 2 * C.size]).view(np.complex128)
+        assert np.array_equal(got, want), (M, N, inplace)
