@@ -255,18 +255,16 @@ __global__ void __launch_bounds__(64 * NW) tsmm_q4c3_kernel(TqArgs a) {
     const int64_t row = g * 16 + 4 * blk + y;
     const bool ok = row < K;
     const double* ar = a.A + 2 * (row * a.lda + mbeg);
+    // one 16-byte load per complex element (A is 16-byte aligned), masked to
+    // the row's Mc columns: a slice may extend past them (Mc < 4 S2), and on
+    // the last rows such a read would leave the matrix
     double xr[S2], xi[S2];
 #pragma unroll
     for (int s = 0; s < S2; s++) {
-      if (a.vec) {
-        const tq2 v = ok ? *gmem(reinterpret_cast<const tq2*>(ar + 2 * s)) : tq2{0.0, 0.0};
-        xr[s] = v.x;
-        xi[s] = v.y;
-      } else {
-        const bool v = ok && mbeg + s < Mc;
-        xr[s] = v ? *gmem(ar + 2 * s) : 0.0;
-        xi[s] = v ? *gmem(ar + 2 * s + 1) : 0.0;
-      }
+      const bool v = ok && mbeg + s < Mc;
+      const tq2 e = v ? *gmem(reinterpret_cast<const tq2*>(ar + 2 * s)) : tq2{0.0, 0.0};
+      xr[s] = e.x;
+      xi[s] = e.y;
     }
     const int64_t orow = g * 16 + 4 * blk + X;  // this lane's output row
     const bool rok = orow < K;
@@ -392,7 +390,7 @@ skinny_status_t launch_tsmm_q4(skinny_handle_t h, size_t blockCount, int64_t M, 
     c.M = Mc;
     c.N = Nc;
     c.QT = QT3;
-    c.vec = 1;
+    c.vec = 0;  // unused: the kernel masks every load by the column bound
     c.alpha = alpha;
     c.beta = beta;
     c.alpha_i = alpha_i;

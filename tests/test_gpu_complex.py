@@ -224,3 +224,29 @@ def test_ztsmm_wide_alignment(gpu, orc, offA, offC):
         assert gpu.ztsmm(0, M, N, K, addrA, lda, complex(2, 1), dB, ldb, beta, addrC, ldc)
         got = host(bufC[oc:oc + 2 * C.size]).view(np.complex128)
         assert np.array_equal(got, want), (M, N, inplace)
+
+
+@pytest.mark.parametrize("M,lda", [(33, 48), (40, 48), (49, 64), (20, 32)])
+def test_ztsmm_reads_only_m_columns(gpu, orc, M, lda):
+    """The wide complex TSMM reads only the first M columns of each row of
+    A: the columns M .. lda - 1 hold NaN, which any read of them would carry
+    into C (the kernels' A slices can reach past M when M is not a multiple
+    of their slice width).  Integer data, bitwise against the oracle, beta 0
+    and complex."""
+    rng = random.Random(77 + M)
+    N = 40
+    for beta in (0.0, complex(1, -2)):
+        K = rng.randint(3000, 7000)
+        seed = rng.randint(0, 1 << 30)
+        note(f"ztsmm NaN padding M={M} lda={lda} N={N} K={K} beta={beta}")
+        A = cint(orc, K * lda, seed, 0).reshape(K, lda)
+        A[:, M:] = complex(np.nan, np.nan)
+        A = A.ravel()
+        B = cint(orc, N * M, seed, 1)
+        C = cint(orc, K * N, seed, 2)
+        want = C.copy()
+        orc.tsmm(M, N, K, A, lda, complex(2, 1), B, M, beta, want, N)
+        assert not np.isnan(want).any()
+        dC = dev(C)
+        assert gpu.ztsmm(0, M, N, K, dev(A), lda, complex(2, 1), dev(B), M, beta, dC, N)
+        assert np.array_equal(host(dC), want), (M, lda, beta)
