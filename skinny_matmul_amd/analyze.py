@@ -21,7 +21,9 @@ the same table and columns (perf.cu:286-307):
 The ceilings are this chip's measured ones (`python -m
 skinny_matmul_amd.ceilings --json FILE`): a TSMTTSM row is a read stream
 (read_GBs), a TSMM row reads and writes (copy_GBs); FP64 types use the
-better of the FP64 vector and matrix-core peaks.  Both can be overridden.
+best of the FP64 vector and matrix-core (16x16x4 and 4x4x4) peaks, FP32
+types the best measured FP32 matrix-core peak (the 157.3 TFLOP/s datasheet
+figure when none was measured).  Both can be overridden.
 
 Each configuration (multype, device, types, name, inplace, zerobeta) found
 in the databases gets its maps plus a line in DIR/index.md with the
@@ -131,14 +133,20 @@ def configurations(conns, branch=None) -> list:
     return sorted(seen, key=lambda t: tuple(str(x) for x in t))
 
 
+# the ceilings.py keys whose best value is the compute peak of a type set
+FP64_PEAKS = ("fp64_fma", "fp64_mfma", "fp64_mfma_4x4x4")
+FP32_PEAKS = ("fp32_mfma", "fp32_mfma_32x32x2", "fp32_mfma_4x4x1")
+
+
 def ceilings_for(multype: str, types: str, ceil: dict) -> tuple:
     """(stream GB/s, peak GFlop/s) for one configuration from measured
     ceilings (ceilings.py's JSON)."""
     bw = ceil.get("read_GBs" if multype.upper() == "TSMTTSM" else "copy_GBs")
     if types in ("DR", "DC", "HR"):
-        tf = max(ceil.get("fp64_fma", 0.0), ceil.get("fp64_mfma", 0.0))
+        tf = max(ceil.get(k, 0.0) for k in FP64_PEAKS)
     else:
-        tf = ceil.get("fp32_peak", 157.3)  # datasheet FP32 matrix peak, TFLOP/s
+        # the measured FP32 matrix-core peaks, else the datasheet figure
+        tf = max((ceil.get(k, 0.0) for k in FP32_PEAKS), default=0.0) or 157.3
     return float(bw), float(tf) * 1000.0
 
 

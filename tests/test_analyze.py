@@ -103,6 +103,11 @@ def test_ceilings_by_operation():
     c = {"read_GBs": 7000.0, "copy_GBs": 5500.0, "fp64_fma": 59.0, "fp64_mfma": 47.0}
     assert an.ceilings_for("TSMTTSM", "DR", c) == (7000.0, 59000.0)
     assert an.ceilings_for("TSMM", "DC", c) == (5500.0, 59000.0)
+    assert an.ceilings_for("TSMM", "FR", c) == (5500.0, 157300.0)  # datasheet FP32
+    # the 4x4x4 FP64 form and the measured FP32 matrix-core peaks take part
+    c.update({"fp64_mfma_4x4x4": 67.0, "fp32_mfma": 126.0, "fp32_mfma_4x4x1": 131.0})
+    assert an.ceilings_for("TSMTTSM", "HR", c) == (7000.0, 67000.0)
+    assert an.ceilings_for("TSMTTSM", "FC", c) == (7000.0, 131000.0)
 
 
 def test_cli_writes_maps_and_index(tmp_path):
