@@ -163,6 +163,240 @@ __global__ void __launch_bounds__(256) tsmm_lds_kernel(TlArgs<T> a) {
   }
 }
 
+// ---- complex float with the three-product form ---------------------------
+// T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi); Re = T1 - T2, Im =
+// (T3 - T1) - T2: 3/4 of the real view's matrix-core steps and LDS operand
+// reads (the real view's 2Mc x 2Nc operand holds each complex B twice).
+// Lane (i, g) owns the complex slice g S2 .. g S2 + S2 - 1 of row i of a
+// 16-row block (8-byte loads of (re, im), masked to the row's Mc columns),
+// forms Ar + Ai in registers, and reads B_j(g S2 + s, 16 u + i) of the three
+// staged matrices Br, Bi, Br + Bi (rows of ns floats, zero outside Mc x Nc).
+// Element v of its tiles T_j[u] is entry (4 g + v, 16 u + i) for all three
+// j, so the lane combines them, applies the complex alpha and beta and
+// stores the complex entry with one 8-byte store.  A wave owns whole
+// 16-row blocks (all their loads precede their stores), so C may alias A.
+// The sums Ar + Ai and Br + Bi are rounded: the imaginary part's error is
+// bounded by eps (|Ar| + |Ai|)(|Br| + |Bi|) terms; integer data is exact.
+template <int S2, int NT, bool BETA0>
+__global__ void __launch_bounds__(256) ctsmm3_kernel(TlArgs<float> a) {
+  using V4 = Mfma<float>::v4;
+  // software pipeline (below) where it measured faster: always with beta != 0
+  // (C's loads hidden behind the matrix cores: Mc = Nc = 64, 2.0 -> 3.6 TB/s)
+  // and at 64 x 64 (one wave per SIMD either way); with beta = 0 and room for
+  // two waves per SIMD their interleaving does better (Mc = Nc = 48: 4.1
+  // against 3.6 TB/s)
+  constexpr bool PF = !BETA0 || (S2 >= 16 && NT >= 4);
+  extern __shared__ char smem_raw[];
+  float* bs = reinterpret_cast<float*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int li = lane & 15, lg = lane >> 4;
+  const int Mc = a.M, Nc = a.N, NS = a.ns, NP = a.np;  // complex widths here
+  const int MR = 4 * S2;                               // staged rows per matrix
+
+  for (int idx = tid; idx < MR * NP; idx += 256) {
+    const int m = idx / NP, n = idx - m * NP;
+    float br = 0.0f, bi = 0.0f;
+    if (m < Mc && n < Nc) {
+      const float* b = a.B + 2 * ((int64_t)n * a.ldb + m);
+      br = b[0];
+      bi = b[1];
+    }
+    bs[m * NS + n] = br;
+    bs[(MR + m) * NS + n] = bi;
+    bs[(2 * MR + m) * NS + n] = br + bi;
+  }
+  __syncthreads();
+
+  const float* bl0 = bs + (lg * S2) * NS + li;
+  const int64_t K = a.K;
+  const int64_t gw = (int64_t)blockIdx.x * 4 + (tid >> 6);
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  const int mbeg = lg * S2;
+  const int64_t nblk = (K + 15) >> 4;
+  typedef float f2 __attribute__((ext_vector_type(2)));
+
+  // A slice of 16-row block blk, zero past column Mc (B's rows there are
+  // zero, but a NaN or infinity times zero is not).  With beta != 0 the
+  // loads go without branches: rows past K read row K - 1 (they feed only
+  // rows of D that are never stored) and columns past Mc read column Mc - 1
+  // and are then zeroed; with beta = 0 the extra address registers cost a
+  // wave per SIMD and the branches measured faster (Mc = Nc = 48: 4.1
+  // against 3.0 TB/s), with beta != 0 slower (3.4 against 3.6 TB/s).
+  auto load = [&](int64_t blk, float (&xr)[S2], float (&xi)[S2]) {
+    if constexpr (BETA0) {
+      const int64_t row = blk * 16 + li;
+      const bool ok = row < K;
+      const float* ar = a.A + 2 * (row * a.lda + mbeg);
+#pragma unroll
+      for (int s = 0; s < S2; s++) {
+        const bool v = ok && mbeg + s < Mc;
+        const f2 e = v ? *gmem(reinterpret_cast<const f2*>(ar + 2 * s)) : f2{0.0f, 0.0f};
+        xr[s] = e.x;
+        xi[s] = e.y;
+      }
+    } else {
+      const int64_t row = blk * 16 + li < K ? blk * 16 + li : K - 1;
+      const float* ar = a.A + 2 * row * a.lda;
+#pragma unroll
+      for (int s = 0; s < S2; s++) {
+        const int col = mbeg + s;
+        const f2 e = *gmem(reinterpret_cast<const f2*>(ar + 2 * (col < Mc ? col : Mc - 1)));
+        xr[s] = col < Mc ? e.x : 0.0f;
+        xi[s] = col < Mc ? e.y : 0.0f;
+      }
+    }
+  };
+  // C entries of block blk that this lane owns (beta != 0 only); clamped
+  // like A, the entries out of range are never stored
+  auto cload = [&](int64_t blk, f2 (&cv)[NT][4]) {
+#pragma unroll
+    for (int u = 0; u < NT; u++) {
+      const int n = 16 * u + li < Nc ? 16 * u + li : Nc - 1;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        int64_t row = blk * 16 + Mfma<float>::drow(lg, v);
+        row = row < K ? row : K - 1;
+        cv[u][v] = *gmem(reinterpret_cast<const f2*>(a.C + 2 * (row * a.ldc + n)));
+      }
+    }
+  };
+  auto work = [&](int64_t blk, const float (&xr)[S2], const float (&xi)[S2],
+                  const f2 (&cv)[NT][4]) {
+    V4 t1[NT], t2[NT], t3[NT];
+#pragma unroll
+    for (int u = 0; u < NT; u++) t1[u] = t2[u] = t3[u] = V4{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < S2; s++) {
+      const float xs = xr[s] + xi[s];
+      const float* bp = bl0 + s * NS;
+#pragma unroll
+      for (int u = 0; u < NT; u++) {
+        t1[u] = Mfma<float>::mac(xr[s], bp[16 * u], t1[u]);
+        t2[u] = Mfma<float>::mac(xi[s], bp[MR * NS + 16 * u], t2[u]);
+        t3[u] = Mfma<float>::mac(xs, bp[2 * MR * NS + 16 * u], t3[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NT; u++) {
+      const int n = 16 * u + li;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int64_t row = blk * 16 + Mfma<float>::drow(lg, v);
+        if (row < K && n < Nc) {
+#pragma clang fp contract(off)
+          const float re = t1[u][v] - t2[u][v];
+          const float im = (t3[u][v] - t1[u][v]) - t2[u][v];
+          float o_re = a.alpha * re - a.alpha_i * im;
+          float o_im = a.alpha * im + a.alpha_i * re;
+          if (!BETA0) {
+            o_re = (a.beta * cv[u][v].x - a.beta_i * cv[u][v].y) + o_re;
+            o_im = (a.beta * cv[u][v].y + a.beta_i * cv[u][v].x) + o_im;
+          }
+          *gmem(reinterpret_cast<f2*>(a.C + 2 * (row * a.ldc + n))) = f2{o_re, o_im};
+        }
+      }
+    }
+  };
+
+  // A block's A loads all precede its stores, and only its own rows of C are
+  // read or written, so C may alias A.
+  f2 cv[NT][4];
+#pragma unroll
+  for (int u = 0; u < NT; u++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) cv[u][v] = f2{0.0f, 0.0f};
+  if constexpr (PF) {
+    // two register sets: the A loads of the wave's next block are in flight
+    // during this block's matrix-core steps; C of this block (beta != 0) is
+    // requested before them, so the epilogue waits for it alone
+    float xr0[S2], xi0[S2], xr1[S2], xi1[S2];
+    if (gw < nblk) {
+      load(gw, xr0, xi0);
+      for (int64_t b0 = gw; b0 < nblk; b0 += 2 * nw) {
+        if (!BETA0) cload(b0, cv);
+        load(b0 + nw, xr1, xi1);
+        work(b0, xr0, xi0, cv);
+        if (b0 + nw >= nblk) break;
+        if (!BETA0) cload(b0 + nw, cv);
+        load(b0 + 2 * nw, xr0, xi0);
+        work(b0 + nw, xr1, xi1, cv);
+      }
+    }
+  } else {
+    // beta = 0 with room for two or more waves per SIMD: RB blocks per pass,
+    // their loads issued together
+    constexpr int RB = S2 >= 16 ? 1 : 2;
+    for (int64_t b0 = gw * RB; b0 < nblk; b0 += nw * RB) {
+      float xr[RB][S2], xi[RB][S2];
+#pragma unroll
+      for (int r = 0; r < RB; r++) load(b0 + r, xr[r], xi[r]);
+#pragma unroll
+      for (int r = 0; r < RB; r++) work(b0 + r, xr[r], xi[r], cv);
+    }
+  }
+}
+
+template <int S2, int NT>
+static hipError_t launch_c3(const TlArgs<float>& a, int blocks, size_t lds, bool beta0,
+                            hipStream_t st) {
+  auto k = beta0 ? ctsmm3_kernel<S2, NT, true> : ctsmm3_kernel<S2, NT, false>;
+  hipError_t e = hipFuncSetAttribute((const void*)k,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k, dim3(blocks), dim3(256), lds, st, a);
+  return hipGetLastError();
+}
+
+// Complex float, 17..64 complex columns of A, at most 64 of C: the
+// three-product kernel above; complex widths, lda / ldc in complex units.
+static skinny_status_t launch_ctsmm3(skinny_handle_t h, size_t blockCount, int Mc, int Nc,
+                                     int64_t K, const float* A, int64_t lda, float alpha,
+                                     float alpha_i, const float* B, int64_t ldb, float beta,
+                                     float beta_i, float* C, int64_t ldc, hipStream_t stream) {
+  TlArgs<float> a;
+  a.A = A;
+  a.B = B;
+  a.C = C;
+  a.K = K;
+  a.lda = lda;
+  a.ldb = ldb;
+  a.ldc = ldc;
+  a.M = Mc;
+  a.N = Nc;
+  a.alpha = alpha;
+  a.beta = beta;
+  a.alpha_i = alpha_i;
+  a.beta_i = beta_i;
+  a.vec = 0;
+  const int S2 = Mc <= 32 ? 8 : 16;
+  const int NT = (Nc + 15) / 16;
+  a.np = 16 * NT;
+  // the four lane groups of an operand read sit S2 rows apart: a row stride
+  // that puts them on different banks where a small pad can
+  a.ns = a.np;
+  for (int pad = 0; pad <= 16; pad++) {
+    const int64_t words = (int64_t)S2 * (a.np + pad);
+    if (words % 64 == 16 || words % 64 == 48) {
+      a.ns = a.np + pad;
+      break;
+    }
+  }
+  const size_t lds = (size_t)3 * 4 * S2 * a.ns * sizeof(float);
+  int blocks = blockCount > 0 ? (int)blockCount
+                              : tuned_blocks_tsmm(h, 2 * Mc, 2 * Nc, K, 4, false, true, false,
+                                                  TT_FC, Mc, Nc);
+  const bool beta0 = beta == 0.0f && beta_i == 0.0f;
+  hipError_t e;
+#define SK_C3(s2, nt) \
+  if (S2 == (s2) && NT == (nt)) e = launch_c3<s2, nt>(a, blocks, lds, beta0, stream); else
+  SK_C3(8, 1) SK_C3(8, 2) SK_C3(8, 3) SK_C3(8, 4) SK_C3(16, 1) SK_C3(16, 2) SK_C3(16, 3)
+  e = launch_c3<16, 4>(a, blocks, lds, beta0, stream);
+#undef SK_C3
+  if (e != hipSuccess) return hip_fail(e, "tsmm (complex float, 3M) launch");
+  return SKINNY_OK;
+}
+
 template <typename T, int S, bool CPLX>
 static hipError_t launch_lds_s(const TlArgs<T>& a, int blocks, size_t lds,
                                bool beta0, hipStream_t st) {
@@ -296,6 +530,13 @@ skinny_status_t launch_tsmm_lds(skinny_handle_t h, size_t blockCount,
     const skinny_status_t q = launch_tsmm_q4(h, blockCount, M, N, K, A, lda, alpha, alpha_i,
                                              B, ldb, beta, beta_i, C, ldc, cplx, stream);
     if (q != SKINNY_UNSUPPORTED) return q;
+  } else {
+    // complex float with 17..64 complex columns of A: the three-product form
+    if (cplx && M / 2 >= 17 && M / 2 <= 64 && N / 2 <= 64 && (uintptr_t)A % 8 == 0 &&
+        (uintptr_t)C % 8 == 0)
+      return launch_ctsmm3(h, blockCount, (int)(M / 2), (int)(N / 2), K, (const float*)A,
+                           lda / 2, (float)alpha, (float)alpha_i, (const float*)B, ldb,
+                           (float)beta, (float)beta_i, (float*)C, ldc / 2, stream);
   }
   TlArgs<T> a;
   a.A = A;

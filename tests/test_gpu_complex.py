@@ -138,7 +138,9 @@ def test_tsmm_one_column_real_beta(gpu, orc, dt):
                 assert np.array_equal(host(dC), want), (K, lda, ldc, inplace, beta, bc)
 
 
-@pytest.mark.parametrize("M,N", [(1, 1), (4, 4), (8, 8), (16, 16), (32, 32), (5, 9)])
+@pytest.mark.parametrize("M,N", [(1, 1), (4, 4), (8, 8), (16, 16), (32, 32), (5, 9),
+                                 (17, 17), (33, 33), (40, 17), (17, 40), (48, 64), (64, 64),
+                                 (64, 8), (57, 3), (24, 61)])
 def test_fuzz_complex64(gpu, orc, M, N):
     rng = random.Random(M * 13 + N)
     for _ in range(2):
@@ -186,67 +188,78 @@ def test_real_tsmm_wide(gpu, orc, M, N):
         assert np.array_equal(host(dC), want)
 
 
-@pytest.mark.parametrize("offA,offC", [(0, 0), (8, 0), (0, 8), (8, 8)])
-def test_ztsmm_wide_alignment(gpu, orc, offA, offC):
-    """Complex TSMM above 32 columns: the three-product kernel with A and C
-    16-byte aligned, the real-view kernel when either sits at an 8-byte
-    offset; both give the oracle's result bitwise on integer data, out of
-    place and in place, beta 0 and complex."""
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("half", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_ztsmm_wide_alignment(gpu, orc, dt, half):
+    """Complex TSMM above 32 columns: the three-product kernels need A and C
+    aligned to a whole complex element (16 bytes for double, 8 for float);
+    with either at half that, the real-view kernel runs.  Both give the
+    oracle's result bitwise on integer data, out of place and in place, beta
+    0 and complex."""
     import torch
-    rng = random.Random(900 + offA + 2 * offC)
+    wide = dt == np.complex128
+    real, npreal = (torch.float64, np.float64) if wide else (torch.float32, np.float32)
+    fn = gpu.ztsmm if wide else gpu.ctsmm
+    rb = np.dtype(dt).itemsize // 2  # bytes per real part
+    offA, offC = half[0] * rb, half[1] * rb
+    rng = random.Random(900 + offA + 2 * offC + rb)
     for M, N, inplace, beta in ((48, 48, False, 0.0), (64, 33, False, complex(1, -1)),
-                                (40, 40, True, complex(-1, 1)), (64, 64, True, 0.0)):
+                                (40, 40, True, complex(-1, 1)), (64, 64, True, 0.0),
+                                (17, 64, False, complex(0, 1))):
         K = rng.randint(2000, 9000)
         seed = rng.randint(0, 1 << 30)
         ldb = M
         ldc = max(M, N) if inplace else N
         lda = ldc if inplace else M
-        note(f"ztsmm alignment offA={offA} offC={offC} M={M} N={N} K={K} inplace={inplace}")
-        B = cint(orc, N * ldb, seed, 1)
-        C = cint(orc, K * ldc, seed, 2)
-        A = C.copy() if inplace else cint(orc, K * lda, seed, 0)
+        note(f"tsmm alignment {np.dtype(dt).name} offA={offA} offC={offC} M={M} N={N} K={K} "
+             f"inplace={inplace}")
+        B = cint(orc, N * ldb, seed, 1).astype(dt)
+        C = cint(orc, K * ldc, seed, 2).astype(dt)
+        A = C.copy() if inplace else cint(orc, K * lda, seed, 0).astype(dt)
         want = C.copy()
         orc.tsmm(M, N, K, want if inplace else A, lda, complex(2, 1), B, ldb, beta, want, ldc)
-        # complex buffers at byte offsets inside float64 device buffers
-        bufC = torch.zeros(2 * C.size + 2, dtype=torch.float64, device="cuda")
-        oc = offC // 8
-        bufC[oc:oc + 2 * C.size] = torch.from_numpy(C.view(np.float64)).cuda()
+        # complex buffers at byte offsets inside real device buffers
+        bufC = torch.zeros(2 * C.size + 2, dtype=real, device="cuda")
+        oc = half[1]
+        bufC[oc:oc + 2 * C.size] = torch.from_numpy(C.view(npreal)).cuda()
         if inplace:
             if offA != offC:
                 continue
             addrA = addrC = bufC.data_ptr() + offC
         else:
-            bufA = torch.zeros(2 * A.size + 2, dtype=torch.float64, device="cuda")
-            oa = offA // 8
-            bufA[oa:oa + 2 * A.size] = torch.from_numpy(A.view(np.float64)).cuda()
+            bufA = torch.zeros(2 * A.size + 2, dtype=real, device="cuda")
+            oa = half[0]
+            bufA[oa:oa + 2 * A.size] = torch.from_numpy(A.view(npreal)).cuda()
             addrA, addrC = bufA.data_ptr() + offA, bufC.data_ptr() + offC
         dB = dev(B)
-        assert gpu.ztsmm(0, M, N, K, addrA, lda, complex(2, 1), dB, ldb, beta, addrC, ldc)
-        got = host(bufC[oc:oc + 2 * C.size]).view(np.complex128)
+        assert fn(0, M, N, K, addrA, lda, complex(2, 1), dB, ldb, beta, addrC, ldc)
+        got = host(bufC[oc:oc + 2 * C.size]).view(dt)
         assert np.array_equal(got, want), (M, N, inplace)
 
 
-@pytest.mark.parametrize("M,lda", [(33, 48), (40, 48), (49, 64), (20, 32)])
-def test_ztsmm_reads_only_m_columns(gpu, orc, M, lda):
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("M,lda", [(33, 48), (40, 48), (49, 64), (20, 32), (17, 24)])
+def test_ztsmm_reads_only_m_columns(gpu, orc, dt, M, lda):
     """The wide complex TSMM reads only the first M columns of each row of
     A: the columns M .. lda - 1 hold NaN, which any read of them would carry
     into C (the kernels' A slices can reach past M when M is not a multiple
     of their slice width).  Integer data, bitwise against the oracle, beta 0
     and complex."""
-    rng = random.Random(77 + M)
+    rng = random.Random(77 + M + np.dtype(dt).itemsize)
+    fn = gpu.ztsmm if dt == np.complex128 else gpu.ctsmm
     N = 40
     for beta in (0.0, complex(1, -2)):
         K = rng.randint(3000, 7000)
         seed = rng.randint(0, 1 << 30)
-        note(f"ztsmm NaN padding M={M} lda={lda} N={N} K={K} beta={beta}")
-        A = cint(orc, K * lda, seed, 0).reshape(K, lda)
+        note(f"tsmm NaN padding {np.dtype(dt).name} M={M} lda={lda} N={N} K={K} beta={beta}")
+        A = cint(orc, K * lda, seed, 0).astype(dt).reshape(K, lda)
         A[:, M:] = complex(np.nan, np.nan)
         A = A.ravel()
-        B = cint(orc, N * M, seed, 1)
-        C = cint(orc, K * N, seed, 2)
+        B = cint(orc, N * M, seed, 1).astype(dt)
+        C = cint(orc, K * N, seed, 2).astype(dt)
         want = C.copy()
         orc.tsmm(M, N, K, A, lda, complex(2, 1), B, M, beta, want, N)
         assert not np.isnan(want).any()
         dC = dev(C)
-        assert gpu.ztsmm(0, M, N, K, dev(A), lda, complex(2, 1), dev(B), M, beta, dC, N)
+        assert fn(0, M, N, K, dev(A), lda, complex(2, 1), dev(B), M, beta, dC, N)
         assert np.array_equal(host(dC), want), (M, lda, beta)
