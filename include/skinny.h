@@ -302,8 +302,9 @@ skinny_status_t skinny_autotune(skinny_handle_t h, int op, int dtype,
                                 double* best_ms);
 /* The table as text lines "op dtype M N log2K variant blockCount" (TSMTTSM
  * variant bit 0: doubled unroll, bit 1: partials summed by the last arrivals
- * of the same launch instead of a second kernel; TSMM: 0); returns the size
- * needed including the terminating NUL. */
+ * of the same launch instead of a second kernel; real TSMM up to 64 columns,
+ * bit 0: the other kernel form, whole rows per thread or matrix cores;
+ * complex TSMM: 0); returns the size needed including the terminating NUL. */
 size_t skinny_tune_export(skinny_handle_t h, char* buf, size_t len);
 skinny_status_t skinny_tune_import(skinny_handle_t h, const char* text);
 void skinny_tune_clear(skinny_handle_t h);

@@ -22,7 +22,7 @@ struct TuneKey {
   }
 };
 struct TuneEntry {
-  int uvar;  // TSMTTSM variant: bit 0 doubled unroll, bit 1 one launch
+  int uvar;  // TSMTTSM: bit 0 doubled unroll, bit 1 one launch; real TSMM: bit 0 other form
   int G;     // workgroups along K
 };
 // Device workspace of one stream: [counter region (zeroed at allocation,

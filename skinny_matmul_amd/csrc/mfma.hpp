@@ -22,6 +22,14 @@ __device__ __forceinline__ __attribute__((address_space(3))) T* lmem(T* p) {
   return (__attribute__((address_space(3))) T*)p;
 }
 
+// Constant-memory view of a pointer to data that no wave writes during the
+// kernel: uniform accesses through it compile to scalar loads (s_load, through
+// the scalar data cache, into SGPRs) instead of per-lane vector loads.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* cmem(const T* p) {
+  return (const __attribute__((address_space(4))) T*)p;
+}
+
 // Hand-over of workgroup partials to the workgroup that sums them, within one
 // launch.  Each XCD has its own L2, and a release at agent scope writes the
 // XCD's L2 back (buffer_wbl2), which costs microseconds per workgroup when

@@ -11,12 +11,13 @@
 //    (the reference's varip kernels need __syncthreads between the reads and
 //    the writes of a row, varip1.cuh:18).
 //  * B is uniform across the wavefront: every B(m, n) access has the same
-//    address in all lanes, so it is fetched with scalar loads into SGPRs and
-//    broadcast for free as the scalar operand of v_fma (the reference stages B
-//    through shared memory, fix3.cuh:10-18).
-//  * Row loads and stores use two-element vector accesses (16 bytes for
-//    double, 8 for float) when the leading dimension and base pointer allow
-//    it.
+//    address in all lanes.  FP32 fetches it with scalar loads into SGPRs,
+//    the scalar operand of v_fma (the reference stages B through shared
+//    memory, fix3.cuh:10-18); FP64 keeps vector loads, which measured faster
+//    there (tm_bload).
+//  * Row loads and stores use 16-byte accesses (two doubles or four floats),
+//    else two-element ones, when the leading dimensions and base pointers
+//    allow it.
 //  * The row length is rounded up to a compile-time class MC in
 //    {1, 2, 4, 8, 16, 32, 64} and masked with uniform branches; N is a runtime
 //    loop unrolled by two, so one instantiation serves a range of widths.
@@ -90,6 +91,20 @@ __device__ __forceinline__ T tm_bval(const TmArgs<T>& a, int m, int n) {
   return (n & 1) == 0 ? -pi : pr;
 }
 
+// B(m, n) in the whole-row kernel, the same address in every lane: FP32
+// reads it with scalar loads into SGPRs (through a constant-memory view; with
+// vector loads each step of the column loop waited for them, which shows from
+// about 8 columns of C: M = 12, N = 8, beta = 1, best grids, 3280 -> 4887
+// GB/s), FP64 with vector loads (scalar ones measured 4-12% slower at M =
+// 2..8 with C one or two columns wide, in place)
+template <typename T>
+__device__ __forceinline__ T tm_bload(const T* p) {
+  if constexpr (sizeof(T) == 4)
+    return *cmem(p);
+  else
+    return *p;
+}
+
 template <typename T>
 __device__ __forceinline__ T fma_t(T a, T b, T c) {
   return __builtin_fma(a, b, c);
@@ -107,17 +122,29 @@ __device__ __forceinline__ T axpby_t(T s, T c, T alpha, T beta, bool beta0) {
   return beta * c + alpha * s;
 }
 
-// rows per thread per iteration (independent rows in flight)
-template <int MC>
+// rows per thread per iteration (independent rows in flight); FP32 takes two
+// rows up to 16 wide (four at 4 wide measured 4-13% slower with C one to
+// three columns wide, 1-4% faster at M = 3 with beta = 0)
+template <int MC, int EB>
 struct TmRows {
-  static constexpr int value = MC <= 2 ? 4 : (MC <= 8 ? 2 : 1);
+  static constexpr int value =
+      EB == 8 ? (MC <= 2 ? 4 : (MC <= 8 ? 2 : 1)) : (MC <= 2 ? 4 : (MC <= 16 ? 2 : 1));
 };
 
-// CPLX: B is the real view Bh of a complex B (tm_bval), for complex
+// VW consecutive elements as one load or store
+template <typename T, int VW>
+struct VecN {
+  typedef T type __attribute__((ext_vector_type(VW)));
+};
+
+// VW: elements per row access, when the leading dimensions and the base
+// pointers allow it: 1, 2 (16 bytes of double, 8 of float) or 4 (16 bytes of
+// float).  CPLX: B is the real view Bh of a complex B (tm_bval), for complex
 // products one column wide (real view M = N = 2)
-template <typename T, int MC, bool BETA0, bool VEC, bool CPLX = false>
+template <typename T, int MC, bool BETA0, int VW, bool CPLX = false>
 __global__ void __launch_bounds__(256) tsmm_kernel(TmArgs<T> a) {
-  constexpr int R = TmRows<MC>::value;
+  constexpr int R = TmRows<MC, (int)sizeof(T)>::value;
+  constexpr int LW = VW < MC ? VW : MC;  // elements per A load
   const int M = a.M, N = a.N;
   const int64_t K = a.K;
   const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
@@ -131,31 +158,61 @@ __global__ void __launch_bounds__(256) tsmm_kernel(TmArgs<T> a) {
       const int64_t row = base + (int64_t)r * nthreads;
       const bool ok = row < K;
       const T* ar = a.A + row * a.lda;
-      if (VEC && (MC % 2 == 0)) {
 #pragma unroll
-        for (int m = 0; m < MC; m += 2) {
-          if (m < M) {
-            if (ok && m + 1 < M) {
-              const typename Vec2<T>::type v =
-                  *reinterpret_cast<const typename Vec2<T>::type*>(ar + m);
-              av[r][m] = v.x;
-              av[r][m + 1] = v.y;
-            } else {
-              av[r][m] = ok ? ar[m] : T(0);
-              av[r][m + 1] = T(0);
-            }
+      for (int m = 0; m < MC; m += LW) {
+        if (m < M) {
+          if (LW > 1 && ok && m + LW <= M) {
+            const typename VecN<T, LW>::type v =
+                *reinterpret_cast<const typename VecN<T, LW>::type*>(ar + m);
+#pragma unroll
+            for (int j = 0; j < LW; j++) av[r][m + j] = v[j];
           } else {
-            av[r][m] = T(0);
-            av[r][m + 1] = T(0);
-          }
-        }
-      } else {
 #pragma unroll
-        for (int m = 0; m < MC; m++) av[r][m] = (m < M && ok) ? ar[m] : T(0);
+            for (int j = 0; j < LW; j++) av[r][m + j] = (ok && m + j < M) ? ar[m + j] : T(0);
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < LW; j++) av[r][m + j] = T(0);
+        }
       }
     }
     // every A value of these rows is now in registers: in-place safe
     int n = 0;
+    if constexpr (VW == 4) {
+      // four columns of C per access
+      for (; n + 4 <= N; n += 4) {
+        T sv[4][R];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+          for (int r = 0; r < R; r++) sv[j][r] = T(0);
+        const T* b0 = B + (int64_t)n * a.ldb;
+#pragma unroll
+        for (int m = 0; m < MC; m++) {
+          if (m < M) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+              const T x = CPLX ? tm_bval(a, m, n + j) : tm_bload(b0 + j * a.ldb + m);
+#pragma unroll
+              for (int r = 0; r < R; r++) sv[j][r] = fma_t(av[r][m], x, sv[j][r]);
+            }
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int64_t row = base + (int64_t)r * nthreads;
+          if (row < K) {
+            typedef typename VecN<T, 4>::type V;
+            V* cr = reinterpret_cast<V*>(a.C + row * a.ldc + n);
+            V c = BETA0 ? V{} : *cr;
+            V o;
+#pragma unroll
+            for (int j = 0; j < 4; j++) o[j] = axpby_t(sv[j][r], c[j], a.alpha, a.beta, BETA0);
+            *cr = o;
+          }
+        }
+      }
+    }
     for (; n + 2 <= N; n += 2) {
       T s0[R], s1[R];
 #pragma unroll
@@ -168,8 +225,8 @@ __global__ void __launch_bounds__(256) tsmm_kernel(TmArgs<T> a) {
 #pragma unroll
       for (int m = 0; m < MC; m++) {
         if (m < M) {
-          T x0 = CPLX ? tm_bval(a, m, n) : b0[m];
-          T x1 = CPLX ? tm_bval(a, m, n + 1) : b1[m];
+          T x0 = CPLX ? tm_bval(a, m, n) : tm_bload(b0 + m);
+          T x1 = CPLX ? tm_bval(a, m, n + 1) : tm_bload(b1 + m);
 #pragma unroll
           for (int r = 0; r < R; r++) {
             s0[r] = fma_t(av[r][m], x0, s0[r]);
@@ -186,8 +243,8 @@ __global__ void __launch_bounds__(256) tsmm_kernel(TmArgs<T> a) {
           T c1 = BETA0 ? T(0) : cr[1];
           T o0 = axpby_t(s0[r], c0, a.alpha, a.beta, BETA0);
           T o1 = axpby_t(s1[r], c1, a.alpha, a.beta, BETA0);
-          if (VEC) {
-            *reinterpret_cast<typename Vec2<T>::type*>(cr) = typename Vec2<T>::type{o0, o1};
+          if (VW >= 2) {
+            *reinterpret_cast<typename VecN<T, 2>::type*>(cr) = typename VecN<T, 2>::type{o0, o1};
           } else {
             cr[0] = o0;
             cr[1] = o1;
@@ -203,7 +260,7 @@ __global__ void __launch_bounds__(256) tsmm_kernel(TmArgs<T> a) {
 #pragma unroll
       for (int m = 0; m < MC; m++) {
         if (m < M) {
-          T x0 = CPLX ? tm_bval(a, m, n) : b0[m];
+          T x0 = CPLX ? tm_bval(a, m, n) : tm_bload(b0 + m);
 #pragma unroll
           for (int r = 0; r < R; r++) s0[r] = fma_t(av[r][m], x0, s0[r]);
         }
@@ -398,7 +455,21 @@ static void launch_mfma_s(const TmArgs<T>& a, int blocks, int nt, bool beta0,
 // N <= 2 (M = 5, N = 1: 1990 -> 4932 GB/s) and at M <= 4 for N = 4; from
 // N = 3 on the matrix cores' transposed tile (coalesced old-C loads) wins
 // otherwise (M = 8, N = 3: 3919 -> 5084 GB/s).
-static bool tm_rows_wins(int64_t M, int64_t N, bool inplace, bool beta0) {
+//
+// FP32 rows take 16-byte accesses and scalar B loads, and measured faster than
+// the matrix cores (K = 2^26, the rows at four workgroups per CU) also for C
+// three or four columns wide with A up to 12 wide (M = N = 4: 3443 -> 5086
+// GB/s; M = 12, N = 4, beta = 1: 3511 -> 5105; in place M = N = 3: 3776 ->
+// 4042), except for C three columns wide with A 7..8 wide, or 9..12 wide and
+// beta = 0 out of place (0.90-0.94 of the matrix cores), and for C eight
+// columns wide with beta != 0 out of place (A 4..12 wide; M = 12, N = 8: 3024
+// -> 4954) or in place (A 2..12 wide).
+static bool tm_rows_wins(int64_t M, int64_t N, bool inplace, bool beta0, int eb) {
+  if (eb == 4 && M <= 12) {
+    if (N == 4) return true;
+    if (N == 3 && (M <= 6 || (M > 8 && (!beta0 || inplace)))) return true;
+    if (N == 8 && (inplace ? M >= 2 : (!beta0 && M >= 4))) return true;
+  }
   if (!beta0 && !inplace) {
     if (N <= 2) return M <= 12;
     return N == 4 && M <= 4;
@@ -425,11 +496,17 @@ int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
   // 4 columns wide (few loads per lane) and two otherwise
   int64_t want = (K + 255) / 256;
   int64_t per_cu;
-  const bool rows = real ? (M <= 64 && N <= 64 && tm_rows_wins(M, N, inplace, beta0))
+  const bool rows = real ? (M <= 64 && N <= 64 && tm_rows_wins(M, N, inplace, beta0, elem_bytes))
                          : (M <= 2 && N <= 2);
-  // FP32 rows carry half the bytes per row in flight: eight per CU
-  // (M = 3, N = 1: 3332 -> 4512 GB/s; M = 4, N = 2: 4178 -> 5138 GB/s)
-  if (rows)
+  // FP32 rows (16-byte accesses, up to four rows per thread): four per CU,
+  // three from 8 columns of C on (more stores per row), eight for A and C at
+  // most 4 and 3 wide with beta != 0 out of place, and for M = N = 1
+  // (the complex view's one-column rows keep eight)
+  if (rows && elem_bytes == 4 && real)
+    per_cu = ((!beta0 && !inplace && M <= 4 && N <= 3) || (M == 1 && N == 1)) ? 8
+             : N >= 8                                                           ? 3
+                                                                                : 4;
+  else if (rows)
     per_cu = elem_bytes == 4 ? 8 : ((M <= 1 || !beta0) ? 4 : 2);
   else
     per_cu = M <= 4 ? 8 : 2;
@@ -439,26 +516,38 @@ int tuned_blocks_tsmm(skinny_handle_t h, int64_t M, int64_t N, int64_t K,
   return (int)want;
 }
 
-template <typename T, int MC, bool BETA0, bool VEC, bool CPLX>
+template <typename T, int MC, bool BETA0, int VW, bool CPLX>
 static void launch_tm(const TmArgs<T>& a, int blocks, hipStream_t s) {
-  hipLaunchKernelGGL((tsmm_kernel<T, MC, BETA0, VEC, CPLX>), dim3(blocks),
-                     dim3(256), 0, s, a);
+  hipLaunchKernelGGL((tsmm_kernel<T, MC, BETA0, VW, CPLX>), dim3(blocks), dim3(256), 0, s, a);
 }
 
-template <typename T, int MC, bool CPLX = false>
-static void launch_mc(const TmArgs<T>& a, int blocks, bool beta0, bool vec,
-                      hipStream_t s) {
-  if (beta0) {
-    if (vec)
-      launch_tm<T, MC, true, true, CPLX>(a, blocks, s);
-    else
-      launch_tm<T, MC, true, false, CPLX>(a, blocks, s);
-  } else {
-    if (vec)
-      launch_tm<T, MC, false, true, CPLX>(a, blocks, s);
-    else
-      launch_tm<T, MC, false, false, CPLX>(a, blocks, s);
+template <typename T, int MC, bool BETA0, bool CPLX>
+static void launch_vw(const TmArgs<T>& a, int blocks, int vw, hipStream_t s) {
+  if constexpr (sizeof(T) == 4) {
+    if (vw >= 4) return launch_tm<T, MC, BETA0, 4, CPLX>(a, blocks, s);
   }
+  if (vw >= 2) return launch_tm<T, MC, BETA0, 2, CPLX>(a, blocks, s);
+  launch_tm<T, MC, BETA0, 1, CPLX>(a, blocks, s);
+}
+
+// vw: elements per row access (row_vw)
+template <typename T, int MC, bool CPLX = false>
+static void launch_mc(const TmArgs<T>& a, int blocks, bool beta0, int vw, hipStream_t s) {
+  if (beta0)
+    launch_vw<T, MC, true, CPLX>(a, blocks, vw, s);
+  else
+    launch_vw<T, MC, false, CPLX>(a, blocks, vw, s);
+}
+
+// widest row access that every row start of A and C allows: 16 bytes (two
+// doubles or four floats), else two elements, else one
+template <typename T>
+static int row_vw(const T* A, int64_t lda, const T* C, int64_t ldc) {
+  const uintptr_t pa = (uintptr_t)A, pc = (uintptr_t)C;
+  if (sizeof(T) == 4 && lda % 4 == 0 && ldc % 4 == 0 && pa % 16 == 0 && pc % 16 == 0) return 4;
+  if (lda % 2 == 0 && ldc % 2 == 0 && pa % (2 * sizeof(T)) == 0 && pc % (2 * sizeof(T)) == 0)
+    return 2;
+  return 1;
 }
 
 template <typename T>
@@ -492,7 +581,13 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
   int blocks = blockCount > 0
                    ? (int)blockCount
                    : tuned_blocks_tsmm(h, M, N, K, (int)sizeof(T), true, beta0, inplace);
-  if (!tm_rows_wins(M, N, inplace, beta0)) {
+  // with blockCount 0, variant bit 0 of a tuned entry: the other kernel form
+  bool rows = tm_rows_wins(M, N, inplace, beta0, (int)sizeof(T));
+  TuneEntry te;
+  if (blockCount == 0 && tune_lookup(h, 1, sizeof(T) == 8 ? TT_DR : TT_FR, M, N, K, &te) &&
+      (te.uvar & 1))
+    rows = !rows;
+  if (!rows) {
     if (!beta0 && (const void*)A == (const void*)C && lda == ldc && N <= M) {
       a.fold = 1;
       a.fa = alpha;
@@ -521,13 +616,11 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
     if (e != hipSuccess) return hip_fail(e, "tsmm launch");
     return SKINNY_OK;
   }
-  // two-element row accesses need rows aligned to two elements in A and C
-  const bool vec = (lda % 2 == 0) && (ldc % 2 == 0) && ((uintptr_t)A % (2 * sizeof(T)) == 0) &&
-                   ((uintptr_t)C % (2 * sizeof(T)) == 0);
+  const int vec = row_vw(A, lda, C, ldc);
   int MC = M <= 1 ? 1 : M <= 2 ? 2 : M <= 4 ? 4 : M <= 8 ? 8 : M <= 16 ? 16
            : M <= 32 ? 32 : 64;
   switch (MC) {
-    case 1: launch_mc<T, 1>(a, blocks, beta0, false, stream); break;
+    case 1: launch_mc<T, 1>(a, blocks, beta0, 1, stream); break;
     case 2: launch_mc<T, 2>(a, blocks, beta0, vec, stream); break;
     case 4: launch_mc<T, 4>(a, blocks, beta0, vec, stream); break;
     case 8: launch_mc<T, 8>(a, blocks, beta0, vec, stream); break;
@@ -576,8 +669,10 @@ skinny_status_t launch_tsmm_cview(skinny_handle_t h, size_t blockCount,
   if (Mc == 1) {
     // one complex column: the real view is 2 x 2, whole rows per thread
     // (a row is one complex value, one two-element access)
-    const bool vec = ((uintptr_t)A % (2 * sizeof(T)) == 0) &&
-                     ((uintptr_t)C % (2 * sizeof(T)) == 0);
+    const int vec = ((uintptr_t)A % (2 * sizeof(T)) == 0) &&
+                            ((uintptr_t)C % (2 * sizeof(T)) == 0)
+                        ? 2
+                        : 1;
     launch_mc<T, 2, true>(a, blocks, beta0, vec, stream);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "tsmm (complex view) launch");

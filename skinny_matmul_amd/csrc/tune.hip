@@ -166,8 +166,12 @@ skinny_status_t skinny_autotune(skinny_handle_t h, int op, int dtype,
   if (s == SKINNY_OK) {
     const double mults[] = {0.5, 1.0, 1.5, 2.0, 3.0, 4.0, 6.0, 8.0};
     // variants of the FP64 / FP32 / complex TSMTTSM kernels: bit 0 the
-    // doubled unroll, bit 1 the partials summed in the same launch
-    const int nuvar = (op == 0 && dtype != TT_HR) ? 4 : 1;
+    // doubled unroll, bit 1 the partials summed in the same launch; of the
+    // real TSMM up to 64 columns: bit 0 the other kernel form (whole rows or
+    // matrix cores)
+    const int nuvar = (op == 0 && dtype != TT_HR)                            ? 4
+                      : (op == 1 && (dtype == TT_DR || dtype == TT_FR) && M <= 64 && N <= 64) ? 2
+                                                                                : 1;
     for (int uvar = 0; uvar < nuvar && s == SKINNY_OK; uvar++) {
       for (double mu : mults) {
         const TuneEntry cand{uvar, std::max(1, (int)(mu * h->cu_count))};

@@ -111,6 +111,53 @@ def test_narrow_grid_fp64(gpu, orc, inplace):
                     assert np.array_equal(got, want), (M, N, K, lda, ldc, bc, beta)
 
 
+@pytest.mark.parametrize("inplace", [False, True])
+@pytest.mark.parametrize("off", [0, 1])
+def test_narrow_grid_fp32(gpu, orc, inplace, off):
+    """FP32 around the switch between whole-row threads (16-byte row accesses
+    when the leading dimensions are multiples of 4 and the rows 16-byte
+    aligned) and the matrix cores: every M in 1..12 with N in {1..6, 8, 9},
+    beta in {0, 1}, the default grid and a random one.  off = 0 keeps A and C
+    16-byte aligned with leading dimensions rounded up to 4; off = 1 starts
+    them one float into their buffers, with other leading dimensions."""
+    import torch
+    rng = random.Random(91 + 2 * off + inplace)
+    for M in range(1, 13):
+        for N in (1, 2, 3, 4, 5, 6, 8, 9):
+            for beta in (0.0, 1.0):
+                if off == 0:
+                    r4 = lambda x: (x + 3) // 4 * 4
+                    ldc = r4(max(M, N)) if inplace else r4(N)
+                    lda = ldc if inplace else r4(M) + 4 * rng.randint(0, 1)
+                elif inplace:
+                    ldc = max(M, N) + rng.randint(0, 2)
+                    lda = ldc
+                else:
+                    lda, ldc = M + rng.randint(0, 2), N + rng.randint(0, 2)
+                K = rng.randint(1000, 5000)
+                seed = rng.randint(0, 1 << 30)
+                for bc in (0, rng.randint(1, 300)):
+                    note(f"tsmm fp32 narrow M={M} N={N} K={K} lda={lda} ldc={ldc} off={off} "
+                         f"inplace={inplace} beta={beta} bc={bc} seed={seed}")
+                    B = orc.integers(N * M, seed, 1).astype(np.float32)
+                    C = orc.integers(K * ldc, seed, 2).astype(np.float32)
+                    A = C.copy() if inplace else orc.integers(K * lda, seed, 0).astype(np.float32)
+                    want = C.copy()
+                    orc.tsmm(M, N, K, want if inplace else A, lda, 1.0, B, M, beta,
+                             want, ldc)
+                    bufC = torch.zeros(C.size + 4, dtype=torch.float32, device="cuda")
+                    bufC[off:off + C.size] = torch.from_numpy(C).cuda()
+                    dC = bufC[off:off + C.size]
+                    if inplace:
+                        dA = dC
+                    else:
+                        bufA = torch.zeros(A.size + 4, dtype=torch.float32, device="cuda")
+                        bufA[off:off + A.size] = torch.from_numpy(A).cuda()
+                        dA = bufA[off:off + A.size]
+                    assert gpu.tsmm(bc, M, N, K, dA, lda, 1.0, dev(B), M, beta, dC, ldc)
+                    assert np.array_equal(host(dC), want), (M, N, K, lda, ldc, bc, beta)
+
+
 @pytest.mark.parametrize("M,N", [(1, 1), (4, 4), (8, 8), (5, 11), (16, 16), (32, 32), (64, 64)])
 def test_fuzz_fp32(gpu, orc, M, N):
     rng = random.Random(M * 17 + N)

@@ -45,8 +45,9 @@ def test_autotune_roundtrip_and_parity(gpu, orc):
 
 # (elem_bytes, M, N, beta0, inplace) -> workgroups per CU of the untuned grid.
 # Whole-row kernel (narrow C): FP64 four per CU for M = 1 or beta != 0, two
-# otherwise; FP32 eight.  Matrix-core kernel: eight for A at most 4 columns
-# wide, two otherwise.
+# otherwise; FP32 four, three for C 8 wide, eight for M <= 4, N <= 3 with
+# beta != 0 out of place and for M = N = 1.  Matrix-core kernel: eight for A
+# at most 4 columns wide, two otherwise.
 DEFAULT_GRID_CASES = [
     (8, 5, 1, False, False, 4),   # whole rows, beta != 0 (M = 5, N = 1, beta = 1)
     (8, 5, 1, True, False, 2),    # whole rows, beta == 0, M > 1
@@ -58,8 +59,13 @@ DEFAULT_GRID_CASES = [
     (8, 8, 8, True, False, 2),    # matrix cores, A 8 wide
     (8, 64, 64, True, False, 2),  # matrix cores, A 64 wide
     (8, 8, 3, False, False, 2),   # matrix cores (transposed tile), beta != 0
-    (4, 3, 1, True, False, 8),    # FP32 whole rows
-    (4, 4, 4, True, False, 8),    # FP32 matrix cores, A 4 wide
+    (4, 3, 1, True, False, 4),    # FP32 whole rows
+    (4, 1, 1, True, False, 8),    # FP32 whole rows, M = N = 1
+    (4, 4, 2, False, False, 8),   # FP32 whole rows, beta != 0, narrow A and C
+    (4, 4, 4, True, False, 4),    # FP32 whole rows, C 4 wide
+    (4, 12, 8, False, False, 3),  # FP32 whole rows, C 8 wide
+    (4, 8, 3, True, False, 2),    # FP32 matrix cores, A 8 wide
+    (4, 4, 6, True, False, 8),    # FP32 matrix cores, A 4 wide
     (4, 16, 16, True, False, 2),  # FP32 matrix cores
 ]
 
@@ -166,3 +172,39 @@ def test_one_launch_variant_exact(gpu, orc, dt, M, N, gram):
             if dt == "z":
                 got = got.view(np.complex128)
             assert np.array_equal(got, want), (variant, G)
+
+
+@pytest.mark.parametrize("dt", ["d", "s"])
+@pytest.mark.parametrize("M,N", [(4, 4), (2, 3), (12, 8), (6, 1), (1, 16), (16, 3), (3, 5)])
+def test_tsmm_form_variant_exact(gpu, orc, dt, M, N):
+    """Variant 1 (bit 0) of a real TSMM entry runs the other kernel form (whole
+    rows per thread where the rule picks the matrix cores, and the reverse);
+    both give the oracle's result bitwise on integer data, beta 0 and 1, out
+    of place and in place, and the variant survives an export."""
+    import torch
+    K = 54321
+    log2K = K.bit_length() - 1
+    dtype = {"d": 0, "s": 1}[dt]
+    npdt = np.float64 if dt == "d" else np.float32
+    h = gpu.Handle()
+    for variant in (0, 1):
+        gpu.tune_clear(handle=h)
+        gpu.tune_import(f"1 {dtype} {M} {N} {log2K} {variant} 300\n", handle=h)
+        assert f"1 {dtype} {M} {N} {log2K} {variant} 300" in gpu.tune_export(handle=h)
+        for beta, inplace in ((0.0, False), (1.0, False), (0.0, True), (-1.0, True)):
+            seed = M * 7 + N + 100 * variant
+            ldc = max(M, N) if inplace else N
+            lda = ldc if inplace else M
+            note(f"tsmm form variant {dt} M={M} N={N} K={K} variant={variant} beta={beta} "
+                 f"inplace={inplace}")
+            B = orc.integers(N * M, seed, 1).astype(npdt)
+            C = orc.integers(K * ldc, seed, 2).astype(npdt)
+            A = C.copy() if inplace else orc.integers(K * lda, seed, 0).astype(npdt)
+            want = C.copy()
+            orc.tsmm(M, N, K, want if inplace else A, lda, 2.0, B, M, beta, want, ldc)
+            dC = torch.from_numpy(C).cuda()
+            dA = dC if inplace else torch.from_numpy(A).cuda()
+            dB = torch.from_numpy(B).cuda()
+            fn = gpu.dtsmm if dt == "d" else gpu.stsmm
+            assert fn(0, M, N, K, dA, lda, 2.0, dB, M, beta, dC, ldc, handle=h)
+            assert np.array_equal(host(dC), want), (variant, beta, inplace)
