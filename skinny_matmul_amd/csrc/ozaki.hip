@@ -217,29 +217,63 @@ __device__ __forceinline__ void oz_flush(v16i (&acc)[L][2 * NSUB],
 // sign-replicating selectors (9: bit 31 of the second operand, 11: bit 31
 // of the first).
 //
-// Returns the largest t (t >= 2^(7 min(S, 4)) iff some |x| >= 2^E: a row
-// outgrew the scale) and the largest limb integer q.
-struct OzSplit {
-  double tmax;
-  unsigned qmax;
+// The split of one column against its scale 2^E.  For S <= 4 the limb
+// integer q = min(rint(t), 2^(7S) - 1) of t = |x| 2^(7S - E) comes from one
+// FMA: r = fma(|x|, 2^(7S - E - k), 2^(52 - k)) rounds t + 2^52 (scaled by
+// 2^-k) to nearest even once, and for t < 2^52 the low 32 bits of r's
+// significand are rint(t).  The product is exact inside the FMA, so this is
+// the same q as scaling, rounding and converting separately.  k = 128 for
+// E < -900 keeps 2^(7S - E - k) finite for the smallest scales; 2^(52 - k)
+// stays normal, so the rounding happens at the same bit.  For S = 5 the
+// truncated digits and the rounded remainder are taken from t itself.
+template <int S>
+struct OzScale {
+  static constexpr int S4 = S < 4 ? S : 4;
+  double sc, mg;
+  int E;
+  __device__ __forceinline__ explicit OzScale(int e) : E(e) {
+    const int k = e < -900 ? 128 : 0;
+    sc = ldexp(1.0, 7 * S4 - e - k);
+    mg = ldexp(1.0, 52 - k);
+  }
+  // the limb integer of a magnitude a = |x| (S <= 4: rounded, else truncated)
+  __device__ __forceinline__ unsigned q(double a) const {
+    constexpr unsigned qcap = (1u << (7 * S4)) - 1u;
+    if constexpr (S <= 4)
+      return min((unsigned)__double2loint(__builtin_fma(a, sc, mg)), qcap);
+    else
+      return (unsigned)fmin(ldexp(a, 7 * S4 - E), (double)qcap);
+  }
+  // a row with |x| >= 2^E outgrew the scale
+  __device__ __forceinline__ bool grew(double amax) const {
+    return amax >= ldexp(1.0, E);
+  }
 };
 
+// max(a, |b|) for finite values: one v_max_f64 with the absolute-value
+// modifier.  Written with fmax, IEEE mode adds a canonicalising v_max_f64 for
+// every loaded operand (to quiet a signalling NaN), doubling the count.
+__device__ __forceinline__ double oz_amax(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, |%2|" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+
+// Raises amax to the largest |x| of the four rows: the limb integers and the
+// growth test of a chunk are monotone in |x|, so its largest one decides both.
 template <int S>
-__device__ __forceinline__ OzSplit oz_pack4(const double (&xv)[4], int E,
-                                            int (&w)[S]) {
-  constexpr int S4 = S < 4 ? S : 4;
-  constexpr double qcap = (double)((1u << (7 * S4)) - 1u);
+__device__ __forceinline__ void oz_pack4(const double (&xv)[4],
+                                         const OzScale<S>& sc, int (&w)[S],
+                                         double& amax) {
+  constexpr int S4 = OzScale<S>::S4;
   unsigned q[4], r5[4];
-  double tmax = 0.0;
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    const double t = ldexp(fabs(xv[j]), 7 * S4 - E);
-    tmax = fmax(tmax, t);
-    if (S <= 4) {
-      q[j] = (unsigned)fmin(rint(t), qcap);
+    q[j] = sc.q(fabs(xv[j]));
+    if constexpr (S <= 4) {
       r5[j] = 0u;
     } else {
-      q[j] = (unsigned)fmin(t, qcap);  // truncation
+      const double t = ldexp(fabs(xv[j]), 7 * S4 - sc.E);
       r5[j] = (unsigned)fmin(rint((t - (double)q[j]) * 128.0), 127.0);
     }
   }
@@ -257,7 +291,8 @@ __device__ __forceinline__ OzSplit oz_pack4(const double (&xv)[4], int E,
     const unsigned nb = (0x80808080u - u) ^ 0x80808080u;
     w[s] = (int)((nb & m) | (u & ~m));
   }
-  return OzSplit{tmax, max(max(q[0], q[1]), max(q[2], q[3]))};
+#pragma unroll
+  for (int j = 0; j < 4; j++) amax = oz_amax(amax, xv[j]);
 }
 
 // Calls f(integral_constant<int, I>) for I = 0 .. N - 1 in order: unrolled
@@ -424,7 +459,7 @@ __global__ void __launch_bounds__(kOzThreads)
   };
   // split the chunk in x into S limbs and stage them in LDS buffer b
   auto limbs = [&](int b, const double (&x)[kOzRows]) {
-    const int Es = split_scale();
+    const OzScale<S> Es(split_scale());
 #pragma unroll
     for (int half = 0; half < 2; half++) {
       int words[S][4];
@@ -433,7 +468,8 @@ __global__ void __launch_bounds__(kOzThreads)
         const double xv[4] = {x[16 * half + 4 * q], x[16 * half + 4 * q + 1],
                               x[16 * half + 4 * q + 2], x[16 * half + 4 * q + 3]};
         int w[S];
-        (void)oz_pack4<S>(xv, Es, w);
+        double amax = 0.0;
+        oz_pack4<S>(xv, Es, w, amax);
 #pragma unroll
         for (int s2 = 0; s2 < S; s2++) words[s2][q] = w[s2];
       }
@@ -493,9 +529,8 @@ __global__ void __launch_bounds__(kOzThreads)
     };
     operands(0);
     const __amdgpu_buffer_rsrc_t rn = chunk_rsrc(cnext);
-    const int Es = split_scale();
-    unsigned qmax = 0;
-    double tmax = 0.0;
+    const OzScale<S> Es(split_scale());
+    double amax = 0.0;
     int words[S][4];
     oz_static_for(
         [&](auto U_) {
@@ -503,9 +538,7 @@ __global__ void __launch_bounds__(kOzThreads)
           {
             const double xv[4] = {x[4 * U], x[4 * U + 1], x[4 * U + 2], x[4 * U + 3]};
             int w[S];
-            const OzSplit sp = oz_pack4<S>(xv, Es, w);
-            qmax = max(qmax, sp.qmax);
-            tmax = fmax(tmax, sp.tmax);
+            oz_pack4<S>(xv, Es, w, amax);
 #pragma unroll
             for (int s2 = 0; s2 < S; s2++) words[s2][U % 4] = w[s2];
           }
@@ -534,10 +567,10 @@ __global__ void __launch_bounds__(kOzThreads)
           __builtin_amdgcn_sched_barrier(0);
         },
         std::make_integer_sequence<int, kOzRows / 4>{});
-    constexpr int S4 = S < 4 ? S : 4;
+    constexpr int S4 = OzScale<S>::S4;
+    unsigned qmax = Es.q(amax);
     if (SELF) qmax = max(qmax, (unsigned)__shfl_xor((int)qmax, 32));
-    return tmax >= (double)(1u << (7 * S4)) ||
-           (qmax != 0u && qmax < (1u << (7 * S4 - 9)));
+    return Es.grew(amax) || (qmax != 0u && qmax < (1u << (7 * S4 - 9)));
   };
 
   // Software pipeline: while chunk c is split (VALU) into one LDS buffer,
