@@ -160,14 +160,16 @@ skinny_status_t skinny_ctsmttsm(skinny_handle_t h, size_t blockCount,
                                 skinny_float_complex* C, int64_t ldc,
                                 skinny_stream_t stream);
 
-/* Ozaki split on the INT8 matrix cores (no reference counterpart): FP64
- * inputs split into `slices` (3..5) signed 7-bit limbs (rounded to nearest)
- * against a running per-column power-of-two scale, exact INT32 accumulation
- * per limb level s + t <= slices + 2, FP64 reconstruction when a column's
- * scale grows, when a chunk falls 9+ bits below it, or every 16384 rows.
- * Relative error about 2^(-7 (slices + 1)) of sum |A||B| (3 slices meet
- * 8 K eps on well-scaled Gram data at K = 2^24 and are the fast setting);
- * with blockCount == 1 the computation is one fixed sequence.
+/* Ozaki split on the INT8 matrix cores (no reference counterpart): each
+ * FP64 input is rounded to an integer of 8 slices - 2 bits against a running
+ * per-column power-of-two scale and written in `slices` (3..5) signed-byte
+ * limbs, with exact INT32 accumulation per limb level s + t < L (L = 3 for
+ * 3 slices, slices + 1 for 4 and 5) and FP64 reconstruction when a
+ * column's scale grows, when a chunk falls 9+ bits below it, or every 16384
+ * rows.  Relative error at most about 2^(-(8 slices - 2)) (3 slices: 2^-20)
+ * of sum |A||B|, in practice far below (3 slices meet 8 K eps on the
+ * cancelling-column data at K = 2^24 and are the fast setting); with
+ * blockCount == 1 the computation is one fixed sequence.
  * M, N <= 1024; lda, ldb <= 2^22. */
 skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
                                       int64_t M, int64_t N, int64_t K,

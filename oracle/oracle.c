@@ -755,8 +755,9 @@ void orc_gram_error(int64_t M, int64_t K, const double* X, int64_t ldx,
 /* Algorithm: Ozaki, Ogita, Oishi, Rump, "Error-free                         */
 /* transformations of matrix multiplication by using fast routines of matrix */
 /* multiplication and its applications", Numer. Algorithms 59 (2012), with   */
-/* 7-bit limbs of the rounded integer q = rint(|x| 2^(7S - E)), levels up to  */
-/* s + t = S + 2 and a running per-column scale.                             */
+/* the rounded integer q = rint(x 2^(8S - 2 - E)) written in S signed bytes  */
+/* (balanced base-256 digits), levels s + t < L (L = 3 for 3 limbs, S + 1   */
+/* for 4 and 5) and a running per-column scale.                              */
 /* ------------------------------------------------------------------------ */
 #define OZ_TILE 128
 #define OZ_ROWS 32      /* rows per chunk */
@@ -764,39 +765,50 @@ void orc_gram_error(int64_t M, int64_t K, const double* X, int64_t ldx,
 #define OZ_FLUSH 16384
 #define OZ_EMIN (-4000)
 
-/* L = S + 1 level accumulators per entry, level g = s + t at index g - 2;
- * TN columns of B in the tile */
+/* Level accumulators kept per limb count (levels g = s + t < L). */
+static int oz_levels(int S) { return S == 3 ? 3 : S + 1; }
+
+/* L level accumulators per entry, level g = s + t at index g; TN columns of
+ * B in the tile.  x y = 2^(Ea + Eb - 2 (8S - 2)) sum d_s d'_t
+ * 2^(8 (2S - 2 - s - t)) = 2^(Ea + Eb - 12) sum_g acc_g 2^-8g. */
 static void oz_flush_tile(int L, int TN, int64_t* acc, double* cacc,
                           const int* ea, const int* eb) {
   for (int n = 0; n < TN; n++)
     for (int m = 0; m < OZ_TILE; m++) {
       int64_t* a = acc + ((int64_t)n * OZ_TILE + m) * L;
       if (ea[m] != OZ_EMIN && eb[n] != OZ_EMIN) {
-        double v = ldexp((double)a[L - 1], -7 * (L + 1));
-        for (int g = L; g >= 2; g--) v = v + ldexp((double)a[g - 2], -7 * g);
-        cacc[n * OZ_TILE + m] = cacc[n * OZ_TILE + m] + ldexp(v, ea[m] + eb[n]);
+        double v = ldexp((double)a[L - 1], -8 * (L - 1));
+        for (int g = L - 2; g >= 0; g--) v = v + ldexp((double)a[g], -8 * g);
+        cacc[n * OZ_TILE + m] = cacc[n * OZ_TILE + m] + ldexp(v, ea[m] + eb[n] - 12);
       }
       for (int g = 0; g < L; g++) a[g] = 0;
     }
 }
 
-/* The limb integer of |x| against 2^E: q = min(rint(t), 2^(7 S4) - 1) with
- * t = |x| 2^(7 S4 - E), S4 = min(S, 4) (truncation for S = 5, whose fifth
- * limb is the rounded remainder); the device computes the same. */
-static uint64_t oz_q(double ax, int E, int S, double* t_out) {
-  int S4 = S < 4 ? S : 4;
-  double t = ldexp(ax, 7 * S4 - E);
-  double cap = ldexp(1.0, 7 * S4) - 1.0;
-  if (t_out) *t_out = t;
-  double q = S <= 4 ? fmin(nearbyint(t), cap) : fmin(trunc(t), cap);
-  return (uint64_t)q;
+/* The limb integer of x against 2^E (|x| < 2^E): q = rint(x 2^(8S - 2 - E)),
+ * round to nearest even; |q| <= 2^(8S - 2).  The device computes the same
+ * with one FMA. */
+static int64_t oz_q(double x, int E, int S) {
+  return (int64_t)nearbyint(ldexp(x, 8 * S - 2 - E));
+}
+
+/* Balanced base-256 digits of q, most significant first: the least
+ * significant digit is the low byte of q read as a signed byte, and the rest
+ * are the digits of (q - d) / 256.  For |q| <= 2^(8S - 2) the top digit lies
+ * in [-64, 64]. */
+static void oz_digits(int64_t q, int S, int* d) {
+  for (int s = S - 1; s >= 0; s--) {
+    const int dig = (int)(int8_t)(uint8_t)(q & 0xFF);
+    d[s] = dig;
+    q = (q - dig) / 256;
+  }
+  if (q != 0) abort(); /* |q| beyond S digits: the scale was not respected */
 }
 
 void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
                        int64_t lda, double alpha, const double* B, int64_t ldb,
                        double beta, double* C, int64_t ldc, int S) {
-  const int L = S + 1;
-  const int S4 = S < 4 ? S : 4;
+  const int L = oz_levels(S);
   int64_t* acc = (int64_t*)malloc(sizeof(int64_t) * OZ_TILE * OZ_TILE * L);
   double* cacc = (double*)malloc(sizeof(double) * OZ_TILE * OZ_TILE);
   signed char* da = (signed char*)malloc((size_t)S * OZ_ROWS_SELF * OZ_TILE);
@@ -833,8 +845,8 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
              * the scale restarts from this chunk (an outlier coarsens only
              * its own chunk) */
             if (e <= erun[mat][c] && erun[mat][c] != OZ_EMIN && mx > 0.0) {
-              uint64_t q = oz_q(mx, erun[mat][c], S, NULL);
-              if (q != 0 && q < ((uint64_t)1 << (7 * S4 - 9))) flush = 1;
+              int64_t q = oz_q(mx, erun[mat][c], S);
+              if (q != 0 && q < ((int64_t)1 << (8 * S - 11))) flush = 1;
             }
           }
         if (flush) {
@@ -857,33 +869,27 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
               int64_t r = r0 + i;
               double x = (valid && r < K) ? (mat == 0 ? A[r * lda + gc] : B[r * ldb + gc]) : 0.0;
               int digit[5] = {0, 0, 0, 0, 0};
-              if (E != OZ_EMIN && x != 0.0) {
-                double t;
-                uint64_t q = oz_q(fabs(x), E, S, &t);
-                for (int s2 = 0; s2 < S4; s2++)
-                  digit[s2] = (int)((q >> (7 * (S4 - 1 - s2))) & 127);
-                if (S > 4) digit[4] = (int)fmin(nearbyint((t - (double)q) * 128.0), 127.0);
-              }
+              if (E != OZ_EMIN && x != 0.0) oz_digits(oz_q(x, E, S), S, digit);
               for (int s2 = 0; s2 < S; s2++)
-                d[((size_t)s2 * OZ_TILE + c) * R + i] =
-                    (signed char)(x < 0.0 ? -digit[s2] : digit[s2]);
+                d[((size_t)s2 * OZ_TILE + c) * R + i] = (signed char)digit[s2];
             }
           }
         }
-        /* exact integer products per level g = s + t (1-based limbs) */
-        for (int g = 2; g <= S + 2; g++)
-          for (int s1 = 1; s1 <= S; s1++) {
+        /* exact integer products per level g = s + t (limb 0 the most
+         * significant) */
+        for (int g = 0; g < L; g++)
+          for (int s1 = 0; s1 < S; s1++) {
             int t1 = g - s1;
-            if (t1 < 1 || t1 > S) continue;
-            const signed char* pa = da + (size_t)(s1 - 1) * OZ_TILE * R;
-            const signed char* pb = db + (size_t)(t1 - 1) * OZ_TILE * R;
+            if (t1 < 0 || t1 >= S) continue;
+            const signed char* pa = da + (size_t)s1 * OZ_TILE * R;
+            const signed char* pb = db + (size_t)t1 * OZ_TILE * R;
             for (int n = 0; n < TN; n++)
               for (int m = 0; m < OZ_TILE; m++) {
                 int64_t sum = 0;
                 const signed char* x = pa + (size_t)m * R;
                 const signed char* y = pb + (size_t)n * R;
                 for (int i = 0; i < R; i++) sum += (int64_t)x[i] * y[i];
-                acc[((int64_t)n * OZ_TILE + m) * L + (g - 2)] += sum;
+                acc[((int64_t)n * OZ_TILE + m) * L + g] += sum;
               }
           }
         since += R;

@@ -5,33 +5,32 @@
 // in chunks of 32 (one MFMA k-depth):
 //  * Each column c has a running scale exponent E_c: every |x| since the last
 //    flush is below 2^E_c (after a flush the scale restarts from the next
-//    chunk's largest |x|).  An element x is split into S signed 7-bit limbs:
-//    with t = |x| 2^(7S - E) < 2^(7S) and q = round-to-nearest(t) (capped at
-//    2^(7S) - 1), limb s is sign(x) times bits [7(S - s), 7(S - s) + 7) of q,
-//    so x = 2^E (sum_s d_s 2^-7s + r 2^-7S), |d_s| <= 127, |r| <= 1/2 (the
-//    cap aside); every step is exact in FP64 (Ozaki et al. 2012, error-free
-//    splitting; rounding rather than truncating the last digit removes the
-//    representation's bias toward zero).  For S = 5 the first four limbs
-//    come from trunc(t') with t' = |x| 2^(28 - E) and the fifth is the
-//    rounded remainder, capped at 127.
-//  * For each limb-sum level g = s + t (2 .. S + 2; pairs with s + t > S + 2
-//    are dropped, as in the truncated Ozaki scheme, one level deeper than
-//    the limb count) the products sum_k d_s(k, m) d'_t(k, n) accumulate
-//    exactly in INT32 through v_mfma_i32_32x32x32_i8, one accumulator per
-//    level.
+//    chunk's largest |x|).  An element x is rounded to the integer
+//    q = rint(x 2^(8S - 2 - E)), |q| <= 2^(8S - 2), and q is written in S
+//    signed bytes (limbs), q = sum_s d_s 2^(8 (S - 1 - s)), the lower ones in
+//    [-128, 127] and the top one in [-64, 64]: x = 2^(E - 8S + 2) (q + r),
+//    |r| <= 1/2.  This is the error-free splitting of Ozaki et al. (2012)
+//    with signed-digit limbs: the sign needs no separate handling, and every
+//    limb carries 8 bits.
+//  * For each limb-sum level g = s + t (limb 0 the most significant;
+//    g < 3 for 3 limbs, g <= S for 4 and 5 limbs, the other pairs dropped as
+//    in the truncated Ozaki scheme) the products sum_k d_s(k, m) d'_t(k, n)
+//    accumulate exactly in INT32 through v_mfma_i32_32x32x32_i8, one
+//    accumulator per level.
 //  * The INT32 accumulators are converted to FP64 ("flushed") only when a
 //    column's magnitude grows past its scale, when a chunk's largest limb
 //    integer falls 9 or more bits below it (the chunk after an outlier), or
 //    before INT32 could overflow (every 16384 rows); the flush forms
-//      V = sum_{g = S + 2 .. 2} acc_g 2^-7g   (in this order, FP64),
-//      C_acc(m, n) += V 2^(E_m + E_n).
+//      V = sum_{g = L - 1 .. 0} acc_g 2^-8g   (in this order, FP64),
+//      C_acc(m, n) += V 2^(E_m + E_n - 12)
+//    (x y = 2^(E_m + E_n - 2 (8S - 2)) sum_(s,t) d_s d'_t 2^(8 (2S - 2 - s - t))).
 //    For data of steady magnitude the conversion cost is amortised over
 //    thousands of rows, so the INT8 units and HBM set the pace.
 //  * Workgroups (4 waves, the level accumulators in the matrix-core
 //    registers) own a tile of C and a grid-stride set of chunks; their FP64
 //    partials are summed by the same two-level last-arrival reduction as the
 //    FP64 kernel.  With 3 limbs a tile is 128 x 128 (a 64 x 64 region per
-//    wave, 4 levels x 64 registers); with 4 and 5 limbs it is 128 x 64 (a
+//    wave, 3 levels x 64 registers); with 4 and 5 limbs it is 128 x 64 (a
 //    64 x 32 region per wave, so 5 or 6 levels take 160 or 192 registers
 //    instead of spilling 320 or 384), which reads A once per 64 columns of
 //    B.  With blockCount == 1 the whole computation is one fixed sequence,
@@ -200,12 +199,12 @@ __device__ __forceinline__ void oz_flush(v16i (&acc)[L][2 * NSUB],
         return (int)__builtin_amdgcn_raw_buffer_load_b32(
             stage, lo, 1024 * ((g * TT + tt) * 4 + (r >> 2)), 0);
       };
-      double v = ldexp((double)level(L - 1), -7 * (L + 1));
+      double v = ldexp((double)level(L - 1), -8 * (L - 1));
 #pragma unroll
-      for (int g = L; g >= 2; g--) v = v + ldexp((double)level(g - 2), -7 * g);
+      for (int g = L - 2; g >= 0; g--) v = v + ldexp((double)level(g), -8 * g);
       const int so = 8 * (32 * (tt % NSUB) * kOzTile + 32 * (tt / NSUB) +
                           8 * (r >> 2) + (r & 3));
-      oz_st(oz_ld(part, lofs, so) + ldexp(v, ea + eb), part, lofs, so);
+      oz_st(oz_ld(part, lofs, so) + ldexp(v, ea + eb - 12), part, lofs, so);
     }
   }
 }
@@ -217,36 +216,58 @@ __device__ __forceinline__ void oz_flush(v16i (&acc)[L][2 * NSUB],
 // sign-replicating selectors (9: bit 31 of the second operand, 11: bit 31
 // of the first).
 //
-// The split of one column against its scale 2^E.  For S <= 4 the limb
-// integer q = min(rint(t), 2^(7S) - 1) of t = |x| 2^(7S - E) comes from one
-// FMA: r = fma(|x|, 2^(7S - E - k), 2^(52 - k)) rounds t + 2^52 (scaled by
-// 2^-k) to nearest even once, and for t < 2^52 the low 32 bits of r's
-// significand are rint(t).  The product is exact inside the FMA, so this is
-// the same q as scaling, rounding and converting separately.  k = 128 for
-// E < -900 keeps 2^(7S - E - k) finite for the smallest scales; 2^(52 - k)
-// stays normal, so the rounding happens at the same bit.  For S = 5 the
-// truncated digits and the rounded remainder are taken from t itself.
+// Level accumulators per limb count: products d_s d'_t with
+// g = s + t < oz_levels<S>() are kept (limb 0 the most significant).  3
+// limbs keep 3 levels (6 products per 32 x 32 block and chunk), 4 limbs 5
+// and 5 limbs 6: the products dropped by the fourth and fifth level of 3
+// limbs lie 2^-24 and 2^-32 below the leading one, under the rounding error
+// of the 23-bit limb integer.
+template <int S>
+constexpr int oz_levels() {
+  return S == 3 ? 3 : S + 1;
+}
+
+// The split of one column against its scale 2^E (every |x| < 2^E).  x is
+// rounded to the integer q = rint(x 2^(8S - 2 - E)), |q| <= 2^(8S - 2), and q
+// is written in S signed bytes, q = sum_s d_s 2^(8 (S - 1 - s)):
+// d_(S-1-p) is byte p of q + sum_(i < p) 2^(8 i + 7) in two's complement, so
+// the lower digits lie in [-128, 127] and the top one in [-64, 64].
+// q comes from one FMA: r = fma(x, 2^(8S - 2 - E - k), 1.5 2^(52 - k)) rounds
+// x 2^(8S - 2 - E) + 1.5 2^52 (scaled by 2^-k) to nearest even once, and for
+// |q| < 2^51 the significand of r holds 2^51 + q, whose low 40 bits are q in
+// two's complement.  The product is exact inside the FMA, so q is rint of
+// the exact t.  k = 128 for E < -900 keeps the scale factor finite (1.5
+// 2^(52 - k) stays normal, so the rounding happens at the same bit).
 template <int S>
 struct OzScale {
-  static constexpr int S4 = S < 4 ? S : 4;
   double sc, mg;
   int E;
   __device__ __forceinline__ explicit OzScale(int e) : E(e) {
     const int k = e < -900 ? 128 : 0;
-    sc = ldexp(1.0, 7 * S4 - e - k);
-    mg = ldexp(1.0, 52 - k);
+    sc = ldexp(1.0, 8 * S - 2 - e - k);
+    mg = ldexp(1.5, 52 - k);
   }
-  // the limb integer of a magnitude a = |x| (S <= 4: rounded, else truncated)
-  __device__ __forceinline__ unsigned q(double a) const {
-    constexpr unsigned qcap = (1u << (7 * S4)) - 1u;
+  __device__ __forceinline__ double raw(double x) const {
+    return __builtin_fma(x, sc, mg);
+  }
+  // q of a magnitude a (0 <= a < 2^E, so q <= 2^(8S - 2) < 2^39)
+  __device__ __forceinline__ uint64_t q(double a) const {
+    const double r = raw(a);
+    const uint64_t lo = (unsigned)__double2loint(r);
     if constexpr (S <= 4)
-      return min((unsigned)__double2loint(__builtin_fma(a, sc, mg)), qcap);
+      return lo;
     else
-      return (unsigned)fmin(ldexp(a, 7 * S4 - E), (double)qcap);
+      return lo | ((uint64_t)((unsigned)__double2hiint(r) & 0xFFu) << 32);
   }
   // a row with |x| >= 2^E outgrew the scale
   __device__ __forceinline__ bool grew(double amax) const {
     return amax >= ldexp(1.0, E);
+  }
+  // the chunk's largest limb integer is nonzero but 9 or more bits below the
+  // scale (the chunk after an outlier)
+  __device__ __forceinline__ bool shrank(double amax) const {
+    const uint64_t qm = q(amax);
+    return qm != 0u && qm < ((uint64_t)1 << (8 * S - 11));
   }
 };
 
@@ -259,40 +280,35 @@ __device__ __forceinline__ double oz_amax(double a, double b) {
   return r;
 }
 
-// Raises amax to the largest |x| of the four rows: the limb integers and the
-// growth test of a chunk are monotone in |x|, so its largest one decides both.
+// Limb words of four consecutive rows of a column: w[s] holds digit s of row
+// j in byte j.  Byte p of the four offset integers is gathered with two
+// v_perm (selector p picks byte p of the second operand, 4 + p of the first,
+// 12 a zero byte).  Raises amax to the rows' largest |x|: the limb integers
+// and the growth test of a chunk are monotone in |x|, so its largest one
+// decides both.
 template <int S>
 __device__ __forceinline__ void oz_pack4(const double (&xv)[4],
                                          const OzScale<S>& sc, int (&w)[S],
                                          double& amax) {
-  constexpr int S4 = OzScale<S>::S4;
-  unsigned q[4], r5[4];
+  unsigned u[S][4];  // u[p][j]: the dword of row j whose byte p (mod 4) is digit S - 1 - p
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    q[j] = sc.q(fabs(xv[j]));
-    if constexpr (S <= 4) {
-      r5[j] = 0u;
-    } else {
-      const double t = ldexp(fabs(xv[j]), 7 * S4 - sc.E);
-      r5[j] = (unsigned)fmin(rint((t - (double)q[j]) * 128.0), 127.0);
-    }
-  }
-  const unsigned m =
-      __builtin_amdgcn_perm((unsigned)__double2hiint(xv[1]), (unsigned)__double2hiint(xv[0]), 0x0C0C0B09u) |
-      __builtin_amdgcn_perm((unsigned)__double2hiint(xv[3]), (unsigned)__double2hiint(xv[2]), 0x0B090C0Cu);
+    const double r = sc.raw(xv[j]);
+    const unsigned lo = (unsigned)__double2loint(r);
 #pragma unroll
-  for (int s = 0; s < S; s++) {
-    unsigned u = 0;
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const unsigned d = s < S4 ? (q[j] >> (7 * (S4 - 1 - s))) & 127u : r5[j];
-      u |= d << (8 * j);
+    for (int p = 0; p < S && p < 4; p++) u[p][j] = lo + (0x80808080u & ((1u << (8 * p)) - 1u));
+    if constexpr (S == 5) {  // byte 4: the high dword plus the carry of lo + 0x80808080
+      const unsigned l4 = lo + 0x80808080u;
+      u[4][j] = (unsigned)__double2hiint(r) + (l4 < lo ? 1u : 0u);
     }
-    const unsigned nb = (0x80808080u - u) ^ 0x80808080u;
-    w[s] = (int)((nb & m) | (u & ~m));
+    amax = oz_amax(amax, xv[j]);
   }
 #pragma unroll
-  for (int j = 0; j < 4; j++) amax = oz_amax(amax, xv[j]);
+  for (int p = 0; p < S; p++) {
+    const unsigned bp = p & 3;
+    w[S - 1 - p] = (int)(__builtin_amdgcn_perm(u[p][1], u[p][0], 0x0C0C0000u | ((4u + bp) << 8) | bp) |
+                         __builtin_amdgcn_perm(u[p][3], u[p][2], ((4u + bp) << 24) | (bp << 16) | 0x0C0Cu));
+  }
 }
 
 // Calls f(integral_constant<int, I>) for I = 0 .. N - 1 in order: unrolled
@@ -304,30 +320,26 @@ __device__ __forceinline__ void oz_static_for(F&& f, std::integer_sequence<int, 
 
 // The k-th INT8 product of a chunk (accumulator block tt of TT, level g,
 // limb s of A, limb g - s of B), block-major over the pairs with
-// 2 <= s + t <= S + 2.
+// s + t < oz_levels<S>().
 struct OzMf {
   int tt, g, s;
 };
-template <int S>
-constexpr int oz_levels() {
-  return S + 1;
-}
 template <int S, int TT>
 constexpr int oz_nmfma() {
   int n = 0;
-  for (int g = 2; g <= S + 2; g++)
-    for (int s = 1; s <= S; s++)
-      if (g - s >= 1 && g - s <= S) n++;
+  for (int g = 0; g < oz_levels<S>(); g++)
+    for (int s = 0; s < S; s++)
+      if (g - s >= 0 && g - s < S) n++;
   return TT * n;
 }
 template <int S, int TT>
 constexpr OzMf oz_mfma_of(int k) {
   int n = 0;
   for (int tt = 0; tt < TT; tt++)
-    for (int g = 2; g <= S + 2; g++)
-      for (int s = 1; s <= S; s++) {
+    for (int g = 0; g < oz_levels<S>(); g++)
+      for (int s = 0; s < S; s++) {
         const int t = g - s;
-        if (t < 1 || t > S) continue;
+        if (t < 0 || t >= S) continue;
         if (n == k) return OzMf{tt, g, s};
         n++;
       }
@@ -388,7 +400,7 @@ __global__ void __launch_bounds__(kOzThreads)
   const int voff = 8 * (cvalid ? gcol : 0) + (SELF ? kOzRows * mat * ld8 : 0);
 
   erun[mat][col] = kOzEmin;
-  constexpr int L = oz_levels<S>();  // accumulator levels g = 2 .. S + 2
+  constexpr int L = oz_levels<S>();  // accumulator levels g = 0 .. L - 1
   v16i acc[L][TT];
 #pragma unroll
   for (int g = 0; g < L; g++)
@@ -491,14 +503,14 @@ __global__ void __launch_bounds__(kOzThreads)
         const int mcol = 64 * wm + 32 * (tt / NSUB) + (lane & 31);
         const int ncol = WN * wn + 32 * (tt % NSUB) + (lane & 31);
 #pragma unroll
-        for (int g = 2; g <= S + 2; g++) {
+        for (int g = 0; g < L; g++) {
 #pragma unroll
-          for (int s1 = 1; s1 <= S; s1++) {
+          for (int s1 = 0; s1 < S; s1++) {
             const int t1 = g - s1;
-            if (t1 < 1 || t1 > S) continue;
-            const v4i av = *reinterpret_cast<const v4i*>(&sl[b][SELF ? h : 0][s1 - 1][mcol][kh]);
-            const v4i bv = *reinterpret_cast<const v4i*>(&sl[b][SELF ? h : 1][t1 - 1][ncol][kh]);
-            acc[g - 2][tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc[g - 2][tt], 0, 0, 0);
+            if (t1 < 0 || t1 >= S) continue;
+            const v4i av = *reinterpret_cast<const v4i*>(&sl[b][SELF ? h : 0][s1][mcol][kh]);
+            const v4i bv = *reinterpret_cast<const v4i*>(&sl[b][SELF ? h : 1][t1][ncol][kh]);
+            acc[g][tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc[g][tt], 0, 0, 0);
           }
         }
       }
@@ -551,9 +563,9 @@ __global__ void __launch_bounds__(kOzThreads)
           oz_static_for(
               [&](auto K_) {
                 constexpr OzMf m = oz_mfma_of<S, TT>((k0 + decltype(K_)::value) % NM);
-                acc[m.g - 2][m.tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                    av[m.tt / NSUB][m.s - 1], bv[m.tt % NSUB][m.g - m.s - 1],
-                    acc[m.g - 2][m.tt], 0, 0, 0);
+                acc[m.g][m.tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                    av[m.tt / NSUB][m.s], bv[m.tt % NSUB][m.g - m.s],
+                    acc[m.g][m.tt], 0, 0, 0);
               },
               std::make_integer_sequence<int, k1 - k0>{});
           if constexpr (SELF && U == 3) operands(1);
@@ -567,10 +579,8 @@ __global__ void __launch_bounds__(kOzThreads)
           __builtin_amdgcn_sched_barrier(0);
         },
         std::make_integer_sequence<int, kOzRows / 4>{});
-    constexpr int S4 = OzScale<S>::S4;
-    unsigned qmax = Es.q(amax);
-    if (SELF) qmax = max(qmax, (unsigned)__shfl_xor((int)qmax, 32));
-    return Es.grew(amax) || (qmax != 0u && qmax < (1u << (7 * S4 - 9)));
+    if (SELF) amax = fmax(amax, __shfl_xor(amax, 32));  // both halves share the scale
+    return Es.grew(amax) || Es.shrank(amax);
   };
 
   // Software pipeline: while chunk c is split (VALU) into one LDS buffer,
@@ -738,6 +748,7 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
   // diagonal and read A once (64-row chunks on the diagonal tiles, whose
   // offsets need lda <= 2^21).  4 and 5 limbs: 128 x 64 tiles, all of them.
   const int TN = slices == 3 ? kOzTile : 64;
+  const int levels = slices == 3 ? oz_levels<3>() : slices == 4 ? oz_levels<4>() : oz_levels<5>();
   const bool gram = slices == 3 && A == B && lda == ldb && M == N && lda <= (1 << 21);
   const int tilesM = (int)((M + kOzTile - 1) / kOzTile);
   const int tilesN = (int)((N + TN - 1) / TN);
@@ -779,7 +790,7 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
       return fail(SKINNY_UNSUPPORTED, "ozaki: too many partial groups");
     size_t b = (size_t)ntiles * a.G * PS * sizeof(double);
     if (a.ngroups > 1) b += (size_t)ntiles * a.ngroups * PS * sizeof(double);
-    b += (size_t)ntiles * a.G * (kOzThreads / 64) * (slices + 1) * 1024 * TT * sizeof(int);
+    b += (size_t)ntiles * a.G * (kOzThreads / 64) * levels * 1024 * TT * sizeof(int);
     bytes = std::max(bytes, b);
   }
   Workspace w;

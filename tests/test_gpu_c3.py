@@ -10,8 +10,8 @@ parallel to A's) run on the three paths:
 * HR (double-double accumulation, numeric_tests.cu:58-72 compares double
   with dd on cancelling data): the dd result within 1e-28;
 * Ozaki-INT8 with S = 3, 4, 5 limbs: within the scheme's bound
-  8 (S + 1) 2^(-7 S) and within 8 K eps of the configuration's K = 2^24
-  (1.49e-8; the limb error does not grow with K).
+  (tests/ozaki_bound.py, against sum |a||b|) and within 8 K eps of the
+  configuration's K = 2^24 (1.49e-8; the limb error does not grow with K).
 
 Errors are taken on sampled entries (m, n) against the exact
 (superaccumulator) product, which the host computes per entry.
@@ -20,6 +20,7 @@ import numpy as np
 import pytest
 
 from gpu_util import EPS64, dev, host, note
+from ozaki_bound import ozaki_bound
 
 pytestmark = pytest.mark.gpu
 
@@ -85,9 +86,12 @@ def test_c3_all_paths(gpu, orc, M, case):
     got = pick(host(dC))
     assert np.all(np.abs(got - (hi + lo)) <= np.spacing(np.abs(hi)))
 
+    Ak, Bk = np.abs(A.reshape(K, M)), np.abs(B.reshape(K, M))
+    absprod = np.einsum("km,km->m", Ak[:, ms], Bk[:, ns])
+    cond = float(np.linalg.norm(absprod) / np.linalg.norm(hi))
     for S in (3, 4, 5):
         note(f"c3 {case} M={M} K={K}: ozaki S={S}")
         assert gpu.dtsmttsm_ozaki(0, M, M, K, dA, M, 1.0, dB, M, 0.0, dC, M, slices=S)
         err = _rel(pick(host(dC)), hi, lo)
-        assert err <= 8 * (S + 1) * 2.0 ** (-7 * S), (S, err)
+        assert err <= ozaki_bound(S) * cond, (S, err)
         assert err <= TOL_C3, (S, err)

@@ -17,6 +17,7 @@ import numpy as np
 import pytest
 
 from gpu_util import dev, host, note, rel_fro
+from ozaki_bound import ozaki_bound
 
 pytestmark = pytest.mark.gpu
 
@@ -119,7 +120,9 @@ def test_c3_error(gpu, orc, S):
     assert gpu.dtsmttsm_ozaki(0, M, M, K, dA, M, 1.0, dA, M, 0.0, dC, M, slices=S)
     got = host(dC).reshape(M, M).T  # column-major C: got[m, n]
     err = np.sqrt(np.sum((got[ms, ns] - hi - lo) ** 2)) / np.sqrt(np.sum(hi ** 2))
-    assert err <= 8 * (S + 1) * 2.0 ** (-7 * S)
+    Ak = np.abs(A.reshape(K, M))
+    cond = np.linalg.norm(np.einsum("km,km->m", Ak[:, ms], Ak[:, ns])) / np.linalg.norm(hi)
+    assert err <= ozaki_bound(S) * cond, (S, err, cond)
     assert err <= 8 * 2 ** 24 * np.finfo(float).eps
 
 
