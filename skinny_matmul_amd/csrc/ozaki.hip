@@ -187,41 +187,52 @@ __device__ __forceinline__ void oz_flush(v16i (&acc)[L][2 * NSUB],
 #pragma unroll
       for (int r = 0; r < 16; r++) acc[g][tt][r] = 0;
     }
+  // NB entries at a time: their level and partial loads are issued together
+  // (one entry at a time, each conversion waited for its own loads)
+  constexpr int NB = 8;
 #pragma unroll 1
-  for (int e = 0; e < 16 * TT; e++) {
-    const int tt = e >> 4, r = e & 15;
-    const int dm = 32 * (tt / NSUB) + i8_drow(lane, r);
-    const int dn = 32 * (tt % NSUB) + (lane & 31);
-    const int ea = ea_w[dm], eb = eb_w[dn];
-    if (ea != kOzEmin && eb != kOzEmin) {
-      const int lo = 16 * lane + 4 * (r & 3);
-      auto level = [&](int g) {
-        return (int)__builtin_amdgcn_raw_buffer_load_b32(
-            stage, lo, 1024 * ((g * TT + tt) * 4 + (r >> 2)), 0);
-      };
-      double v = ldexp((double)level(L - 1), -8 * (L - 1));
+  for (int e0 = 0; e0 < 16 * TT; e0 += NB) {
+    int lv[NB][L];
+    double pv[NB];
+    int so[NB];
+    bool live[NB];
 #pragma unroll
-      for (int g = L - 2; g >= 0; g--) v = v + ldexp((double)level(g), -8 * g);
-      const int so = 8 * (32 * (tt % NSUB) * kOzTile + 32 * (tt / NSUB) +
-                          8 * (r >> 2) + (r & 3));
-      oz_st(oz_ld(part, lofs, so) + ldexp(v, ea + eb - 12), part, lofs, so);
+    for (int i = 0; i < NB; i++) {
+      const int e = e0 + i;
+      const int tt = e >> 4, r = e & 15;
+      const int dm = 32 * (tt / NSUB) + i8_drow(lane, r);
+      const int dn = 32 * (tt % NSUB) + (lane & 31);
+      live[i] = ea_w[dm] != kOzEmin && eb_w[dn] != kOzEmin;
+      const int lo = 16 * lane + 4 * (r & 3);
+#pragma unroll
+      for (int g = 0; g < L; g++)
+        lv[i][g] = (int)__builtin_amdgcn_raw_buffer_load_b32(
+            stage, lo, 1024 * ((g * TT + tt) * 4 + (r >> 2)), 0);
+      so[i] = 8 * (32 * (tt % NSUB) * kOzTile + 32 * (tt / NSUB) + 8 * (r >> 2) + (r & 3));
+      pv[i] = oz_ld(part, lofs, so[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < NB; i++) {
+      const int e = e0 + i;
+      const int tt = e >> 4, r = e & 15;
+      const int dm = 32 * (tt / NSUB) + i8_drow(lane, r);
+      const int dn = 32 * (tt % NSUB) + (lane & 31);
+      if (live[i]) {
+        double v = ldexp((double)lv[i][L - 1], -8 * (L - 1));
+#pragma unroll
+        for (int g = L - 2; g >= 0; g--) v = v + ldexp((double)lv[i][g], -8 * g);
+        oz_st(pv[i] + ldexp(v, ea_w[dm] + eb_w[dn] - 12), part, lofs, so[i]);
+      }
     }
   }
 }
 
-// Limb words of four consecutive rows of a column: w[s] holds limb s + 1 of
-// row j in byte j (two's complement).  The bytes are built unsigned and
-// negated together: for a byte u <= 127, ((0x80 - u) ^ 0x80) == -u mod 256
-// with no borrow into the next byte; the sign bytes come from v_perm's
-// sign-replicating selectors (9: bit 31 of the second operand, 11: bit 31
-// of the first).
-//
 // Level accumulators per limb count: products d_s d'_t with
 // g = s + t < oz_levels<S>() are kept (limb 0 the most significant).  3
 // limbs keep 3 levels (6 products per 32 x 32 block and chunk), 4 limbs 5
-// and 5 limbs 6: the products dropped by the fourth and fifth level of 3
-// limbs lie 2^-24 and 2^-32 below the leading one, under the rounding error
-// of the 23-bit limb integer.
+// and 5 limbs 6.  For 3 limbs the dropped levels 3 and 4 are at most 2^-21
+// and 2^-30 of the product of the two column scales, the order of the
+// rounding of the two 23-bit limb integers (2^-22).
 template <int S>
 constexpr int oz_levels() {
   return S == 3 ? 3 : S + 1;
@@ -363,6 +374,7 @@ __global__ void __launch_bounds__(kOzThreads)
   __shared__ __attribute__((aligned(16))) signed char sl[2][2][S][kOzTile][kOzRows + 16];
   __shared__ int erun[2][kOzTile];
   __shared__ int s_last;
+  __shared__ int vote[3];  // rescale votes, one slot per step in rotation
   __shared__ double tree[2 * kOzThreads];
 
   const int tid = threadIdx.x;
@@ -400,6 +412,7 @@ __global__ void __launch_bounds__(kOzThreads)
   const int voff = 8 * (cvalid ? gcol : 0) + (SELF ? kOzRows * mat * ld8 : 0);
 
   erun[mat][col] = kOzEmin;
+  if (tid < 3) vote[tid] = 0;
   constexpr int L = oz_levels<S>();  // accumulator levels g = 0 .. L - 1
   v16i acc[L][TT];
 #pragma unroll
@@ -591,6 +604,23 @@ __global__ void __launch_bounds__(kOzThreads)
   // since the last flush.  Growth shows up while c is being split: the
   // previous chunk has been multiplied by then, and c is loaded and split
   // again after the flush.
+  // Workgroup-wide any(p) with one barrier: a wave ballot, then one lane of
+  // each voting wave sets this call's LDS slot.  The slots rotate over
+  // three: call n clears call n + 1's slot before its own barrier (the votes
+  // for n + 1 come after it), and call n's slot is cleared again only in
+  // call n + 2, after every thread has passed call n + 1's barrier and so
+  // has read it.  (__syncthreads_or computes the workgroup size from the
+  // dispatch packet and takes three barriers.)
+  int vslot = 0;
+  auto wg_any = [&](bool p) {
+    const int now = vslot;
+    vslot = vslot == 2 ? 0 : vslot + 1;
+    if (tid == 0) vote[vslot] = 0;
+    if (__builtin_amdgcn_ballot_w64(p) != 0 && lane == 0) vote[now] = 1;
+    __syncthreads();
+    return vote[now] != 0;
+  };
+
   int64_t ch = kp;
   int pb = 0;
   while (ch < nch) {
@@ -614,21 +644,20 @@ __global__ void __launch_bounds__(kOzThreads)
       oz_pin<L, TT>(acc);
       if (since + CR > kOzFlushRows) return 1;  // uniform
       const bool rescale = split_multiply(pb, pb ^ 1, x, min(ch + D * a.G, nch - 1));
-      if (__syncthreads_or(rescale)) return 2;
+      if (wg_any(rescale)) return 2;
       since += CR;
       pb ^= 1;
       ch += a.G;
       return 0;
     };
-    // xb holds chunk ch and xa chunk ch + G, the two alternate
+    // xb holds chunk ch and xa chunk ch + G, the two alternate; each step
+    // leaves the loop on its own
+    int r;
     for (;;) {
-      int r = step(xb);
-      if (r == 0) r = step(xa);
-      if (r != 0) {
-        staged = r == 1;
-        break;
-      }
+      if ((r = step(xb)) != 0) break;
+      if ((r = step(xa)) != 0) break;
     }
+    staged = r == 1;
     // drain: multiply the staged chunk, convert the accumulators
     if (staged) multiply(pb);
     oz_flush<L, NSUB>(acc, stage, part, lofs, ea_w, eb_w, lane);
