@@ -362,7 +362,13 @@ constexpr OzMf oz_mfma_of(int k) {
 // TN: the tile's width in columns of B (128, or 64 for 4 and 5 limbs, whose
 // S + 1 accumulator levels for a 64 x 64 region per wave would not fit the
 // 256 matrix-core registers: a 64 x 32 region per wave halves them).
-template <int S, bool SELF, int TN>
+// SHARE: the launch has several tiles, which read the same rows (tiles in
+// one row of the tile grid share A's, in one column B's).  The tiles of one
+// chunk set kp then run on one XCD (workgroups go to the XCDs round robin,
+// so workgroup w sits on XCD w mod 8), and rows are loaded without the
+// nontemporal hint, so that the tiles after the first find them in that
+// XCD's L2.
+template <int S, bool SELF, int TN, bool SHARE = false>
 __global__ void __launch_bounds__(kOzThreads)
     __attribute__((amdgpu_waves_per_eu(1, 1))) ozaki_kernel(OzArgs a) {
   static_assert(TN == 128 ? S == 3 : (TN == 64 && !SELF), "tile width");
@@ -383,8 +389,17 @@ __global__ void __launch_bounds__(kOzThreads)
   // this wave's 64 x (TN / 2) region: rows 64 wm.., columns (TN / 2) wn..
   const int wm = wave >> 1, wn = wave & 1;
   const int ntiles = a.ntiles;
-  const int tile = blockIdx.x % ntiles;
-  const int kp = blockIdx.x / ntiles;
+  // chunk set kp of tile `tile`; SHARE with G a multiple of 8: workgroups
+  // w = 8 j + x take tile j mod ntiles of chunk set x + 8 (j / ntiles)
+  int tile, kp;
+  if (SHARE && (a.G & 7) == 0) {
+    const int x = blockIdx.x & 7, j = blockIdx.x >> 3;
+    tile = j % ntiles;
+    kp = x + 8 * (j / ntiles);
+  } else {
+    tile = blockIdx.x % ntiles;
+    kp = blockIdx.x / ntiles;
+  }
   int tm, tn;
   oz_tile_of(a, tile, tm, tn);
   const int m0 = tm * kOzTile;
@@ -462,7 +477,8 @@ __global__ void __launch_bounds__(kOzThreads)
   auto load_chunk = [&](int64_t c, double (&dst)[kOzRows]) {
     const __amdgpu_buffer_rsrc_t rs = chunk_rsrc(c);
 #pragma unroll
-    for (int i = 0; i < kOzRows; i++) dst[i] = oz_ld_stream(rs, voff, i * ld8);
+    for (int i = 0; i < kOzRows; i++)
+      dst[i] = SHARE ? oz_ld(rs, voff, i * ld8) : oz_ld_stream(rs, voff, i * ld8);
   };
   // frexp exponent of the chunk's largest |x| (max |x| < 2^e), kOzEmin for
   // an all-zero chunk or a padding column
@@ -569,7 +585,9 @@ __global__ void __launch_bounds__(kOzThreads)
           }
           // these 4 rows are split: their registers take chunk cnext
 #pragma unroll
-          for (int j = 0; j < 4; j++) x[4 * U + j] = oz_ld_stream(rn, voff, (4 * U + j) * ld8);
+          for (int j = 0; j < 4; j++)
+            x[4 * U + j] = SHARE ? oz_ld(rn, voff, (4 * U + j) * ld8)
+                                 : oz_ld_stream(rn, voff, (4 * U + j) * ld8);
           constexpr int NM = oz_nmfma<S, TT>();
           constexpr int NT = SELF ? 2 * NM : NM;  // products per chunk
           constexpr int k0 = U * NT / 8, k1 = (U + 1) * NT / 8;
@@ -851,13 +869,20 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
     a.stage = reinterpret_cast<int*>(static_cast<char*>(ws_scratch(w)) + part_bytes);
     a.cnt = ws_counters(w);
     const int blocks = a.G * ntiles;
+    const dim3 grid(blocks), wg(kOzThreads);
     if (sets[i].self) {
-      hipLaunchKernelGGL((ozaki_kernel<3, true, 128>), dim3(blocks), dim3(kOzThreads), 0, s, a);
+      hipLaunchKernelGGL((ozaki_kernel<3, true, 128>), grid, wg, 0, s, a);
+    } else if (ntiles > 1) {  // tiles sharing rows: same XCD, cached loads
+      switch (slices) {
+        case 3: hipLaunchKernelGGL((ozaki_kernel<3, false, 128, true>), grid, wg, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((ozaki_kernel<4, false, 64, true>), grid, wg, 0, s, a); break;
+        default: hipLaunchKernelGGL((ozaki_kernel<5, false, 64, true>), grid, wg, 0, s, a); break;
+      }
     } else {
       switch (slices) {
-        case 3: hipLaunchKernelGGL((ozaki_kernel<3, false, 128>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
-        case 4: hipLaunchKernelGGL((ozaki_kernel<4, false, 64>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
-        default: hipLaunchKernelGGL((ozaki_kernel<5, false, 64>), dim3(blocks), dim3(kOzThreads), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((ozaki_kernel<3, false, 128>), grid, wg, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((ozaki_kernel<4, false, 64>), grid, wg, 0, s, a); break;
+        default: hipLaunchKernelGGL((ozaki_kernel<5, false, 64>), grid, wg, 0, s, a); break;
       }
     }
     hipError_t e = hipGetLastError();

@@ -10,6 +10,8 @@
   blockCount = 1 (ragged K, beta != 0, several tiles, an outlier);
 * default blockCount: error against the exact product within the scheme's
   bound and within 8 K eps at K = 2^24 (1.5e-8) from 3 limbs on;
+* launches of several tiles (placed by chunk set on one XCD): each tile
+  bitwise equal to the one-tile product of its column slices;
 * operands above 2 GiB (4 GiB each, every limb count, Gram too): exact on
   small-integer data, against the FP64 kernel and host int64 dot products.
 """
@@ -104,6 +106,35 @@ def test_bitwise_gram_outlier(gpu, orc):
         dC = dev(np.zeros(M * M))
         assert gpu.dtsmttsm_ozaki(1, M, M, K, dA, M, 1.0, dA, M, 0.0, dC, M, slices=S)
         assert np.array_equal(host(dC), want), S
+
+
+@pytest.mark.parametrize("M,N,S", [(256, 256, 3), (128, 128, 4), (256, 192, 5)])
+def test_shared_tiles_match_single_tiles(gpu, orc, M, N, S):
+    """Several tiles of one launch share rows: their workgroups are placed by
+    chunk set on one XCD and load without the nontemporal hint.  Placement
+    does not change what a tile computes (the same chunk sets, partials and
+    reduction order), so each tile of the whole product is bitwise the
+    product of its own column slices, computed alone at the same
+    blockCount (a one-tile launch, placed as before).  blockCount 16 takes
+    the XCD placement; 12 does not."""
+    import torch
+    K = 40000
+    TN = 128 if S == 3 else 64
+    A = torch.from_numpy(orc.uniform(K * M, 31, 0)).cuda()
+    B = torch.from_numpy(orc.uniform(K * N, 31, 1)).cuda()
+    for bc in (16, 12):
+        note(f"ozaki shared tiles M={M} N={N} S={S} blockCount={bc}")
+        full = torch.zeros(M * N, dtype=torch.float64, device="cuda")
+        assert gpu.dtsmttsm_ozaki(bc, M, N, K, A, M, 1.0, B, N, 0.0, full, M, slices=S)
+        full_h = host(full).reshape(N, M)  # column-major: [n, m]
+        for m0 in range(0, M, 128):
+            for n0 in range(0, N, TN):
+                mt, nt = min(128, M - m0), min(TN, N - n0)
+                part = torch.zeros(mt * nt, dtype=torch.float64, device="cuda")
+                assert gpu.dtsmttsm_ozaki(bc, mt, nt, K, A[m0:], M, 1.0, B[n0:], N, 0.0,
+                                          part, mt, slices=S)
+                got = host(part).reshape(nt, mt)
+                assert np.array_equal(full_h[n0:n0 + nt, m0:m0 + mt], got), (bc, m0, n0)
 
 
 @pytest.mark.parametrize("S", [3, 4, 5])
