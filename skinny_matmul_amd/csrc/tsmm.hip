@@ -278,6 +278,69 @@ __global__ void __launch_bounds__(256) tsmm_kernel(TmArgs<T> a) {
   }
 }
 
+// Outer product C = alpha a b^T + beta C (M = 1, FP64) into a packed C
+// (ldc == N, 16-byte aligned).  A wave owns 64 consecutive rows, which are
+// 64 N consecutive values of C, and lane l computes the element pairs
+// 2 (l + 64 s): every store of the wave is one contiguous 1 KiB, and so is
+// every load of old C.  Whole-row threads stored 16 bytes per lane at a
+// stride of one row, so that each store of a wave touched 64 cache lines.
+// The wave's 64 values of a and the N values of b are read through LDS.
+// Each value is fma(a, b, 0) and the same epilogue as the other kernels,
+// so results are bitwise theirs.  In place (A == C, lda == N) a wave reads
+// its rows' first values before it stores any of them.
+template <bool BETA0>
+__global__ void __launch_bounds__(256) tsmm_outer_kernel(TmArgs<double> a) {
+  __shared__ double sa[4][64];
+  __shared__ double sb[64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int N = a.N;
+  for (int n = threadIdx.x; n < N; n += 256) sb[n] = a.B[(int64_t)n * a.ldb];
+  __syncthreads();
+  const int64_t nblk = (a.K + 63) / 64;
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  // pair p + 64 of a lane is element 2 p + 128: 128 / N rows and 128 % N
+  // columns further on
+  const int drow = 128 / N, dcol = 128 % N;
+  const int row_l = (2 * lane) / N, col_l = (2 * lane) % N;
+  for (int64_t blk = (int64_t)blockIdx.x * 4 + w; blk < nblk; blk += nw) {
+    const int64_t r0 = blk * 64;
+    const int rows = (int)min<int64_t>(64, a.K - r0);
+    sa[w][lane] = lane < rows ? a.A[(r0 + lane) * a.lda] : 0.0;
+    // the wave's LDS accesses are done in order: every lane's value is
+    // written before any lane reads another's
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+    const int nel = rows * N;
+    double* cb = a.C + r0 * N;
+    int row = row_l, col = col_l;
+    for (int e = 2 * lane; e < nel; e += 128) {
+      const int row1 = col + 1 == N ? row + 1 : row;
+      const int col1 = col + 1 == N ? 0 : col + 1;
+      const double v0 = fma_t(sa[w][row], sb[col], 0.0);
+      if (e + 1 < nel) {
+        const double v1 = fma_t(sa[w][row1], sb[col1], 0.0);
+        typedef double d2 __attribute__((ext_vector_type(2)));
+        d2* cp = reinterpret_cast<d2*>(cb + e);
+        const d2 c = BETA0 ? d2{0.0, 0.0} : *cp;
+        *cp = d2{axpby_t(v0, c.x, a.alpha, a.beta, BETA0), axpby_t(v1, c.y, a.alpha, a.beta, BETA0)};
+      } else {
+        const double c = BETA0 ? 0.0 : cb[e];
+        cb[e] = axpby_t(v0, c, a.alpha, a.beta, BETA0);
+      }
+      col += dcol;
+      row += drow;
+      if (col >= N) {
+        col -= N;
+        row++;
+      }
+    }
+    // the next block's a values replace this one's only after every lane
+    // has read them
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+  }
+}
+
 // ---------------------------------------------------------------------------
 // matrix-core TSMM
 // ---------------------------------------------------------------------------
@@ -587,6 +650,31 @@ skinny_status_t launch_tsmm_real(skinny_handle_t h, size_t blockCount,
   if (blockCount == 0 && tune_lookup(h, 1, sizeof(T) == 8 ? TT_DR : TT_FR, M, N, K, &te) &&
       (te.uvar & 1))
     rows = !rows;
+  // M = 1 into a packed, 16-byte aligned C: one contiguous block per wave
+  // (tsmm_outer_kernel), where it measured faster than the other kernels
+  // (K = 2^25, same box): out of place for N = 3..16 with beta = 0 (N = 3:
+  // 3033 -> 4945 GB/s, N = 5: 3415 -> 5083, N = 11: 3913 -> 4795) and
+  // N = 4..16 otherwise (N = 4: 3537 -> 5300), in place with beta = 0 for
+  // N = 2..3 (3424 -> 3995) and with beta != 0 for N = 3..8 (N = 4: 1866 ->
+  // 3198).  Outside these ranges the other kernels were as fast or up to 9%
+  // faster.
+  if constexpr (sizeof(T) == 8) {
+    const bool outer_wins = inplace ? (beta0 ? N >= 2 && N <= 3 : N >= 3 && N <= 8)
+                                    : (beta0 ? N >= 3 : N >= 4) && N <= 16;
+    if (M == 1 && outer_wins && ldc == N && (uintptr_t)C % 16 == 0 &&
+        (!inplace || lda == ldc)) {
+      const int ob = blockCount > 0 ? (int)blockCount
+                                    : (int)std::min<int64_t>(std::max<int64_t>(1, (K + 255) / 256),
+                                                             4 * (int64_t)h->cu_count);
+      if (beta0)
+        hipLaunchKernelGGL((tsmm_outer_kernel<true>), dim3(ob), dim3(256), 0, stream, a);
+      else
+        hipLaunchKernelGGL((tsmm_outer_kernel<false>), dim3(ob), dim3(256), 0, stream, a);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return hip_fail(e, "tsmm outer launch");
+      return SKINNY_OK;
+    }
+  }
   if (!rows) {
     if (!beta0 && (const void*)A == (const void*)C && lda == ldc && N <= M) {
       a.fold = 1;
