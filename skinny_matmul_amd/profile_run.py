@@ -38,6 +38,7 @@ def main() -> int:
                     required=True)
     ap.add_argument("--S", type=int, default=4, help="Ozaki limb count")
     ap.add_argument("--M", type=int, default=64)
+    ap.add_argument("--N", type=int, help="TSMM: width of C (default M)")
     ap.add_argument("--logK", type=int, default=26)
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--gram", action="store_true", help="TSMTTSM/Ozaki/HR: B is A")
@@ -66,13 +67,18 @@ def main() -> int:
     Bs = torch.empty(M * M, dtype=torch.float64, device="cuda")
     sk.fill_uniform(Bs, M * M, 1, 2)
     Cs = torch.zeros(M * M, dtype=torch.float64, device="cuda")
-    Cw = B  # TSMM output (K x M)
+    N = a.N or M
+    if a.op == "tsmm" and N != M:
+        Bs = torch.empty(M * N, dtype=torch.float64, device="cuda")
+        sk.fill_uniform(Bs, M * N, 1, 2)
+        B = torch.empty(K * N, dtype=torch.float64, device="cuda")
+    Cw = B  # TSMM output (K x N)
     torch.cuda.synchronize()
     if a.gram:
         B = A
     for _ in range(a.reps):
         if a.op == "tsmm":
-            ok = sk.dtsmm(0, M, M, K, A, M, 1.0, Bs, M, 0.0, Cw, M)
+            ok = sk.dtsmm(0, M, N, K, A, M, 1.0, Bs, M, 0.0, Cw, N)
         elif a.op == "ozaki":
             ok = sk.dtsmttsm_ozaki(0, M, M, K, A, M, 1.0, B, M, 0.0, Cs, M, slices=a.S)
         elif a.op == "hr":
@@ -81,7 +87,7 @@ def main() -> int:
             ok = sk.dtsmttsm(0, M, M, K, A, M, 1.0, B, M, 0.0, Cs, M)
         assert ok
     torch.cuda.synchronize()
-    print(f"{a.op} M=N={M} K=2^{a.logK}: {a.reps} calls done")
+    print(f"{a.op} M={M} N={N} K=2^{a.logK}: {a.reps} calls done")
     return 0
 
 
