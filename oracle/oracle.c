@@ -751,7 +751,9 @@ void orc_gram_error(int64_t M, int64_t K, const double* X, int64_t ldx,
 /* for 4 and 5 (a flush is decided over the tile's columns, so the width    */
 /* shapes the result).  Chunks are 32 rows, or 64 rows for the diagonal     */
 /* tiles of a Gram product with 3 limbs (A == B, lda == ldb, M == N,        */
-/* lda <= 2^21), where both operands share one scale per column.            */
+/* lda <= 2^21), where both operands share one scale per column, and for    */
+/* every other product with M, N <= 64 and lda, ldb <= 2^21 (one 128 x 64   */
+/* tile).                                                                   */
 /* Algorithm: Ozaki, Ogita, Oishi, Rump, "Error-free                         */
 /* transformations of matrix multiplication by using fast routines of matrix */
 /* multiplication and its applications", Numer. Algorithms 59 (2012), with   */
@@ -814,10 +816,14 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
   signed char* da = (signed char*)malloc((size_t)S * OZ_ROWS_SELF * OZ_TILE);
   signed char* db = (signed char*)malloc((size_t)S * OZ_ROWS_SELF * OZ_TILE);
   const int gram = A == B && lda == ldb && M == N && lda <= ((int64_t)1 << 21);
-  const int TN = S == 3 ? OZ_TILE : 64; /* the tile's columns of B */
+  /* other products with M, N <= 64: one tile of 64-row chunks (the device
+   * lays A and B side by side as the columns of one Gram tile) */
+  const int cat = !(gram && S == 3) && M <= 64 && N <= 64 && lda <= ((int64_t)1 << 21) &&
+                  ldb <= ((int64_t)1 << 21);
+  const int TN = S == 3 && !cat ? OZ_TILE : 64; /* the tile's columns of B */
   for (int64_t m0 = 0; m0 < M; m0 += OZ_TILE)
     for (int64_t n0 = 0; n0 < N; n0 += TN) {
-      const int R = (gram && m0 == n0 && S == 3) ? OZ_ROWS_SELF : OZ_ROWS;
+      const int R = (gram && m0 == n0 && S == 3) || cat ? OZ_ROWS_SELF : OZ_ROWS;
       int erun[2][OZ_TILE], ecur[2][OZ_TILE];
       for (int c = 0; c < OZ_TILE; c++) erun[0][c] = erun[1][c] = OZ_EMIN;
       memset(acc, 0, sizeof(int64_t) * OZ_TILE * OZ_TILE * L);

@@ -158,7 +158,7 @@ __device__ __forceinline__ void oz_pin(v16i (&acc)[L][TT]) {
 }
 
 // Convert the INT32 level accumulators to FP64 and add them to this wave's
-// 64 x (32 NSUB) region of the workgroup's partial tile in global memory
+// region of the workgroup's partial tile in global memory
 // (entry (m, n) at n * 128 + m, owned by exactly one lane), then clear them.
 // Accumulator tt holds the 32 x 32 block (tt / NSUB, tt % NSUB) of the
 // region.  The
@@ -166,13 +166,12 @@ __device__ __forceinline__ void oz_pin(v16i (&acc)[L][TT]) {
 // staging area and are converted from there one entry at a time, so the
 // conversion holds no copy of them.  `part` is the resource of the region's
 // first entry, `lofs` the lane's byte offset in it.
-template <int L, int NSUB>
-__device__ __forceinline__ void oz_flush(v16i (&acc)[L][2 * NSUB],
+template <int L, int TT, int NSUB>
+__device__ __forceinline__ void oz_flush(v16i (&acc)[L][TT],
                                          __amdgpu_buffer_rsrc_t stage,
                                          __amdgpu_buffer_rsrc_t part, int lofs,
                                          const int* ea_w, const int* eb_w,
                                          int lane) {
-  constexpr int TT = 2 * NSUB;
 #pragma unroll
   for (int g = 0; g < L; g++)
 #pragma unroll
@@ -368,12 +367,21 @@ constexpr OzMf oz_mfma_of(int k) {
 // so workgroup w sits on XCD w mod 8), and rows are loaded without the
 // nontemporal hint, so that the tiles after the first find them in that
 // XCD's L2.
-template <int S, bool SELF, int TN, bool SHARE = false>
+// CAT: a general product with M, N <= 64 in one 64 x 64 tile, laid out as the
+// diagonal tile of the Gram product of [A B]: wave w streams columns 32 w ..
+// 32 w + 31 of [A B] (A's columns 0..63, B's 64..127) over 64-row chunks as
+// SELF does, and computes the 32 x 32 block (w & 1, w >> 1) of C.  A chunk
+// carries 128 columns that C uses, against 64 + 64 of the 128 + TN that a
+// general tile loads at these widths.
+template <int S, bool SELF, int TN, bool SHARE = false, bool CAT = false>
 __global__ void __launch_bounds__(kOzThreads)
     __attribute__((amdgpu_waves_per_eu(1, 1))) ozaki_kernel(OzArgs a) {
-  static_assert(TN == 128 ? S == 3 : (TN == 64 && !SELF), "tile width");
-  constexpr int NSUB = TN / 64;  // 32-column blocks per wave
-  constexpr int TT = 2 * NSUB;   // 32 x 32 accumulator blocks per wave
+  static_assert(CAT ? (TN == 64 && !SELF && !SHARE)
+                    : (TN == 128 ? S == 3 : (TN == 64 && !SELF)), "tile width");
+  constexpr bool HALVES = SELF || CAT;  // 64-row chunks, two 32-row halves
+  constexpr int NSUB = CAT ? 1 : TN / 64;  // 32-column blocks per wave
+  constexpr int TT = CAT ? 1 : 2 * NSUB;   // 32 x 32 accumulator blocks per wave
+  constexpr int NAV = TT / NSUB;           // 32-column blocks of A per wave
   // limbs of two chunks (double buffer): [buffer][A/B or half][limb][column][row];
   // a column's 32 bytes sit in a 48-byte row, so the 16-byte accesses of 16
   // consecutive lanes (32-byte strides otherwise) fall on distinct banks
@@ -411,20 +419,20 @@ __global__ void __launch_bounds__(kOzThreads)
   // SELF: wave w streams columns 32 w .. 32 w + 31, lanes 0-31 the chunk's
   // first 32 rows and lanes 32-63 its last 32 (mat = the half), so the two
   // halves of a column meet in one wave.
-  constexpr int CR = SELF ? 2 * kOzRows : kOzRows;  // rows per chunk
-  const int mat = SELF ? lane >> 5 : wave >> 1;
-  const int col = SELF ? 32 * wave + (lane & 31) : tid & 127;
-  const bool isA = SELF || mat == 0;
+  constexpr int CR = HALVES ? 2 * kOzRows : kOzRows;  // rows per chunk
+  const int mat = HALVES ? lane >> 5 : wave >> 1;
+  const int col = HALVES ? 32 * wave + (lane & 31) : tid & 127;
+  const bool isA = SELF || (CAT ? wave < 2 : mat == 0);
   const int width = isA ? a.M : a.N;
-  const int gcol = (isA ? m0 : n0) + col;
+  const int gcol = CAT ? (isA ? col : col - 64) : (isA ? m0 : n0) + col;
   // a column of the tile (for TN = 64, waves 0-2; stated through the wave
   // index so that the compiler sees a wave-uniform load resource)
-  const bool cload = TN == 128 || isA || (wave & 1) == 0;
+  const bool cload = CAT || TN == 128 || isA || (wave & 1) == 0;
   const bool cvalid = cload && gcol < width;
   const double* mbase = isA ? a.A : a.B;
   const int64_t ld = isA ? a.lda : a.ldb;
   const int ld8 = (int)(8 * ld);
-  const int voff = 8 * (cvalid ? gcol : 0) + (SELF ? kOzRows * mat * ld8 : 0);
+  const int voff = 8 * (cvalid ? gcol : 0) + (HALVES ? kOzRows * mat * ld8 : 0);
 
   erun[mat][col] = kOzEmin;
   if (tid < 3) vote[tid] = 0;
@@ -438,11 +446,14 @@ __global__ void __launch_bounds__(kOzThreads)
       for (int r = 0; r < 16; r++) acc[g][tt][r] = 0;
   // this workgroup's FP64 partial tile, accumulated by the (rare) flushes
   constexpr int PS = kOzTile * TN;
-  constexpr int WN = TN / 2;  // the wave region's width
+  // the wave's region of C: rows (columns of A) from AM, columns (of B)
+  // from BN; its first staged column of A (AM) and of B (BS)
+  const int AM = CAT ? 32 * (wave & 1) : 64 * wm;
+  const int BN = CAT ? 32 * (wave >> 1) : (TN / 2) * wn;
+  const int BS = CAT ? 64 + BN : BN;
   double* mine = a.part1 + ((int64_t)tile * a.G + kp) * PS;
   const __amdgpu_buffer_rsrc_t part =
-      oz_rsrc(mine + WN * wn * kOzTile + 64 * wm,
-              8 * (int64_t)(PS - WN * wn * kOzTile - 64 * wm));
+      oz_rsrc(mine + BN * kOzTile + AM, 8 * (int64_t)(PS - BN * kOzTile - AM));
   const int lofs = 8 * ((lane & 31) * kOzTile + 4 * (lane >> 5));
 #pragma unroll
   for (int tt = 0; tt < TT; tt++)
@@ -453,8 +464,8 @@ __global__ void __launch_bounds__(kOzThreads)
   const __amdgpu_buffer_rsrc_t stage = oz_rsrc(
       a.stage + ((int64_t)blockIdx.x * (kOzThreads / 64) + wave) * (L * TT * 1024),
       (int64_t)L * TT * 1024 * 4);
-  const int* ea_w = &erun[0][64 * wm];
-  const int* eb_w = &erun[1][WN * wn];
+  const int* ea_w = &erun[0][AM];
+  const int* eb_w = &erun[1][BS];
   __syncthreads();
 
   const int64_t nch = (a.K + CR - 1) / CR;
@@ -488,7 +499,7 @@ __global__ void __launch_bounds__(kOzThreads)
     for (int i = 0; i < kOzRows; i++) mx = fmax(mx, fabs(x[i]));
     int e = kOzEmin;
     if (cvalid && mx > 0.0) (void)frexp(mx, &e);
-    if (SELF) e = max(e, __shfl_xor(e, 32));  // both halves of the column
+    if (HALVES) e = max(e, __shfl_xor(e, 32));  // both halves of the column
     return e;
   };
   // the scale the limbs are taken against: 2^E for a live column; an
@@ -526,19 +537,19 @@ __global__ void __launch_bounds__(kOzThreads)
   const int kh = 16 * (lane >> 5);
   auto multiply = [&](int b) {
 #pragma unroll
-    for (int h = 0; h < (SELF ? 2 : 1); h++)  // SELF: both halves of the chunk
+    for (int h = 0; h < (HALVES ? 2 : 1); h++)  // both halves of the chunk
 #pragma unroll
       for (int tt = 0; tt < TT; tt++) {
-        const int mcol = 64 * wm + 32 * (tt / NSUB) + (lane & 31);
-        const int ncol = WN * wn + 32 * (tt % NSUB) + (lane & 31);
+        const int mcol = AM + 32 * (tt / NSUB) + (lane & 31);
+        const int ncol = BS + 32 * (tt % NSUB) + (lane & 31);
 #pragma unroll
         for (int g = 0; g < L; g++) {
 #pragma unroll
           for (int s1 = 0; s1 < S; s1++) {
             const int t1 = g - s1;
             if (t1 < 0 || t1 >= S) continue;
-            const v4i av = *reinterpret_cast<const v4i*>(&sl[b][SELF ? h : 0][s1][mcol][kh]);
-            const v4i bv = *reinterpret_cast<const v4i*>(&sl[b][SELF ? h : 1][t1][ncol][kh]);
+            const v4i av = *reinterpret_cast<const v4i*>(&sl[b][HALVES ? h : 0][s1][mcol][kh]);
+            const v4i bv = *reinterpret_cast<const v4i*>(&sl[b][HALVES ? h : 1][t1][ncol][kh]);
             acc[g][tt] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bv, acc[g][tt], 0, 0, 0);
           }
         }
@@ -556,16 +567,16 @@ __global__ void __launch_bounds__(kOzThreads)
   // second half's operands are read at the end of unit 3 and multiplied in
   // units 4-7.
   auto split_multiply = [&](int bm, int bs, double (&x)[kOzRows], int64_t cnext) {
-    v4i av[2][S], bv[NSUB][S];
+    v4i av[NAV][S], bv[NSUB][S];
     auto operands = [&](int h2) {
 #pragma unroll
       for (int s2 = 0; s2 < S; s2++) {
 #pragma unroll
-        for (int h = 0; h < 2; h++)
-          av[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][SELF ? h2 : 0][s2][64 * wm + 32 * h + (lane & 31)][kh]);
+        for (int h = 0; h < NAV; h++)
+          av[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][HALVES ? h2 : 0][s2][AM + 32 * h + (lane & 31)][kh]);
 #pragma unroll
         for (int h = 0; h < NSUB; h++)
-          bv[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][SELF ? h2 : 1][s2][WN * wn + 32 * h + (lane & 31)][kh]);
+          bv[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][HALVES ? h2 : 1][s2][BS + 32 * h + (lane & 31)][kh]);
       }
     };
     operands(0);
@@ -589,7 +600,7 @@ __global__ void __launch_bounds__(kOzThreads)
             x[4 * U + j] = SHARE ? oz_ld(rn, voff, (4 * U + j) * ld8)
                                  : oz_ld_stream(rn, voff, (4 * U + j) * ld8);
           constexpr int NM = oz_nmfma<S, TT>();
-          constexpr int NT = SELF ? 2 * NM : NM;  // products per chunk
+          constexpr int NT = HALVES ? 2 * NM : NM;  // products per chunk
           constexpr int k0 = U * NT / 8, k1 = (U + 1) * NT / 8;
           oz_static_for(
               [&](auto K_) {
@@ -599,7 +610,7 @@ __global__ void __launch_bounds__(kOzThreads)
                     acc[m.g][m.tt], 0, 0, 0);
               },
               std::make_integer_sequence<int, k1 - k0>{});
-          if constexpr (SELF && U == 3) operands(1);
+          if constexpr (HALVES && U == 3) operands(1);
           if constexpr (U % 4 == 3) {
 #pragma unroll
             for (int s2 = 0; s2 < S; s2++) {
@@ -610,7 +621,7 @@ __global__ void __launch_bounds__(kOzThreads)
           __builtin_amdgcn_sched_barrier(0);
         },
         std::make_integer_sequence<int, kOzRows / 4>{});
-    if (SELF) amax = fmax(amax, __shfl_xor(amax, 32));  // both halves share the scale
+    if (HALVES) amax = fmax(amax, __shfl_xor(amax, 32));  // both halves share the scale
     return Es.grew(amax) || Es.shrank(amax);
   };
 
@@ -678,7 +689,7 @@ __global__ void __launch_bounds__(kOzThreads)
     staged = r == 1;
     // drain: multiply the staged chunk, convert the accumulators
     if (staged) multiply(pb);
-    oz_flush<L, NSUB>(acc, stage, part, lofs, ea_w, eb_w, lane);
+    oz_flush<L, TT, NSUB>(acc, stage, part, lofs, ea_w, eb_w, lane);
     __syncthreads();
   }
 
@@ -794,9 +805,12 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
   // 3 limbs: 128 x 128 tiles.  Gram products skip the tiles below the
   // diagonal and read A once (64-row chunks on the diagonal tiles, whose
   // offsets need lda <= 2^21).  4 and 5 limbs: 128 x 64 tiles, all of them.
-  const int TN = slices == 3 ? kOzTile : 64;
+  // Other products with M, N <= 64: one tile on [A B] (64-row chunks, so
+  // lda, ldb <= 2^21 as well).
   const int levels = slices == 3 ? oz_levels<3>() : slices == 4 ? oz_levels<4>() : oz_levels<5>();
   const bool gram = slices == 3 && A == B && lda == ldb && M == N && lda <= (1 << 21);
+  const bool cat = !gram && M <= 64 && N <= 64 && lda <= (1 << 21) && ldb <= (1 << 21);
+  const int TN = slices == 3 && !cat ? kOzTile : 64;
   const int tilesM = (int)((M + kOzTile - 1) / kOzTile);
   const int tilesN = (int)((N + TN - 1) / TN);
   struct Set {
@@ -813,12 +827,12 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
     sets[nsets++] = Set{OZ_ALL, tilesM * tilesN, false, {}};
   }
   const size_t PS = (size_t)kOzTile * TN;
-  const int TT = 2 * (TN / 64);  // accumulator blocks per wave
+  const int TT = cat ? 1 : 2 * (TN / 64);  // accumulator blocks per wave
   size_t bytes = 0;  // the launches run one after the other: the larger one
   for (int i = 0; i < nsets; i++) {
     OzArgs& a = sets[i].a;
     const int ntiles = sets[i].ntiles;
-    const int64_t rows = sets[i].self ? 2 * kOzRows : kOzRows;
+    const int64_t rows = sets[i].self || cat ? 2 * kOzRows : kOzRows;
     int64_t G = (int64_t)blockCount;
     if (G <= 0) {
       G = std::max<int64_t>(1, h->cu_count / ntiles);
@@ -872,6 +886,12 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
     const dim3 grid(blocks), wg(kOzThreads);
     if (sets[i].self) {
       hipLaunchKernelGGL((ozaki_kernel<3, true, 128>), grid, wg, 0, s, a);
+    } else if (cat) {
+      switch (slices) {
+        case 3: hipLaunchKernelGGL((ozaki_kernel<3, false, 64, false, true>), grid, wg, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((ozaki_kernel<4, false, 64, false, true>), grid, wg, 0, s, a); break;
+        default: hipLaunchKernelGGL((ozaki_kernel<5, false, 64, false, true>), grid, wg, 0, s, a); break;
+      }
     } else if (ntiles > 1) {  // tiles sharing rows: same XCD, cached loads
       switch (slices) {
         case 3: hipLaunchKernelGGL((ozaki_kernel<3, false, 128, true>), grid, wg, 0, s, a); break;

@@ -33,7 +33,9 @@ def test_i8_layout(gpu):
 
 
 @pytest.mark.parametrize("M,N,K,S", [(5, 3, 1000, 4), (32, 32, 3001, 3), (128, 128, 2000, 4),
-                                     (130, 70, 999, 5), (1, 1, 100, 4)])
+                                     (130, 70, 999, 5), (1, 1, 100, 4), (64, 64, 5000, 3),
+                                     (64, 17, 3001, 5), (33, 64, 777, 4), (65, 64, 2000, 3),
+                                     (64, 65, 1500, 4)])
 def test_bitwise_blockcount1(gpu, orc, M, N, K, S):
     note(f"ozaki bitwise M={M} N={N} K={K} S={S}")
     A = orc.uniform(K * M, M + N, 0)
