@@ -753,7 +753,8 @@ void orc_gram_error(int64_t M, int64_t K, const double* X, int64_t ldx,
 /* tiles of a Gram product with 3 limbs (A == B, lda == ldb, M == N,        */
 /* lda <= 2^21), where both operands share one scale per column, and for    */
 /* every other product with M, N <= 64 and lda, ldb <= 2^21 (one 128 x 64   */
-/* tile).                                                                   */
+/* tile).  Gram products with M <= 64 take 128-row chunks whose two 64-row  */
+/* blocks share each column's scale, tested for shrinking block by block.   */
 /* Algorithm: Ozaki, Ogita, Oishi, Rump, "Error-free                         */
 /* transformations of matrix multiplication by using fast routines of matrix */
 /* multiplication and its applications", Numer. Algorithms 59 (2012), with   */
@@ -813,17 +814,23 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
   const int L = oz_levels(S);
   int64_t* acc = (int64_t*)malloc(sizeof(int64_t) * OZ_TILE * OZ_TILE * L);
   double* cacc = (double*)malloc(sizeof(double) * OZ_TILE * OZ_TILE);
-  signed char* da = (signed char*)malloc((size_t)S * OZ_ROWS_SELF * OZ_TILE);
-  signed char* db = (signed char*)malloc((size_t)S * OZ_ROWS_SELF * OZ_TILE);
+  signed char* da = (signed char*)malloc((size_t)S * 2 * OZ_ROWS_SELF * OZ_TILE);
+  signed char* db = (signed char*)malloc((size_t)S * 2 * OZ_ROWS_SELF * OZ_TILE);
   const int gram = A == B && lda == ldb && M == N && lda <= ((int64_t)1 << 21);
+  /* Gram products with M <= 64: one tile of 128-row chunks whose two 64-row
+   * blocks share each column's scale and are tested for shrinking block by
+   * block (the device lays the blocks side by side) */
+  const int gcat = gram && M <= 64;
   /* other products with M, N <= 64: one tile of 64-row chunks (the device
    * lays A and B side by side as the columns of one Gram tile) */
-  const int cat = !(gram && S == 3) && M <= 64 && N <= 64 && lda <= ((int64_t)1 << 21) &&
-                  ldb <= ((int64_t)1 << 21);
+  const int cat = gcat || (!(gram && S == 3) && M <= 64 && N <= 64 &&
+                           lda <= ((int64_t)1 << 21) && ldb <= ((int64_t)1 << 21));
   const int TN = S == 3 && !cat ? OZ_TILE : 64; /* the tile's columns of B */
   for (int64_t m0 = 0; m0 < M; m0 += OZ_TILE)
     for (int64_t n0 = 0; n0 < N; n0 += TN) {
-      const int R = (gram && m0 == n0 && S == 3) || cat ? OZ_ROWS_SELF : OZ_ROWS;
+      const int R = gcat ? 2 * OZ_ROWS_SELF
+                    : (gram && m0 == n0 && S == 3) || cat ? OZ_ROWS_SELF : OZ_ROWS;
+      const int RB = gcat ? OZ_ROWS_SELF : R; /* rows per scale test */
       int erun[2][OZ_TILE], ecur[2][OZ_TILE];
       for (int c = 0; c < OZ_TILE; c++) erun[0][c] = erun[1][c] = OZ_EMIN;
       memset(acc, 0, sizeof(int64_t) * OZ_TILE * OZ_TILE * L);
@@ -835,24 +842,27 @@ void orc_ozaki_tsmttsm(int64_t M, int64_t N, int64_t K, const double* A,
           for (int c = 0; c < (mat == 0 ? OZ_TILE : TN); c++) {
             int64_t gc = (mat == 0 ? m0 : n0) + c;
             int valid = gc < (mat == 0 ? M : N);
-            double mx = 0.0;
-            for (int i = 0; i < R && valid; i++) {
-              int64_t r = r0 + i;
-              if (r >= K) break;
-              double x = mat == 0 ? A[r * lda + gc] : B[r * ldb + gc];
-              if (fabs(x) > mx) mx = fabs(x);
-            }
-            int e = OZ_EMIN;
-            if (mx > 0.0) frexp(mx, &e);
-            ecur[mat][c] = e;
-            if (e > erun[mat][c]) flush = 1; /* growth */
-            /* shrink: the chunk's largest limb integer q (oz_q of its
-             * largest |x|) is nonzero but 9 or more bits below the scale, so
-             * the scale restarts from this chunk (an outlier coarsens only
-             * its own chunk) */
-            if (e <= erun[mat][c] && erun[mat][c] != OZ_EMIN && mx > 0.0) {
-              int64_t q = oz_q(mx, erun[mat][c], S);
-              if (q != 0 && q < ((int64_t)1 << (8 * S - 11))) flush = 1;
+            ecur[mat][c] = OZ_EMIN;
+            for (int b0 = 0; b0 < R; b0 += RB) {
+              double mx = 0.0;
+              for (int i = b0; i < b0 + RB && valid; i++) {
+                int64_t r = r0 + i;
+                if (r >= K) break;
+                double x = mat == 0 ? A[r * lda + gc] : B[r * ldb + gc];
+                if (fabs(x) > mx) mx = fabs(x);
+              }
+              int e = OZ_EMIN;
+              if (mx > 0.0) frexp(mx, &e);
+              if (e > ecur[mat][c]) ecur[mat][c] = e;
+              if (e > erun[mat][c]) flush = 1; /* growth */
+              /* shrink: the block's largest limb integer q (oz_q of its
+               * largest |x|) is nonzero but 9 or more bits below the scale,
+               * so the scale restarts from this chunk (an outlier coarsens
+               * only its own chunk) */
+              if (e <= erun[mat][c] && erun[mat][c] != OZ_EMIN && mx > 0.0) {
+                int64_t q = oz_q(mx, erun[mat][c], S);
+                if (q != 0 && q < ((int64_t)1 << (8 * S - 11))) flush = 1;
+              }
             }
           }
         if (flush) {

@@ -373,12 +373,19 @@ constexpr OzMf oz_mfma_of(int k) {
 // SELF does, and computes the 32 x 32 block (w & 1, w >> 1) of C.  A chunk
 // carries 128 columns that C uses, against 64 + 64 of the 128 + TN that a
 // general tile loads at these widths.
+// SELF and CAT: a Gram product with M <= 64 in one 64 x 64 tile: columns
+// 64..127 are A's columns again, 64 rows further down (128-row chunks, two
+// 64-row blocks), and the two copies of a column share one scale.  Wave w
+// computes the 32 x 32 block (w & 1, w >> 1) of C from both blocks.
 template <int S, bool SELF, int TN, bool SHARE = false, bool CAT = false>
 __global__ void __launch_bounds__(kOzThreads)
     __attribute__((amdgpu_waves_per_eu(1, 1))) ozaki_kernel(OzArgs a) {
-  static_assert(CAT ? (TN == 64 && !SELF && !SHARE)
+  static_assert(CAT ? (TN == 64 && !SHARE)
                     : (TN == 128 ? S == 3 : (TN == 64 && !SELF)), "tile width");
-  constexpr bool HALVES = SELF || CAT;  // 64-row chunks, two 32-row halves
+  constexpr bool HALVES = SELF || CAT;  // 64-row blocks, two 32-row halves
+  constexpr bool GCAT = SELF && CAT;    // two 64-row blocks per chunk
+  // operand sets per chunk: (half, block) pairs, each multiplied in turn
+  constexpr int NPART = GCAT ? 4 : HALVES ? 2 : 1;
   constexpr int NSUB = CAT ? 1 : TN / 64;  // 32-column blocks per wave
   constexpr int TT = CAT ? 1 : 2 * NSUB;   // 32 x 32 accumulator blocks per wave
   constexpr int NAV = TT / NSUB;           // 32-column blocks of A per wave
@@ -419,12 +426,12 @@ __global__ void __launch_bounds__(kOzThreads)
   // SELF: wave w streams columns 32 w .. 32 w + 31, lanes 0-31 the chunk's
   // first 32 rows and lanes 32-63 its last 32 (mat = the half), so the two
   // halves of a column meet in one wave.
-  constexpr int CR = HALVES ? 2 * kOzRows : kOzRows;  // rows per chunk
+  constexpr int CR = NPART * kOzRows;  // rows per chunk
   const int mat = HALVES ? lane >> 5 : wave >> 1;
   const int col = HALVES ? 32 * wave + (lane & 31) : tid & 127;
   const bool isA = SELF || (CAT ? wave < 2 : mat == 0);
   const int width = isA ? a.M : a.N;
-  const int gcol = CAT ? (isA ? col : col - 64) : (isA ? m0 : n0) + col;
+  const int gcol = CAT ? (GCAT ? col & 63 : isA ? col : col - 64) : (isA ? m0 : n0) + col;
   // a column of the tile (for TN = 64, waves 0-2; stated through the wave
   // index so that the compiler sees a wave-uniform load resource)
   const bool cload = CAT || TN == 128 || isA || (wave & 1) == 0;
@@ -432,7 +439,8 @@ __global__ void __launch_bounds__(kOzThreads)
   const double* mbase = isA ? a.A : a.B;
   const int64_t ld = isA ? a.lda : a.ldb;
   const int ld8 = (int)(8 * ld);
-  const int voff = 8 * (cvalid ? gcol : 0) + (HALVES ? kOzRows * mat * ld8 : 0);
+  const int voff = 8 * (cvalid ? gcol : 0) + (HALVES ? kOzRows * mat * ld8 : 0) +
+                   (GCAT && wave >= 2 ? 2 * kOzRows * ld8 : 0);
 
   erun[mat][col] = kOzEmin;
   if (tid < 3) vote[tid] = 0;
@@ -450,7 +458,7 @@ __global__ void __launch_bounds__(kOzThreads)
   // from BN; its first staged column of A (AM) and of B (BS)
   const int AM = CAT ? 32 * (wave & 1) : 64 * wm;
   const int BN = CAT ? 32 * (wave >> 1) : (TN / 2) * wn;
-  const int BS = CAT ? 64 + BN : BN;
+  const int BS = CAT && !SELF ? 64 + BN : BN;
   double* mine = a.part1 + ((int64_t)tile * a.G + kp) * PS;
   const __amdgpu_buffer_rsrc_t part =
       oz_rsrc(mine + BN * kOzTile + AM, 8 * (int64_t)(PS - BN * kOzTile - AM));
@@ -537,11 +545,12 @@ __global__ void __launch_bounds__(kOzThreads)
   const int kh = 16 * (lane >> 5);
   auto multiply = [&](int b) {
 #pragma unroll
-    for (int h = 0; h < (HALVES ? 2 : 1); h++)  // both halves of the chunk
+    for (int p = 0; p < NPART; p++)  // halves (p & 1) and blocks (p >> 1) of the chunk
 #pragma unroll
       for (int tt = 0; tt < TT; tt++) {
-        const int mcol = AM + 32 * (tt / NSUB) + (lane & 31);
-        const int ncol = BS + 32 * (tt % NSUB) + (lane & 31);
+        const int h = p & 1;
+        const int mcol = 64 * (p >> 1) + AM + 32 * (tt / NSUB) + (lane & 31);
+        const int ncol = 64 * (p >> 1) + BS + 32 * (tt % NSUB) + (lane & 31);
 #pragma unroll
         for (int g = 0; g < L; g++) {
 #pragma unroll
@@ -563,20 +572,23 @@ __global__ void __launch_bounds__(kOzThreads)
   // nonzero if the column's scale must restart: a value outgrew it, or the
   // chunk's largest limb integer is nonzero but 9 or more bits below it (the
   // chunk after an outlier).
-  // SELF: the products of the chunk's first half run in units 0-3, its
-  // second half's operands are read at the end of unit 3 and multiplied in
-  // units 4-7.
+  // With several operand sets (SELF and CAT: the chunk's two halves; both
+  // with two blocks: four sets) set p's products run in units p * 8 / NPART
+  // onwards, and set p + 1's operands are read at the end of set p's last
+  // unit.
   auto split_multiply = [&](int bm, int bs, double (&x)[kOzRows], int64_t cnext) {
     v4i av[NAV][S], bv[NSUB][S];
-    auto operands = [&](int h2) {
+    // operand set p: half p & 1, block p >> 1
+    auto operands = [&](int p) {
+      const int h2 = p & 1, c0 = 64 * (p >> 1);
 #pragma unroll
       for (int s2 = 0; s2 < S; s2++) {
 #pragma unroll
         for (int h = 0; h < NAV; h++)
-          av[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][HALVES ? h2 : 0][s2][AM + 32 * h + (lane & 31)][kh]);
+          av[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][HALVES ? h2 : 0][s2][c0 + AM + 32 * h + (lane & 31)][kh]);
 #pragma unroll
         for (int h = 0; h < NSUB; h++)
-          bv[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][HALVES ? h2 : 1][s2][BS + 32 * h + (lane & 31)][kh]);
+          bv[h][s2] = *reinterpret_cast<const v4i*>(&sl[bm][HALVES ? h2 : 1][s2][c0 + BS + 32 * h + (lane & 31)][kh]);
       }
     };
     operands(0);
@@ -600,7 +612,7 @@ __global__ void __launch_bounds__(kOzThreads)
             x[4 * U + j] = SHARE ? oz_ld(rn, voff, (4 * U + j) * ld8)
                                  : oz_ld_stream(rn, voff, (4 * U + j) * ld8);
           constexpr int NM = oz_nmfma<S, TT>();
-          constexpr int NT = HALVES ? 2 * NM : NM;  // products per chunk
+          constexpr int NT = NPART * NM;  // products per chunk
           constexpr int k0 = U * NT / 8, k1 = (U + 1) * NT / 8;
           oz_static_for(
               [&](auto K_) {
@@ -610,7 +622,8 @@ __global__ void __launch_bounds__(kOzThreads)
                     acc[m.g][m.tt], 0, 0, 0);
               },
               std::make_integer_sequence<int, k1 - k0>{});
-          if constexpr (HALVES && U == 3) operands(1);
+          constexpr int UP = 8 / NPART;  // units per operand set
+          if constexpr (NPART > 1 && U % UP == UP - 1 && U < 7) operands((U + 1) / UP);
           if constexpr (U % 4 == 3) {
 #pragma unroll
             for (int s2 = 0; s2 < S; s2++) {
@@ -659,6 +672,12 @@ __global__ void __launch_bounds__(kOzThreads)
     load_chunk(ch, xa);
     erun[mat][col] = chunk_exp(xa);
     __syncthreads();
+    if constexpr (GCAT) {  // both blocks of a column share the scale
+      const int e2 = max(erun[mat][col], erun[mat][col ^ 64]);
+      __syncthreads();
+      erun[mat][col] = e2;
+      __syncthreads();
+    }
     load_chunk(min(ch + a.G, nch - 1), xb);
     limbs(pb, xa);
     load_chunk(min(ch + D * a.G, nch - 1), xa);
@@ -806,10 +825,13 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
   // diagonal and read A once (64-row chunks on the diagonal tiles, whose
   // offsets need lda <= 2^21).  4 and 5 limbs: 128 x 64 tiles, all of them.
   // Other products with M, N <= 64: one tile on [A B] (64-row chunks, so
-  // lda, ldb <= 2^21 as well).
+  // lda, ldb <= 2^21 as well); Gram products with M <= 64 (any limb count):
+  // one tile on two 64-row blocks of A (128-row chunks).
   const int levels = slices == 3 ? oz_levels<3>() : slices == 4 ? oz_levels<4>() : oz_levels<5>();
-  const bool gram = slices == 3 && A == B && lda == ldb && M == N && lda <= (1 << 21);
-  const bool cat = !gram && M <= 64 && N <= 64 && lda <= (1 << 21) && ldb <= (1 << 21);
+  const bool same = A == B && lda == ldb && M == N && lda <= (1 << 21);
+  const bool gcat = same && M <= 64;
+  const bool gram = slices == 3 && same && !gcat;
+  const bool cat = gcat || (!gram && M <= 64 && N <= 64 && lda <= (1 << 21) && ldb <= (1 << 21));
   const int TN = slices == 3 && !cat ? kOzTile : 64;
   const int tilesM = (int)((M + kOzTile - 1) / kOzTile);
   const int tilesN = (int)((N + TN - 1) / TN);
@@ -832,7 +854,7 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
   for (int i = 0; i < nsets; i++) {
     OzArgs& a = sets[i].a;
     const int ntiles = sets[i].ntiles;
-    const int64_t rows = sets[i].self || cat ? 2 * kOzRows : kOzRows;
+    const int64_t rows = gcat ? 4 * kOzRows : sets[i].self || cat ? 2 * kOzRows : kOzRows;
     int64_t G = (int64_t)blockCount;
     if (G <= 0) {
       G = std::max<int64_t>(1, h->cu_count / ntiles);
@@ -886,6 +908,12 @@ skinny_status_t skinny_dtsmttsm_ozaki(skinny_handle_t h, size_t blockCount,
     const dim3 grid(blocks), wg(kOzThreads);
     if (sets[i].self) {
       hipLaunchKernelGGL((ozaki_kernel<3, true, 128>), grid, wg, 0, s, a);
+    } else if (gcat) {
+      switch (slices) {
+        case 3: hipLaunchKernelGGL((ozaki_kernel<3, true, 64, false, true>), grid, wg, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((ozaki_kernel<4, true, 64, false, true>), grid, wg, 0, s, a); break;
+        default: hipLaunchKernelGGL((ozaki_kernel<5, true, 64, false, true>), grid, wg, 0, s, a); break;
+      }
     } else if (cat) {
       switch (slices) {
         case 3: hipLaunchKernelGGL((ozaki_kernel<3, false, 64, false, true>), grid, wg, 0, s, a); break;

@@ -79,7 +79,9 @@ def test_bitwise_outlier(gpu, orc):
 
 
 @pytest.mark.parametrize("M,K,S", [(5, 999, 3), (128, 3001, 3), (128, 40, 4), (128, 20, 3),
-                                   (200, 2000, 4), (300, 130, 5), (130, 7000, 3)])
+                                   (200, 2000, 4), (300, 130, 5), (130, 7000, 3),
+                                   (64, 3001, 4), (37, 1000, 5), (64, 40000, 3), (65, 2000, 3),
+                                   (1, 300, 3)])
 def test_bitwise_gram(gpu, orc, M, K, S):
     note(f"ozaki gram bitwise M={M} K={K} S={S}")
     A = orc.uniform(K * M, 2 * M, 0)
@@ -93,15 +95,17 @@ def test_bitwise_gram(gpu, orc, M, K, S):
 
 
 def test_bitwise_gram_outlier(gpu, orc):
-    """Outliers in one 32-row half of a 64-row chunk: the column's shared
-    scale, the growth flush and the restart, bitwise against the oracle."""
+    """Outliers in one 32-row half of a 64-row block: the column's shared
+    scale, the growth flush and the restart, bitwise against the oracle
+    (M = 40: two 64-row blocks per chunk; row 100 is in the second block of
+    chunk 0)."""
     M, K = 40, 6000
     note(f"ozaki gram outlier M={M} K={K}")
     A = orc.uniform(K * M, 23, 0).copy()
     A[100 * M + 3] = 1e5      # row 100: second half of chunk 1
     A[2000 * M + 7] = -3e7    # row 2000: first half of chunk 31
     A[4033 * M + 7] = 1e-6    # tiny rows do not trigger anything
-    for S in (3, 4):
+    for S in (3, 4, 5):
         want = np.zeros(M * M)
         orc.ozaki_tsmttsm(M, M, K, A, M, 1.0, A, M, 0.0, want, M, S)
         dA = dev(A)
