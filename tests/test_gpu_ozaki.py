@@ -48,6 +48,35 @@ def test_bitwise_blockcount1(gpu, orc, M, N, K, S):
     assert np.array_equal(host(dC), want)
 
 
+@pytest.mark.parametrize("M,N,S,gram", [(48, 40, 3, False), (48, 48, 5, True), (64, 64, 4, True),
+                                         (20, 64, 5, False)])
+def test_bitwise_strided_small(gpu, orc, M, N, S, gram):
+    """The small-width tiles ([A B] and two row blocks of A) with leading
+    dimensions beyond the widths, bitwise against the oracle; the padding
+    columns hold NaN, which any read of them would carry into C."""
+    K = 3001
+    lda, ldb = M + 3, (M + 3 if gram else N + 5)
+    note(f"ozaki strided M={M} N={N} S={S} gram={gram}")
+    A = orc.uniform(K * lda, 51, 0).reshape(K, lda).copy()
+    A[:, M:] = np.nan
+    if gram:
+        B = A
+    else:
+        B = orc.uniform(K * ldb, 51, 1).reshape(K, ldb).copy()
+        B[:, N:] = np.nan
+    A, B = A.ravel(), B.ravel()
+    C0 = orc.uniform(M * N, 51, 2)
+    want = C0.copy()
+    orc.ozaki_tsmttsm(M, N, K, A, lda, 1.5, A if gram else B, ldb, -1.0, want, M, S)
+    dA = dev(A)
+    dB = dA if gram else dev(B)
+    dC = dev(C0.copy())
+    assert gpu.dtsmttsm_ozaki(1, M, N, K, dA, lda, 1.5, dB, ldb, -1.0, dC, M, slices=S)
+    got = host(dC)
+    assert np.all(np.isfinite(got))
+    assert np.array_equal(got, want)
+
+
 def test_bitwise_growth(gpu, orc):
     M, N, K = 16, 9, 40000
     A = orc.uniform(K * M, 9, 0)
